@@ -1,0 +1,184 @@
+"""Benchmark of the VOF timestep on one MI355X.
+
+Workload (one "step" = one full time step of the hot path): the 200^3 FP32
+dam break in a 0.1 m cube, liquid block x<0.04, y<0.04, z<0.06 m, rho 1000/50,
+sigma 0.007 (unused by the reference physics), g = (0,0,-9.81), dt = 4e-6 s,
+10 Jacobi sweeps per step.  Fields are resident in HBM before the timed
+region; the timed region replays one hipGraph per step.
+
+Prints ONE JSON line.  `value` is interior cell-updates per second over the
+timed steps (wall clock between two stream synchronisations).  `roofline`
+describes the kernel with the largest share of step time, from HIP events
+recorded live on the solver's stream.  `cpu_baseline` times the CPU oracle
+(the reference's kernel bodies as OpenMP loops) on a bounded sample of the
+same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cuda-lbm-ibm-fvm_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "cell-updates/s per timestep at 200³ + % HBM peak; batched & adjoint vs plain"
+
+# Algorithmic HBM bytes per interior cell for each stage (FP32; x2 for FP64):
+# every array the stage must read or write, touched once.
+STAGE_BYTES = {
+    "predictor": 28,   # read u,v,w,F; write u*,v*,w*
+    "rhs": 20,         # read u*,v*,w*,F; write rhs
+    "jacobi": 12,      # per sweep: read p, rhs; write p'
+    "correction": 32,  # read u*,v*,w*,p,F; write u,v,w
+    "fct_x": 12,       # read F, u; write F
+    "fct_y": 12,
+    "fct_z": 12,
+}
+LAUNCHES_PER_STEP = {"predictor": 1, "rhs": 1, "jacobi": 10, "correction": 1, "fct_x": 1, "fct_y": 1, "fct_z": 1}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--n", type=int, default=200)
+    ap.add_argument("--precision", type=int, default=4)
+    ap.add_argument("--eager", action="store_true", help="launch kernels per step instead of graph replay")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--stage-steps", type=int, default=30)
+    return ap.parse_args()
+
+
+def make_solver(vof, n, prec):
+    prm = vof.default_params(n=n, precision=prec, g=(0.0, 0.0, -9.81))
+    s = vof.Solver(prm)
+    s.init_box(0.04, 0.04, 0.06)
+    return s
+
+
+def cpu_baseline(n, prec, budget_s):
+    """Time the oracle on the same workload: one warm-up step, then as many
+    steps as fit the budget (at least one)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import vof
+
+    o = oracle.Oracle(vof.default_params(n=n, precision=prec, g=(0.0, 0.0, -9.81)))
+    o.init_box(0.04, 0.04, 0.06)
+    t0 = time.perf_counter()
+    o.step(1, 1)
+    t1 = time.perf_counter()
+    per = max(t1 - t0, 1e-6)
+    k = max(1, min(50, int(budget_s / per)))
+    t0 = time.perf_counter()
+    o.step(2, k)
+    el = time.perf_counter() - t0
+    cores = oracle.threads()
+    o.close()
+    return {
+        "value": n ** 3 * k / el,
+        "unit": "cell-updates/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{k} steps (after 1 warm-up) of the same {n}^3 {'FP64' if prec == 8 else 'FP32'} dam break, "
+                  f"{el:.1f} s, OpenMP oracle restating voftrial1.cu's kernels on {cores} threads",
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (the hot path is a single-GPU stencil step); got --gpus {a.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+    import vof
+
+    vof.load_library()
+    n, prec = a.n, a.precision
+    esz = 8 if prec == 8 else 4
+    cells = n ** 3
+    s = make_solver(vof, n, prec)
+    graph = not a.eager
+
+    # warm-up (also captures the three graphs of the sweep rotation)
+    s.step(1, a.warmup, graph=graph)
+    s.sync()
+    istep = 1 + a.warmup
+    t0 = time.perf_counter()
+    s.step(istep, a.steps, graph=graph)
+    s.sync()
+    wall = time.perf_counter() - t0
+    istep += a.steps
+    value = cells * a.steps / wall
+    ms_step = 1e3 * wall / a.steps
+
+    # per-stage device times from HIP events on the solver's stream
+    st_steps = max(3, a.stage_steps)
+    _, stages = s.time_steps(istep, 0, st_steps, stages=True)
+    istep += st_steps
+    per_step = {k: v / st_steps for k, v in stages.items()}
+    kern = {}
+    for k, b in STAGE_BYTES.items():
+        t_launch = per_step[k] / LAUNCHES_PER_STEP[k] * 1e-3
+        if t_launch <= 0:
+            continue
+        byts = b * (esz // 4) * cells
+        kern[k] = {"ms_per_launch": t_launch * 1e3, "bytes_per_launch": byts, "gbs": byts / t_launch / 1e9,
+                   "frac": byts / t_launch / 1e9 / HBM_PEAK_GBS}
+    dom = max(kern, key=lambda k: per_step[k])
+    d = kern[dom]
+    step_bytes = sum(STAGE_BYTES[k] * LAUNCHES_PER_STEP[k] for k in STAGE_BYTES) * (esz // 4) * cells
+    diag = s.diagnostics()
+    s.close()
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == 8 else "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"dam-break {n}^3 {'FP64' if prec == 8 else 'FP32'}, 0.1 m cube, liquid x<0.04 y<0.04 "
+                        f"z<0.06 m, rho 1000/50, g=(0,0,-9.81), dt=4e-6 s, 10 Jacobi sweeps/step, "
+                        f"{'eager launches' if a.eager else 'hipGraph per step'}",
+            "grid": [n, n, n],
+            "interior_cells": cells,
+            "jacobi_sweeps": 10,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": d["gbs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": d["frac"],
+            "traffic": None,
+            "bytes_per_launch": d["bytes_per_launch"],
+            "ms_per_launch": d["ms_per_launch"],
+        },
+        "step_hbm_frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "stage_ms_per_step": {k: round(v, 5) for k, v in per_step.items()},
+        "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                    for k, v in kern.items()},
+        "final_liquid_volume": diag["liquid_volume"],
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(n, prec, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

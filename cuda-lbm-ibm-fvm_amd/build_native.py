@@ -1,0 +1,74 @@
+"""Build the native pieces in-tree.
+
+* ``lib/libvof_hip.so`` — gfx950 HIP kernels + C ABI (hipcc).
+* ``lib/3dvof``         — the drop-in CLI, linked against the library.
+* ``oracle/liboracle.so`` — the CPU oracle used by tests and the CPU baseline
+  (test infrastructure, built here because ``build()`` builds every checker).
+
+Run ``python cuda-lbm-ibm-fvm_amd/build_native.py`` or call :func:`build_all`.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIB = os.path.join(PKG, "lib")
+ORACLE = os.path.join(ROOT, "oracle")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+# -ffp-contract=off: no a*b+c contraction, so every source operation is one
+# IEEE rounding, as in the oracle.  Packed-FP32 instructions are disabled.
+HIP_FLAGS = [
+    "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def _run(cmd: list[str]) -> None:
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_hip() -> str:
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libvof_hip.so")
+    tmp = out + ".tmp"
+    srcs = [os.path.join(CSRC, f) for f in ("vof_kernels.hip", "vof_host.hip", "vof_vtk.cpp")]
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-pthread", *srcs, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_cli() -> str:
+    out = os.path.join(LIB, "3dvof")
+    _run(["g++", "-O2", "-std=c++17", os.path.join(CSRC, "vof_cli.cpp"), "-L" + LIB, "-lvof_hip",
+          "-Wl,-rpath,$ORIGIN", "-o", out])
+    return out
+
+
+def build_oracle() -> str:
+    out = os.path.join(ORACLE, "liboracle.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-O3", "-march=x86-64-v3", "-std=c++17", "-fopenmp", "-ffp-contract=off", "-fPIC", "-shared",
+          os.path.join(ORACLE, "vof_oracle.cpp"), "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_all(hip: bool = True) -> None:
+    build_oracle()
+    if hip:
+        build_hip()
+        build_cli()
+
+
+if __name__ == "__main__":
+    build_all(hip="--oracle-only" not in sys.argv)

@@ -1,0 +1,86 @@
+// vof_device.h — device-side types and helpers shared by every gfx950 kernel.
+//
+// One `Inst` describes one dam-break instance: its grid, the constants every
+// stage needs (pre-multiplied exactly the way the reference's expressions
+// associate them, so results stay bit-identical), and the device pointers of
+// its fields.  The plain solver is an ensemble of one instance; the batched
+// solver is an ensemble of many, found per work-group through a prefix table of
+// work-group counts.  Field layout is the reference's: (nx+2)(ny+2)(nz+2),
+// one ghost layer, x fastest (voftrial1.cu:66-70).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vof {
+
+template <class R>
+struct Inst {
+    int nx, ny, nz;  // interior cells
+    int sx, sy;      // row pitch (nx+2), plane pitch (nx+2)(ny+2), in elements
+    int ntot;        // (nx+2)(ny+2)(nz+2)
+    R dt, dx, dy, dz, dxi, dyi, dzi;
+    R vol;                     // (dx*dy)*dz                        (voftrial1.cu:518)
+    R dxdy;                    // dx*dy, y-sweep stage-1 divisor     (:672)
+    R axx, ayy, azz;           // dxi*dxi, dyi*dyi, dzi*dzi          (:424-429)
+    R dtdydz, dtdxdz, dtdxdy;  // (dt*dy)*dz etc., dilation factors  (:519, :661, :800)
+    R rho_l, rho_g, nu_l, nu_g;
+    R gx, gy, gz;
+    R sigma;
+    R* F[2];  // volume fraction, ping-pong across directional sweeps
+    R* u;
+    R* v;
+    R* w;     // face velocities (MAC: u on the low-x face of cell i)
+    R* us;
+    R* vs;
+    R* ws;    // predicted velocities u*, v*, w*
+    R* p[2];  // pressure, ping-pong across Jacobi launches
+    R* rhs;   // rho/dt * div(u*), formed once per step
+};
+
+// Work-group -> instance lookup.  boff has ninst+1 entries: boff[i] is the
+// first work-group of instance i for this launch shape, boff[ninst] the total.
+__device__ __forceinline__ int find_inst(const int* __restrict__ boff, int ninst, int b) {
+    int lo = 0, hi = ninst;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (boff[mid] <= b) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Work-groups are dealt round-robin to the 8 XCDs (each with its own L2).
+// Renumber so that every XCD works through one contiguous range of logical
+// work-groups; neighbouring tiles then share an L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+template <class R> __device__ __forceinline__ R vmax(R a, R b);
+template <class R> __device__ __forceinline__ R vmin(R a, R b);
+template <> __device__ __forceinline__ float vmax<float>(float a, float b) { return fmaxf(a, b); }
+template <> __device__ __forceinline__ float vmin<float>(float a, float b) { return fminf(a, b); }
+template <> __device__ __forceinline__ double vmax<double>(double a, double b) { return fmax(a, b); }
+template <> __device__ __forceinline__ double vmin<double>(double a, double b) { return fmin(a, b); }
+
+// median(0, 1, v) as ((0 + 1) + v) - max(max(0,1), v) - min(min(0,1), v); the
+// constant sub-terms fold exactly, leaving the form below.  Not a plain clamp:
+// values in (0,1) are rounded to the 2^-23 grid of 1+v (voftrial1.cu:52-60).
+template <class R>
+__device__ __forceinline__ R clamp01(R v) {
+    return ((R(1) + v) - vmax(R(1), v)) - vmin(R(0), v);
+}
+
+template <class R>
+__device__ __forceinline__ R rho_of(const Inst<R>& I, R F) {  // :100
+    const R f = clamp01(F);
+    return I.rho_g * (R(1) - f) + I.rho_l * f;
+}
+
+template <class R>
+__device__ __forceinline__ R nu_of(const Inst<R>& I, R F) {  // :101
+    const R f = clamp01(F);
+    return I.nu_l * f + I.nu_g * (R(1) - f);
+}
+
+}  // namespace vof

@@ -1,0 +1,828 @@
+// vof_host.hip — C ABI (include/vof_capi.h) over the gfx950 kernels.
+//
+// One Ensemble<R> owns the device arena, the instance table, the launch
+// shapes and one HIP stream.  The plain solver is an ensemble of one
+// instance; the batched solver an ensemble of many.  A step is 15 launches
+// for the reference configuration (predictor, rhs, 10 Jacobi sweeps,
+// correction+BC, 3 FCT sweeps, F ghosts) against the reference's 36, and can
+// be replayed as one hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/vof_capi.h"
+#include "vof_kernels.h"
+#include "vof_vtk.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+void set_err(const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+#define VOF_HIP(call)                                                                             \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) {                                                                   \
+            set_err("HIP failure at %s:%d : %s", __FILE__, __LINE__, hipGetErrorString(e_));     \
+            return VOF_ERR_HIP;                                                                   \
+        }                                                                                         \
+    } while (0)
+
+#define VOF_TRY(call)                       \
+    do {                                    \
+        vof_status s_ = (call);             \
+        if (s_ != VOF_OK) return s_;        \
+    } while (0)
+
+// Liquid extents of the reference IC: number of interior cells whose lower
+// face coordinate float(i-1)*dx satisfies 0 <= x <= L*frac, evaluated in float
+// exactly as set_init_F_kernel does (voftrial1.cu:80-91).
+int ref_ic_extent(int n, double L, float divisor) {
+    const float fL = static_cast<float>(L);
+    const float d = fL / static_cast<float>(n);
+    const float lim = fL / divisor;
+    int cnt = 0;
+    for (int i = 1; i <= n; ++i) {
+        const float x = static_cast<float>(i - 1) * d;
+        if (x >= 0.0f && x <= lim) cnt = i;
+    }
+    return cnt;
+}
+
+// Extension box IC: count of cells with lower face (i-1)*L/n < b, i.e.
+// i-1 < b*n/L, decided with a tolerance of 1e-9 cells so that exact ties
+// (b*n/L integral) are not counted.
+int box_extent(int n, double L, double b) {
+    const double q = b * double(n) / L;
+    const int c = int(std::ceil(q - 1e-9));
+    return std::max(0, std::min(n, c));
+}
+
+struct EnsBase {
+    virtual ~EnsBase() {}
+    int prec = 4;
+    virtual vof_status init_extents(const std::vector<int>& ex, const std::vector<int>& ey,
+                                    const std::vector<int>& ez) = 0;
+    virtual vof_status step(int64_t first, int n, bool graph) = 0;
+    virtual vof_status download(int inst, vof_field f, void* host, size_t count) = 0;
+    virtual vof_status upload(int inst, vof_field f, const void* host, size_t count) = 0;
+    virtual vof_status diagnostics(int inst, vof_diag* d) = 0;
+    virtual vof_status sync() = 0;
+    virtual vof_status time_steps(int64_t first, int warmup, int steps, bool graph, double* ms,
+                                  double* stage_ms) = 0;
+    virtual vof_status set_tuning(const vof_tuning* t) = 0;
+    virtual void get_tuning(vof_tuning* t) const = 0;
+    virtual vof_status debug_stage(int stage) = 0;
+    virtual int64_t interior_cells() const = 0;
+    virtual int64_t total_cells(int inst) const = 0;
+    virtual int ninst() const = 0;
+    virtual const vof_params& params(int inst) const = 0;
+};
+
+template <class R>
+struct Ensemble : EnsBase {
+    std::vector<vof_params> prm;
+    std::vector<vof::Inst<R>> ih;
+    vof::Ens<R> e;
+    vof::Geo g_pred, g_rhs, g_jac, g_corr, g_fx, g_fy, g_fz, g_ghost, g_red, g_sub;
+    vof_tuning tun{4, 8, 8, 4, 32, 1};
+    int fsel = 0, psel = 0;
+    int sweeps = 10;
+    uint32_t flags = 0;
+    hipStream_t st = nullptr;
+    void* arena = nullptr;
+    std::vector<int*> boffs;
+    double* part = nullptr;
+    double* red_out = nullptr;
+    double* means = nullptr;
+    long long* ncell = nullptr;
+    size_t part_cap = 0;
+    std::map<int, hipGraphExec_t> graphs;
+    std::vector<hipGraph_t> graph_defs;
+
+    ~Ensemble() override {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        for (auto g : graph_defs) (void)hipGraphDestroy(g);
+        for (int* p : boffs) (void)hipFree(p);
+        if (part) (void)hipFree(part);
+        if (red_out) (void)hipFree(red_out);
+        if (means) (void)hipFree(means);
+        if (ncell) (void)hipFree(ncell);
+        if (e.tab) (void)hipFree(e.tab);
+        if (arena) (void)hipFree(arena);
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    int ninst() const override { return int(ih.size()); }
+    const vof_params& params(int i) const override { return prm[size_t(i)]; }
+    int64_t interior_cells() const override {
+        int64_t s = 0;
+        for (auto& I : ih) s += int64_t(I.nx) * I.ny * I.nz;
+        return s;
+    }
+    int64_t total_cells(int i) const override { return ih[size_t(i)].ntot; }
+
+    static void derive(const vof_params& p, vof::Inst<R>& I) {
+        I.nx = p.nx; I.ny = p.ny; I.nz = p.nz;
+        I.sx = p.nx + 2;
+        I.sy = (p.nx + 2) * (p.ny + 2);
+        I.ntot = I.sy * (p.nz + 2);
+        // Inputs rounded to R first; derived values formed in R (voftrial1.cu:1272-1290).
+        const R Lx = R(p.Lx), Ly = R(p.Ly), Lz = R(p.Lz);
+        I.dt = R(p.dt);
+        I.dx = Lx / static_cast<R>(p.nx);
+        I.dy = Ly / static_cast<R>(p.ny);
+        I.dz = Lz / static_cast<R>(p.nz);
+        I.dxi = R(1) / I.dx; I.dyi = R(1) / I.dy; I.dzi = R(1) / I.dz;
+        I.vol = I.dx * I.dy * I.dz;
+        I.dxdy = I.dx * I.dy;
+        I.axx = I.dxi * I.dxi; I.ayy = I.dyi * I.dyi; I.azz = I.dzi * I.dzi;
+        I.dtdydz = I.dt * I.dy * I.dz;
+        I.dtdxdz = I.dt * I.dx * I.dz;
+        I.dtdxdy = I.dt * I.dx * I.dy;
+        I.rho_l = R(p.rho_l); I.rho_g = R(p.rho_g);
+        I.nu_l = R(p.nu_l); I.nu_g = R(p.nu_g);
+        I.gx = R(p.gx); I.gy = R(p.gy); I.gz = R(p.gz);
+        I.sigma = R(p.sigma);
+    }
+
+    vof_status make_geo(vof::Geo& g, int kz, int (*count)(const vof::Inst<R>&, int)) {
+        std::vector<int> off(ih.size() + 1, 0);
+        for (size_t i = 0; i < ih.size(); ++i) off[i + 1] = off[i] + count(ih[i], kz);
+        if (!g.boff) {
+            VOF_HIP(hipMalloc(&g.boff, off.size() * sizeof(int)));
+            boffs.push_back(g.boff);
+        }
+        VOF_HIP(hipMemcpy(g.boff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+        g.nblk = off.back();
+        g.kz = kz;
+        return VOF_OK;
+    }
+
+    vof_status build_geos() {
+        VOF_TRY(make_geo(g_pred, tun.kz_predict, [](const vof::Inst<R>& I, int kz) {
+            return vof::blocks_col(I.nx, I.ny, I.nz, kz); }));
+        VOF_TRY(make_geo(g_rhs, tun.kz_rhs, [](const vof::Inst<R>& I, int kz) {
+            return vof::blocks_col(I.nx, I.ny, I.nz, kz); }));
+        VOF_TRY(make_geo(g_jac, tun.kz_jacobi, [](const vof::Inst<R>& I, int kz) {
+            return vof::blocks_col(I.nx, I.ny, I.nz, kz); }));
+        VOF_TRY(make_geo(g_sub, 8, [](const vof::Inst<R>& I, int kz) {
+            return vof::blocks_col(I.nx, I.ny, I.nz, kz); }));
+        VOF_TRY(make_geo(g_corr, tun.kz_correct, [](const vof::Inst<R>& I, int kz) {
+            return vof::blocks_col(I.nx, I.ny, I.nz + 2, kz); }));
+        VOF_TRY(make_geo(g_red, 8, [](const vof::Inst<R>& I, int kz) {
+            return vof::blocks_col(I.nx, I.ny, I.nz + 2, kz); }));
+        VOF_TRY(make_geo(g_fx, 1, [](const vof::Inst<R>& I, int) {
+            return vof::blocks_fct_x(I.nx, I.ny, I.nz); }));
+        VOF_TRY(make_geo(g_fy, tun.fct_chunk, [](const vof::Inst<R>& I, int ch) {
+            return vof::blocks_march(I.nx, I.ny, I.nz, ch); }));
+        VOF_TRY(make_geo(g_fz, tun.fct_chunk, [](const vof::Inst<R>& I, int ch) {
+            return vof::blocks_march(I.nx, I.nz, I.ny, ch); }));
+        VOF_TRY(make_geo(g_ghost, 1, [](const vof::Inst<R>& I, int) {
+            return vof::blocks_ghost(I.nx, I.ny, I.nz); }));
+        const size_t need = size_t(g_red.nblk) * vof::kNumReduce;
+        if (need > part_cap) {
+            if (part) (void)hipFree(part);
+            part = nullptr;
+            VOF_HIP(hipMalloc(&part, need * sizeof(double)));
+            part_cap = need;
+        }
+        return VOF_OK;
+    }
+
+    vof_status create(const vof_params* p, int n) {
+        prec = sizeof(R);
+        prm.assign(p, p + n);
+        sweeps = p[0].jacobi_sweeps;
+        flags = p[0].flags;
+        if (p[0].jacobi_tblock > 0) tun.jacobi_tblock = p[0].jacobi_tblock;
+        ih.resize(size_t(n));
+        size_t total = 0;
+        for (int i = 0; i < n; ++i) {
+            derive(p[i], ih[size_t(i)]);
+            total += size_t(ih[size_t(i)].ntot) * 11;
+        }
+        VOF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        const size_t bytes = total * sizeof(R);
+        if (hipMalloc(&arena, bytes) != hipSuccess) {
+            arena = nullptr;
+            set_err("device allocation of %zu bytes failed", bytes);
+            return VOF_ERR_ALLOC;
+        }
+        // Every field starts at zero: never-written entries (ghost p, wall faces
+        // of u*, ...) must read zero, as after the reference's cudaMemset (:1202).
+        VOF_HIP(hipMemset(arena, 0, bytes));
+        R* cur = static_cast<R*>(arena);
+        for (auto& I : ih) {
+            R** slots[11] = {&I.F[0], &I.F[1], &I.u, &I.v, &I.w, &I.us, &I.vs, &I.ws, &I.p[0], &I.p[1], &I.rhs};
+            for (R** s : slots) {
+                *s = cur;
+                cur += I.ntot;
+            }
+        }
+        VOF_HIP(hipMalloc(&e.tab, ih.size() * sizeof(vof::Inst<R>)));
+        VOF_HIP(hipMemcpy(e.tab, ih.data(), ih.size() * sizeof(vof::Inst<R>), hipMemcpyHostToDevice));
+        e.ninst = n;
+        VOF_HIP(hipMalloc(&red_out, size_t(n) * vof::kNumReduce * sizeof(double)));
+        VOF_HIP(hipMalloc(&means, size_t(n) * sizeof(double)));
+        VOF_HIP(hipMalloc(&ncell, size_t(n) * sizeof(long long)));
+        std::vector<long long> nc(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i) nc[size_t(i)] = (long long)p[i].nx * p[i].ny * p[i].nz;
+        VOF_HIP(hipMemcpy(ncell, nc.data(), nc.size() * sizeof(long long), hipMemcpyHostToDevice));
+        VOF_TRY(build_geos());
+        VOF_HIP(hipDeviceSynchronize());
+        return VOF_OK;
+    }
+
+    void drop_graphs() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+        for (auto g : graph_defs) (void)hipGraphDestroy(g);
+        graph_defs.clear();
+    }
+
+    vof_status set_tuning(const vof_tuning* t) override {
+        if (!t) { set_err("null tuning"); return VOF_ERR_INVALID; }
+        if (t->kz_predict > 0) tun.kz_predict = t->kz_predict;
+        if (t->kz_rhs > 0) tun.kz_rhs = t->kz_rhs;
+        if (t->kz_jacobi > 0) tun.kz_jacobi = t->kz_jacobi;
+        if (t->kz_correct > 0) tun.kz_correct = t->kz_correct;
+        if (t->fct_chunk > 0) tun.fct_chunk = t->fct_chunk;
+        if (t->jacobi_tblock > 0) tun.jacobi_tblock = t->jacobi_tblock;
+        VOF_HIP(hipStreamSynchronize(st));
+        drop_graphs();
+        return build_geos();
+    }
+    void get_tuning(vof_tuning* t) const override { *t = tun; }
+
+    // ---- initial condition: F = [ci <= ex][cj <= ey][ck <= ez] over every
+    // stored cell (c* = index clamped to the interior), i.e. the liquid box and
+    // its zero-gradient ghosts, as IC + apply_bc_F leave it.
+    vof_status init_extents(const std::vector<int>& ex, const std::vector<int>& ey,
+                            const std::vector<int>& ez) override {
+        VOF_HIP(hipStreamSynchronize(st));
+        for (size_t n = 0; n < ih.size(); ++n) {
+            const auto& I = ih[n];
+            std::vector<R> h(size_t(I.ntot), R(0));
+            for (int k = 0; k < I.nz + 2; ++k) {
+                const int ck = std::min(std::max(k, 1), I.nz);
+                if (ck > ez[n]) continue;
+                for (int j = 0; j < I.ny + 2; ++j) {
+                    const int cj = std::min(std::max(j, 1), I.ny);
+                    if (cj > ey[n]) continue;
+                    for (int i = 0; i < I.nx + 2; ++i) {
+                        const int ci = std::min(std::max(i, 1), I.nx);
+                        if (ci <= ex[n]) h[size_t(k) * I.sy + size_t(j) * I.sx + i] = R(1);
+                    }
+                }
+            }
+            VOF_HIP(hipMemcpy(I.F[fsel], h.data(), h.size() * sizeof(R), hipMemcpyHostToDevice));
+        }
+        return VOF_OK;
+    }
+
+    // ---- one step, the reference loop body (voftrial1.cu:1362-1401).
+    void enqueue_step(int64_t istep) {
+        vof::launch_predict(e, g_pred, fsel, st);             // :1364-1368
+        vof::launch_rhs(e, g_rhs, fsel, st);                  // rhs of :418-422
+        for (int it = 0; it < sweeps; ++it) {                  // :1371-1375
+            vof::launch_jacobi(e, g_jac, psel, st);
+            psel ^= 1;
+        }
+        if (flags & VOF_FLAG_REMOVE_P_MEAN) {
+            vof::launch_reduce(e, g_red, fsel, psel, 1, part, ncell, red_out, means, st);
+            vof::launch_sub_mean(e, g_sub, psel, means, st);
+        }
+        vof::launch_correct(e, g_corr, fsel, psel, st);       // :1377-1382
+        static const int order[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};  // :1384-1396
+        const int r = int(istep % 3);
+        for (int d = 0; d < 3; ++d) {
+            vof::Geo& g = (order[r][d] == 0) ? g_fx : (order[r][d] == 1) ? g_fy : g_fz;
+            vof::launch_fct(e, g, order[r][d], fsel, st);
+            fsel ^= 1;
+        }
+        vof::launch_ghost_F(e, g_ghost, fsel, st);            // :1398-1401
+    }
+
+    vof_status replay(int64_t istep) {
+        const int key = int(istep % 3) * 4 + fsel * 2 + psel;
+        auto it = graphs.find(key);
+        if (it == graphs.end()) {
+            hipGraph_t g = nullptr;
+            const int f0 = fsel, p0 = psel;
+            VOF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            enqueue_step(istep);
+            const hipError_t ce = hipStreamEndCapture(st, &g);
+            if (ce != hipSuccess) {
+                set_err("HIP failure at %s:%d : %s", __FILE__, __LINE__, hipGetErrorString(ce));
+                return VOF_ERR_HIP;
+            }
+            fsel = f0; psel = p0;  // capture recorded the step without running it
+            hipGraphExec_t ex = nullptr;
+            VOF_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+            graph_defs.push_back(g);
+            it = graphs.emplace(key, ex).first;
+        }
+        VOF_HIP(hipGraphLaunch(it->second, st));
+        // advance the buffer parities exactly as enqueue_step does
+        psel ^= (sweeps & 1);
+        fsel ^= 1;
+        return VOF_OK;
+    }
+
+    vof_status step(int64_t first, int n, bool graph) override {
+        if (n < 0) { set_err("negative step count"); return VOF_ERR_INVALID; }
+        for (int s = 0; s < n; ++s) {
+            if (graph) {
+                VOF_TRY(replay(first + s));
+            } else {
+                enqueue_step(first + s);
+            }
+        }
+        VOF_HIP(hipGetLastError());
+        if (flags & VOF_FLAG_WATCHDOG) {
+            for (int i = 0; i < ninst(); ++i) {
+                vof_diag d;
+                VOF_TRY(diagnostics(i, &d));
+                if (d.nonfinite != 0 || !(d.f_min >= 0.0) || !(d.f_max <= 1.0)) {
+                    set_err("watchdog: instance %d has %lld non-finite values, F in [%g, %g]", i,
+                            (long long)d.nonfinite, d.f_min, d.f_max);
+                    return VOF_ERR_NUMERICS;
+                }
+            }
+        }
+        return VOF_OK;
+    }
+
+    vof_status debug_stage(int stage) override {
+        switch (stage) {
+            case 0: vof::launch_predict(e, g_pred, fsel, st); break;
+            case 1: vof::launch_rhs(e, g_rhs, fsel, st); break;
+            case 2: vof::launch_jacobi(e, g_jac, psel, st); psel ^= 1; break;
+            case 3: vof::launch_correct(e, g_corr, fsel, psel, st); break;
+            case 4: vof::launch_fct(e, g_fx, 0, fsel, st); fsel ^= 1; break;
+            case 5: vof::launch_fct(e, g_fy, 1, fsel, st); fsel ^= 1; break;
+            case 6: vof::launch_fct(e, g_fz, 2, fsel, st); fsel ^= 1; break;
+            case 7: vof::launch_ghost_F(e, g_ghost, fsel, st); break;
+            default: set_err("unknown stage %d", stage); return VOF_ERR_INVALID;
+        }
+        VOF_HIP(hipGetLastError());
+        VOF_HIP(hipStreamSynchronize(st));
+        return VOF_OK;
+    }
+
+    R* field_ptr(int inst, vof_field f) {
+        auto& I = ih[size_t(inst)];
+        switch (f) {
+            case VOF_FIELD_F: return I.F[fsel];
+            case VOF_FIELD_U: return I.u;
+            case VOF_FIELD_V: return I.v;
+            case VOF_FIELD_W: return I.w;
+            case VOF_FIELD_P: return I.p[psel];
+            case VOF_FIELD_USTAR: return I.us;
+            case VOF_FIELD_VSTAR: return I.vs;
+            case VOF_FIELD_WSTAR: return I.ws;
+            case VOF_FIELD_RHS: return I.rhs;
+            default: return nullptr;
+        }
+    }
+
+    vof_status download(int inst, vof_field f, void* host, size_t count) override {
+        if (inst < 0 || inst >= ninst()) { set_err("instance %d out of range", inst); return VOF_ERR_INVALID; }
+        const auto& I = ih[size_t(inst)];
+        if (!host || count != size_t(I.ntot)) {
+            set_err("download: count %zu != %d stored cells", count, I.ntot);
+            return VOF_ERR_INVALID;
+        }
+        if (f == VOF_FIELD_RHO || f == VOF_FIELD_NU) {
+            // Derived from F on the host with the same float operations
+            // (cal_nu_rho_kernel, :95-104).
+            std::vector<R> F(count);
+            VOF_TRY(download(inst, VOF_FIELD_F, F.data(), count));
+            R* out = static_cast<R*>(host);
+            for (size_t t = 0; t < count; ++t) {
+                const R v = F[t];
+                const R fc = ((R(1) + v) - std::fmax(R(1), v)) - std::fmin(R(0), v);
+                out[t] = (f == VOF_FIELD_RHO) ? I.rho_g * (R(1) - fc) + I.rho_l * fc
+                                              : I.nu_l * fc + I.nu_g * (R(1) - fc);
+            }
+            return VOF_OK;
+        }
+        R* src = field_ptr(inst, f);
+        if (!src) { set_err("download: unknown field %d", int(f)); return VOF_ERR_INVALID; }
+        VOF_HIP(hipMemcpyAsync(host, src, count * sizeof(R), hipMemcpyDeviceToHost, st));
+        VOF_HIP(hipStreamSynchronize(st));
+        return VOF_OK;
+    }
+
+    vof_status upload(int inst, vof_field f, const void* host, size_t count) override {
+        if (inst < 0 || inst >= ninst()) { set_err("instance %d out of range", inst); return VOF_ERR_INVALID; }
+        const auto& I = ih[size_t(inst)];
+        if (!host || count != size_t(I.ntot)) {
+            set_err("upload: count %zu != %d stored cells", count, I.ntot);
+            return VOF_ERR_INVALID;
+        }
+        if (f == VOF_FIELD_RHO || f == VOF_FIELD_NU) { set_err("rho and nu are derived from F"); return VOF_ERR_INVALID; }
+        R* dst = field_ptr(inst, f);
+        if (!dst) { set_err("upload: unknown field %d", int(f)); return VOF_ERR_INVALID; }
+        VOF_HIP(hipMemcpyAsync(dst, host, count * sizeof(R), hipMemcpyHostToDevice, st));
+        VOF_HIP(hipStreamSynchronize(st));
+        return VOF_OK;
+    }
+
+    vof_status diagnostics(int inst, vof_diag* d) override {
+        if (!d || inst < 0 || inst >= ninst()) { set_err("diagnostics: bad argument"); return VOF_ERR_INVALID; }
+        vof::launch_reduce(e, g_red, fsel, psel, 0, part, ncell, red_out, nullptr, st);
+        VOF_HIP(hipGetLastError());
+        double h[vof::kNumReduce];
+        VOF_HIP(hipMemcpyAsync(h, red_out + size_t(inst) * vof::kNumReduce, sizeof(h), hipMemcpyDeviceToHost, st));
+        VOF_HIP(hipStreamSynchronize(st));
+        const auto& I = ih[size_t(inst)];
+        const double ncl = double(I.nx) * I.ny * I.nz;
+        d->liquid_volume = double(I.vol) * h[0];
+        d->p_mean = h[1] / ncl;
+        d->residual_l2 = std::sqrt(h[2]);
+        d->rhs_l2 = std::sqrt(h[3]);
+        d->f_min = h[4];
+        d->f_max = h[5];
+        d->nonfinite = (int64_t)h[6];
+        return VOF_OK;
+    }
+
+    vof_status sync() override {
+        VOF_HIP(hipStreamSynchronize(st));
+        return VOF_OK;
+    }
+
+    vof_status time_steps(int64_t first, int warmup, int steps, bool graph, double* ms,
+                          double* stage_ms) override {
+        int64_t is = first;
+        VOF_TRY(step(is, warmup, graph));
+        is += warmup;
+        VOF_HIP(hipStreamSynchronize(st));
+        hipEvent_t a, b;
+        VOF_HIP(hipEventCreate(&a));
+        VOF_HIP(hipEventCreate(&b));
+        if (stage_ms) {
+            // Per-stage device time: an event after every stage of eager steps.
+            // Slot q of ev brackets stage q-1; stages follow the order of the
+            // header (predictor, rhs, jacobi, p-mean, correction, fct x, y, z, ghosts).
+            constexpr int NS = VOF_NSTAGE_TIMES;
+            for (int q = 0; q < NS; ++q) stage_ms[q] = 0.0;
+            hipEvent_t ev[NS + 1];
+            for (auto& x : ev) VOF_HIP(hipEventCreate(&x));
+            for (int s = 0; s < steps; ++s, ++is) {
+                VOF_HIP(hipEventRecord(ev[0], st));
+                vof::launch_predict(e, g_pred, fsel, st);
+                VOF_HIP(hipEventRecord(ev[1], st));
+                vof::launch_rhs(e, g_rhs, fsel, st);
+                VOF_HIP(hipEventRecord(ev[2], st));
+                for (int it = 0; it < sweeps; ++it) { vof::launch_jacobi(e, g_jac, psel, st); psel ^= 1; }
+                VOF_HIP(hipEventRecord(ev[3], st));
+                if (flags & VOF_FLAG_REMOVE_P_MEAN) {
+                    vof::launch_reduce(e, g_red, fsel, psel, 1, part, ncell, red_out, means, st);
+                    vof::launch_sub_mean(e, g_sub, psel, means, st);
+                }
+                VOF_HIP(hipEventRecord(ev[4], st));
+                vof::launch_correct(e, g_corr, fsel, psel, st);
+                VOF_HIP(hipEventRecord(ev[5], st));
+                static const int order[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
+                const int r = int(is % 3);
+                hipEvent_t dir_end[3];
+                for (int d = 0; d < 3; ++d) {
+                    const int dir = order[r][d];
+                    vof::Geo& g = (dir == 0) ? g_fx : (dir == 1) ? g_fy : g_fz;
+                    vof::launch_fct(e, g, dir, fsel, st);
+                    fsel ^= 1;
+                    dir_end[d] = ev[6 + d];
+                    VOF_HIP(hipEventRecord(dir_end[d], st));
+                }
+                vof::launch_ghost_F(e, g_ghost, fsel, st);
+                VOF_HIP(hipEventRecord(ev[NS], st));
+                VOF_HIP(hipEventSynchronize(ev[NS]));
+                float t = 0.f;
+                for (int q = 0; q < 5; ++q) {
+                    VOF_HIP(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
+                    stage_ms[q] += t;
+                }
+                hipEvent_t prev = ev[5];
+                for (int d = 0; d < 3; ++d) {
+                    VOF_HIP(hipEventElapsedTime(&t, prev, dir_end[d]));
+                    stage_ms[5 + order[r][d]] += t;
+                    prev = dir_end[d];
+                }
+                VOF_HIP(hipEventElapsedTime(&t, prev, ev[NS]));
+                stage_ms[8] += t;
+            }
+            for (auto& x : ev) (void)hipEventDestroy(x);
+            *ms = 0.0;
+            for (int q = 0; q < NS; ++q) *ms += stage_ms[q];
+        } else {
+            VOF_HIP(hipEventRecord(a, st));
+            VOF_TRY(step(is, steps, graph));
+            VOF_HIP(hipEventRecord(b, st));
+            VOF_HIP(hipEventSynchronize(b));
+            float t = 0.f;
+            VOF_HIP(hipEventElapsedTime(&t, a, b));
+            *ms = t;
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return VOF_OK;
+    }
+};
+
+vof_status validate(const vof_params& p) {
+    if (p.nx < 1 || p.ny < 1 || p.nz < 1) { set_err("grid must have at least one cell per axis"); return VOF_ERR_INVALID; }
+    const double ntot = double(p.nx + 2) * (p.ny + 2) * (p.nz + 2);
+    if (ntot >= 2147483647.0) { set_err("grid too large: %.0f stored cells exceed 32-bit indexing", ntot); return VOF_ERR_INVALID; }
+    if (p.precision != 4 && p.precision != 8) { set_err("precision must be 4 or 8, got %d", p.precision); return VOF_ERR_INVALID; }
+    if (p.jacobi_sweeps < 0) { set_err("jacobi_sweeps must be >= 0"); return VOF_ERR_INVALID; }
+    if (!(p.dt > 0.0) || !(p.Lx > 0.0) || !(p.Ly > 0.0) || !(p.Lz > 0.0)) { set_err("dt and L must be positive"); return VOF_ERR_INVALID; }
+    if (p.flags & (VOF_FLAG_CSF | VOF_FLAG_RHO_PREDICTOR | VOF_FLAG_NO_SLIP)) {
+        set_err("flags 0x%x are not available in this build", p.flags & (VOF_FLAG_CSF | VOF_FLAG_RHO_PREDICTOR | VOF_FLAG_NO_SLIP));
+        return VOF_ERR_UNSUPPORTED;
+    }
+    return VOF_OK;
+}
+
+EnsBase* make_ensemble(const vof_params* p, int n, vof_status* st) {
+    for (int i = 0; i < n; ++i) {
+        *st = validate(p[i]);
+        if (*st != VOF_OK) return nullptr;
+        if (p[i].precision != p[0].precision) { set_err("all instances must share one precision"); *st = VOF_ERR_INVALID; return nullptr; }
+    }
+    EnsBase* e = nullptr;
+    try {
+        if (p[0].precision == 8) {
+            auto* x = new Ensemble<double>();
+            *st = x->create(p, n);
+            e = x;
+        } else {
+            auto* x = new Ensemble<float>();
+            *st = x->create(p, n);
+            e = x;
+        }
+    } catch (const std::bad_alloc&) {
+        set_err("host allocation failed");
+        *st = VOF_ERR_ALLOC;
+        return nullptr;
+    }
+    if (*st != VOF_OK) {
+        delete e;
+        return nullptr;
+    }
+    return e;
+}
+
+vof_status init_ref_ic(EnsBase* e, int ic) {
+    const int n = e->ninst();
+    std::vector<int> ex(static_cast<size_t>(n)), ey(static_cast<size_t>(n)), ez(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) {
+        const vof_params& p = e->params(i);
+        if (ic == 1) {
+            ex[size_t(i)] = ref_ic_extent(p.nx, p.Lx, 3.0f);
+            ey[size_t(i)] = ref_ic_extent(p.ny, p.Ly, 2.0f);
+            ez[size_t(i)] = ref_ic_extent(p.nz, p.Lz, 3.0f);
+        } else {
+            ex[size_t(i)] = ey[size_t(i)] = ez[size_t(i)] = 0;
+        }
+    }
+    return e->init_extents(ex, ey, ez);
+}
+
+}  // namespace
+
+struct vof_solver {
+    EnsBase* e;
+};
+struct vof_batch {
+    EnsBase* e;
+};
+
+extern "C" {
+
+void vof_params_default(vof_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->nx = p->ny = p->nz = 200;
+    p->precision = 4;
+    p->Lx = p->Ly = p->Lz = 0.1;
+    p->rho_l = 1000.0; p->rho_g = 50.0;
+    p->nu_l = 1.0e-6; p->nu_g = 1.5e-5;
+    p->sigma = 0.007;
+    p->gx = 0.0; p->gy = -5.0; p->gz = 0.0;
+    p->dt = 4.0e-6;
+    p->jacobi_sweeps = 10;
+    p->flags = 0;
+    p->jacobi_tblock = 0;
+}
+
+vof_status vof_create(const vof_params* p, vof_solver** out) {
+    if (!p || !out) { set_err("null argument"); return VOF_ERR_INVALID; }
+    *out = nullptr;
+    vof_status st = VOF_OK;
+    EnsBase* e = make_ensemble(p, 1, &st);
+    if (!e) return st;
+    *out = new (std::nothrow) vof_solver{e};
+    if (!*out) { delete e; set_err("host allocation failed"); return VOF_ERR_ALLOC; }
+    return VOF_OK;
+}
+
+void vof_destroy(vof_solver* s) {
+    if (!s) return;
+    delete s->e;
+    delete s;
+}
+
+#define VOF_NEED(s) do { if (!(s) || !(s)->e) { set_err("null handle"); return VOF_ERR_INVALID; } } while (0)
+
+vof_status vof_init_ic(vof_solver* s, int ic) {
+    VOF_NEED(s);
+    if (ic < 1 || ic > 3) { set_err("Invalid initial condition. Allowed values: 1, 2, 3"); return VOF_ERR_INVALID; }
+    return init_ref_ic(s->e, ic);
+}
+
+vof_status vof_init_box(vof_solver* s, double bx, double by, double bz) {
+    VOF_NEED(s);
+    const vof_params& p = s->e->params(0);
+    return s->e->init_extents({box_extent(p.nx, p.Lx, bx)}, {box_extent(p.ny, p.Ly, by)},
+                              {box_extent(p.nz, p.Lz, bz)});
+}
+
+vof_status vof_step(vof_solver* s, int64_t first_istep, int nsteps) {
+    VOF_NEED(s);
+    return s->e->step(first_istep, nsteps, false);
+}
+vof_status vof_step_graph(vof_solver* s, int64_t first_istep, int nsteps) {
+    VOF_NEED(s);
+    return s->e->step(first_istep, nsteps, true);
+}
+vof_status vof_download(vof_solver* s, vof_field f, void* host, size_t count) {
+    VOF_NEED(s);
+    return s->e->download(0, f, host, count);
+}
+vof_status vof_upload(vof_solver* s, vof_field f, const void* host, size_t count) {
+    VOF_NEED(s);
+    return s->e->upload(0, f, host, count);
+}
+vof_status vof_diagnostics(vof_solver* s, vof_diag* d) {
+    VOF_NEED(s);
+    return s->e->diagnostics(0, d);
+}
+vof_status vof_sync(vof_solver* s) {
+    VOF_NEED(s);
+    return s->e->sync();
+}
+vof_status vof_time_steps(vof_solver* s, int64_t first_istep, int warmup, int steps, int use_graph,
+                          double* ms_total, double* stage_ms) {
+    VOF_NEED(s);
+    if (!ms_total || warmup < 0 || steps < 1) { set_err("time_steps: bad argument"); return VOF_ERR_INVALID; }
+    return s->e->time_steps(first_istep, warmup, steps, use_graph != 0, ms_total, stage_ms);
+}
+vof_status vof_set_tuning(vof_solver* s, const vof_tuning* t) {
+    VOF_NEED(s);
+    return s->e->set_tuning(t);
+}
+vof_status vof_get_tuning(const vof_solver* s, vof_tuning* t) {
+    if (!s || !s->e || !t) { set_err("null argument"); return VOF_ERR_INVALID; }
+    s->e->get_tuning(t);
+    return VOF_OK;
+}
+vof_status vof_debug_stage(vof_solver* s, int stage) {
+    VOF_NEED(s);
+    return s->e->debug_stage(stage);
+}
+
+vof_status vof_write_vtk(vof_solver* s, const char* path) {
+    VOF_NEED(s);
+    if (!path) { set_err("null path"); return VOF_ERR_INVALID; }
+    const vof_params& p = s->e->params(0);
+    const size_t n = size_t(s->e->total_cells(0));
+    int rc;
+    if (p.precision == 8) {
+        std::vector<double> h(n);
+        VOF_TRY(s->e->download(0, VOF_FIELD_F, h.data(), n));
+        rc = vof::write_legacy_vtk_f64(path, h.data(), p.nx + 2, p.ny + 2, p.nz + 2);
+    } else {
+        std::vector<float> h(n);
+        VOF_TRY(s->e->download(0, VOF_FIELD_F, h.data(), n));
+        rc = vof::write_legacy_vtk_f32(path, h.data(), p.nx + 2, p.ny + 2, p.nz + 2);
+    }
+    if (rc != 0) { set_err("Failed to open output file: %s", path); return VOF_ERR_INVALID; }
+    return VOF_OK;
+}
+
+vof_status vof_write_vtk_host(const char* path, const void* data, int precision, int nxTot, int nyTot,
+                              int nzTot) {
+    if (!path || !data || nxTot < 2 || nyTot < 2 || nzTot < 2 || (precision != 4 && precision != 8)) {
+        set_err("write_vtk_host: bad argument");
+        return VOF_ERR_INVALID;
+    }
+    const int rc = (precision == 8)
+                       ? vof::write_legacy_vtk_f64(path, static_cast<const double*>(data), nxTot, nyTot, nzTot)
+                       : vof::write_legacy_vtk_f32(path, static_cast<const float*>(data), nxTot, nyTot, nzTot);
+    if (rc != 0) { set_err("Failed to open output file: %s", path); return VOF_ERR_INVALID; }
+    return VOF_OK;
+}
+
+int64_t vof_interior_cells(const vof_solver* s) { return (s && s->e) ? s->e->interior_cells() : 0; }
+int64_t vof_total_cells(const vof_solver* s) { return (s && s->e) ? s->e->total_cells(0) : 0; }
+
+const char* vof_status_str(vof_status st) {
+    switch (st) {
+        case VOF_OK: return "ok";
+        case VOF_ERR_INVALID: return "invalid argument";
+        case VOF_ERR_HIP: return "HIP runtime error";
+        case VOF_ERR_ALLOC: return "allocation failed";
+        case VOF_ERR_UNSUPPORTED: return "unsupported";
+        case VOF_ERR_NUMERICS: return "numerical watchdog tripped";
+    }
+    return "unknown status";
+}
+const char* vof_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------- batch
+vof_status vof_batch_create(const vof_params* params, int ninst, vof_batch** out) {
+    if (!params || !out || ninst < 1) { set_err("batch_create: bad argument"); return VOF_ERR_INVALID; }
+    *out = nullptr;
+    vof_status st = VOF_OK;
+    EnsBase* e = make_ensemble(params, ninst, &st);
+    if (!e) return st;
+    *out = new (std::nothrow) vof_batch{e};
+    if (!*out) { delete e; set_err("host allocation failed"); return VOF_ERR_ALLOC; }
+    return VOF_OK;
+}
+void vof_batch_destroy(vof_batch* b) {
+    if (!b) return;
+    delete b->e;
+    delete b;
+}
+vof_status vof_batch_init_box(vof_batch* b, const double* bx, const double* by, const double* bz) {
+    VOF_NEED(b);
+    if (!bx || !by || !bz) { set_err("null box arrays"); return VOF_ERR_INVALID; }
+    const int n = b->e->ninst();
+    std::vector<int> ex(static_cast<size_t>(n)), ey(static_cast<size_t>(n)), ez(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) {
+        const vof_params& p = b->e->params(i);
+        ex[size_t(i)] = box_extent(p.nx, p.Lx, bx[i]);
+        ey[size_t(i)] = box_extent(p.ny, p.Ly, by[i]);
+        ez[size_t(i)] = box_extent(p.nz, p.Lz, bz[i]);
+    }
+    return b->e->init_extents(ex, ey, ez);
+}
+vof_status vof_batch_init_ic(vof_batch* b, int ic) {
+    VOF_NEED(b);
+    if (ic < 1 || ic > 3) { set_err("Invalid initial condition. Allowed values: 1, 2, 3"); return VOF_ERR_INVALID; }
+    return init_ref_ic(b->e, ic);
+}
+vof_status vof_batch_step(vof_batch* b, int64_t first_istep, int nsteps, int use_graph) {
+    VOF_NEED(b);
+    return b->e->step(first_istep, nsteps, use_graph != 0);
+}
+vof_status vof_batch_download(vof_batch* b, int inst, vof_field f, void* host, size_t count) {
+    VOF_NEED(b);
+    return b->e->download(inst, f, host, count);
+}
+vof_status vof_batch_upload(vof_batch* b, int inst, vof_field f, const void* host, size_t count) {
+    VOF_NEED(b);
+    return b->e->upload(inst, f, host, count);
+}
+vof_status vof_batch_diagnostics(vof_batch* b, int inst, vof_diag* d) {
+    VOF_NEED(b);
+    return b->e->diagnostics(inst, d);
+}
+vof_status vof_batch_sync(vof_batch* b) {
+    VOF_NEED(b);
+    return b->e->sync();
+}
+vof_status vof_batch_time_steps(vof_batch* b, int64_t first_istep, int warmup, int steps, int use_graph,
+                                double* ms_total) {
+    VOF_NEED(b);
+    if (!ms_total || warmup < 0 || steps < 1) { set_err("time_steps: bad argument"); return VOF_ERR_INVALID; }
+    return b->e->time_steps(first_istep, warmup, steps, use_graph != 0, ms_total, nullptr);
+}
+vof_status vof_batch_set_tuning(vof_batch* b, const vof_tuning* t) {
+    VOF_NEED(b);
+    return b->e->set_tuning(t);
+}
+int64_t vof_batch_interior_cells(const vof_batch* b) { return (b && b->e) ? b->e->interior_cells() : 0; }
+int vof_batch_size(const vof_batch* b) { return (b && b->e) ? b->e->ninst() : 0; }
+
+}  // extern "C"

@@ -1,0 +1,56 @@
+// vof_kernels.h — host-side launchers of the gfx950 kernels (vof_kernels.hip).
+#pragma once
+#include "vof_device.h"
+
+namespace vof {
+
+// Launch shape of one kernel over an ensemble: prefix table of work-group
+// counts per instance (device memory, ninst+1 entries), the total, and the
+// number of cells each lane walks along its march direction.
+struct Geo {
+    int* boff = nullptr;
+    int nblk = 0;
+    int kz = 1;
+};
+
+template <class R>
+struct Ens {
+    Inst<R>* tab = nullptr;  // device copy of the instance table
+    int ninst = 0;
+};
+
+constexpr int kNumReduce = 7;
+
+template <class R> void launch_predict(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
+template <class R> void launch_rhs(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
+template <class R> void launch_jacobi(const Ens<R>& e, const Geo& g, int psrc, hipStream_t st);
+template <class R> void launch_correct(const Ens<R>& e, const Geo& g, int fsel, int psel, hipStream_t st);
+template <class R> void launch_fct(const Ens<R>& e, const Geo& g, int dir, int src, hipStream_t st);
+template <class R> void launch_ghost_F(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
+template <class R> void launch_sub_mean(const Ens<R>& e, const Geo& g, int psel, const double* means, hipStream_t st);
+template <class R>
+void launch_reduce(const Ens<R>& e, const Geo& g, int fsel, int psel, int mode, double* part,
+                   const long long* ncell, double* out, double* means, hipStream_t st);
+
+// Work-groups per instance for each launch shape (host mirror of the device
+// index arithmetic).
+inline int blocks_col(int nx, int ny, int nzrange, int kz) {
+    const int plane = (nx + 2) * (ny + 2);
+    return ((plane + 255) / 256) * ((nzrange + kz - 1) / kz);
+}
+inline int blocks_fct_x(int nx, int ny, int nz) {
+    const int P = (nx + 2) * (ny + 2);
+    const int wpp = (P + 57) / 58;
+    return ((wpp + 3) / 4) * nz;
+}
+inline int blocks_march(int nx, int nd, int no, int ch) {
+    const int ntr = no * (nx + 2);
+    return ((ntr + 255) / 256) * ((nd + ch - 1) / ch);
+}
+inline int blocks_ghost(int nx, int ny, int nz) {
+    const int sx = nx + 2;
+    const long n = 2L * sx * (ny + 2) + 2L * sx * nz + 2L * ny * nz;
+    return int((n + 255) / 256);
+}
+
+}  // namespace vof

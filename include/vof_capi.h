@@ -1,0 +1,204 @@
+/*
+ * vof_capi.h — C ABI of the MI355X (gfx950) VOF timestep library `libvof_hip.so`.
+ *
+ * The reference solver (voftrial1.cu) has no plugin or FFI surface; its step is a
+ * fixed sequence of host wrappers over `DeviceFields` with the parameters in
+ * `__constant__ Params cpar`.  Each entry point below replaces one piece of that
+ * sequence; the comment on each names the reference code it stands in for.
+ *
+ * Conventions
+ *  - The library owns all device memory; the caller owns host buffers and no
+ *    pointer is retained after a call returns.
+ *  - Every call returns a vof_status; nothing inside the library calls exit().
+ *    vof_last_error() gives a message for the last failure on the calling thread.
+ *  - One handle = one HIP stream; a handle is not thread-safe, separate handles
+ *    may be driven from separate threads.  vof_step is asynchronous; the
+ *    download/diagnostics/vtk calls synchronise the handle's stream.
+ *  - Field layout in host buffers is the reference's: (nx+2)(ny+2)(nz+2)
+ *    values, one ghost layer, x fastest, idx = (k*(ny+2) + j)*(nx+2) + i
+ *    (voftrial1.cu:66-70).  Element type is float for precision 4 and double
+ *    for precision 8.
+ */
+#ifndef VOF_CAPI_H
+#define VOF_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum vof_status {
+    VOF_OK = 0,
+    VOF_ERR_INVALID = 1,     /* bad argument / size / precision */
+    VOF_ERR_HIP = 2,         /* a HIP runtime call failed */
+    VOF_ERR_ALLOC = 3,       /* device or host allocation failed */
+    VOF_ERR_UNSUPPORTED = 4, /* feature not available in this build */
+    VOF_ERR_NUMERICS = 5     /* watchdog: NaN/Inf or F outside [0,1] */
+} vof_status;
+
+/* Fields a caller can move across the boundary (names follow DeviceFields,
+ * voftrial1.cu:1176-1198).  RHO and NU are derived from F and are read-only. */
+typedef enum vof_field {
+    VOF_FIELD_F = 0,
+    VOF_FIELD_U = 1,
+    VOF_FIELD_V = 2,
+    VOF_FIELD_W = 3,
+    VOF_FIELD_P = 4,
+    VOF_FIELD_USTAR = 5,
+    VOF_FIELD_VSTAR = 6,
+    VOF_FIELD_WSTAR = 7,
+    VOF_FIELD_RHS = 8,
+    VOF_FIELD_RHO = 9,
+    VOF_FIELD_NU = 10
+} vof_field;
+
+/* Extension flags (no counterpart in the reference; all default off). */
+#define VOF_FLAG_REMOVE_P_MEAN 0x1u /* subtract the FP64 interior mean of p after the Poisson sweeps */
+#define VOF_FLAG_CSF           0x2u /* continuum-surface-force surface tension in the predictor */
+#define VOF_FLAG_RHO_PREDICTOR 0x4u /* gravity/CSF scaled by face density in the predictor */
+#define VOF_FLAG_NO_SLIP       0x8u /* tangential wall ghosts mirror to give zero wall velocity */
+#define VOF_FLAG_WATCHDOG      0x10u/* check F in [0,1] and finite u,v,w,p after every step */
+
+/* Solver parameters.  Replaces `struct Params` + `cpar_host` defaults
+ * (voftrial1.cu:27-41, :1258-1291).  Physical inputs are double; they are
+ * rounded to the field precision before any derived value (dx = L/n,
+ * dxi = 1/dx, vol) is computed in that precision, as the reference does in float. */
+typedef struct vof_params {
+    int32_t nx, ny, nz;         /* interior cells */
+    int32_t precision;          /* 4 = FP32 fields, 8 = FP64 fields */
+    double Lx, Ly, Lz;          /* domain extents [m] */
+    double rho_l, rho_g;        /* densities [kg/m^3] */
+    double nu_l, nu_g;          /* kinematic viscosities [m^2/s] */
+    double sigma;               /* surface tension [N/m]; used only with VOF_FLAG_CSF */
+    double gx, gy, gz;          /* gravity [m/s^2] */
+    double dt;                  /* time step [s] */
+    int32_t jacobi_sweeps;      /* Jacobi sweeps per step (reference: 10, voftrial1.cu:1371) */
+    uint32_t flags;             /* VOF_FLAG_* */
+    int32_t jacobi_tblock;      /* sweeps fused per Jacobi launch (temporal blocking); 0 = auto */
+    int32_t reserved;
+} vof_params;
+
+/* FP64 global diagnostics (no counterpart in the reference, which has no reductions). */
+typedef struct vof_diag {
+    double liquid_volume;   /* vol * sum_interior F  [m^3] */
+    double p_mean;          /* mean of p over interior cells */
+    double residual_l2;     /* || rhs - A p ||_2 over interior cells, after the last step's sweeps */
+    double rhs_l2;          /* || rhs ||_2 (for a relative residual) */
+    double f_min, f_max;    /* min/max of F over interior cells */
+    int64_t nonfinite;      /* count of NaN/Inf in u,v,w,p,F (interior+ghost) */
+} vof_diag;
+
+typedef struct vof_solver vof_solver;
+
+/* Reference defaults of voftrial1.cu:1258-1283 (200^3, 0.1 m cube, rho 1000/50,
+ * nu 1e-6/1.5e-5, sigma 0.007, g = (0,-5,0), dt = 4e-6, 10 sweeps, FP32). */
+void vof_params_default(vof_params* p);
+
+/* Allocate and zero every device field (replaces alloc_field / the 20 cudaMalloc +
+ * cudaMemset(0) calls, voftrial1.cu:1200-1203, :1303-1326; the zero fill is
+ * load-bearing: never-written ghost entries must read 0). */
+vof_status vof_create(const vof_params* p, vof_solver** out);
+void vof_destroy(vof_solver* s);                      /* replaces free_all, :1205-1226 */
+
+/* Dam-break initial condition, then the F ghost fill.  ic == 1 is the
+ * reference predicate (set_init_F_kernel + apply_bc_F, :72-93, :1328-1336),
+ * evaluated in float exactly as the reference does; ic 2 and 3 leave F = 0
+ * (the reference accepts them and sets no liquid). */
+vof_status vof_init_ic(vof_solver* s, int ic);
+
+/* Extension IC: liquid in the box x < bx, y < by, z < bz [m] by the lower-face
+ * rule with strict '<', evaluated in exact integer arithmetic (no reference
+ * counterpart; the reference box is fixed at Lx/3, Ly/2, Lz/3 inclusive). */
+vof_status vof_init_box(vof_solver* s, double bx, double by, double bz);
+
+/* Advance nsteps time steps.  first_istep is the 1-based index of the first
+ * step (the reference's ++istep, :1362); it selects the FCT sweep rotation
+ * istep%3 (:1384-1396).  Replaces the loop body :1362-1401. */
+vof_status vof_step(vof_solver* s, int64_t first_istep, int nsteps);
+
+/* Same as vof_step, but replays one captured hipGraph per step (captured on
+ * first use per rotation/buffer parity). Results are identical to vof_step. */
+vof_status vof_step_graph(vof_solver* s, int64_t first_istep, int nsteps);
+
+/* Host <-> device copies of a whole field in the reference layout.  count is
+ * the number of elements and must equal (nx+2)(ny+2)(nz+2).  Replaces the
+ * cudaMemcpy of F at :1405 (and generalises it to every state field). */
+vof_status vof_download(vof_solver* s, vof_field f, void* host, size_t count);
+vof_status vof_upload(vof_solver* s, vof_field f, const void* host, size_t count);
+
+/* FP64 diagnostics of the current state (synchronises). */
+vof_status vof_diagnostics(vof_solver* s, vof_diag* d);
+
+/* Legacy ASCII VTK of F, byte-identical to write_legacy_vtk (:907-941). */
+vof_status vof_write_vtk(vof_solver* s, const char* path);
+
+/* Host-only VTK writer on a caller buffer (same bytes as vof_write_vtk; no
+ * device access).  precision 4: data is float*, 8: double*. */
+vof_status vof_write_vtk_host(const char* path, const void* data, int precision, int nxTot, int nyTot,
+                              int nzTot);
+
+/* Synchronise the handle's stream. */
+vof_status vof_sync(vof_solver* s);
+
+/* Time the plain step with HIP events on the handle's stream: runs `warmup`
+ * untimed and `steps` timed steps starting at first_istep and returns the
+ * milliseconds in *ms_total.  When stage_ms is not NULL the steps run eagerly
+ * with an event between stages and stage_ms[VOF_NSTAGE_TIMES] receives the
+ * device time per stage summed over the timed steps, in the order
+ * predictor, rhs, jacobi (all sweeps), p-mean removal, correction+BC,
+ * fct x, fct y, fct z, F ghosts. */
+#define VOF_NSTAGE_TIMES 9
+vof_status vof_time_steps(vof_solver* s, int64_t first_istep, int warmup, int steps,
+                          int use_graph, double* ms_total, double* stage_ms);
+
+/* Launch-shape tuning (no reference counterpart).  kz_* = planes each lane
+ * walks in the column kernels, fct_chunk = cells per lane in the y/z FCT
+ * marches, jacobi_tblock = Jacobi sweeps fused per launch (1 or 2).  Values
+ * <= 0 keep the current setting.  Changing them drops captured graphs. */
+typedef struct vof_tuning {
+    int32_t kz_predict, kz_rhs, kz_jacobi, kz_correct, fct_chunk, jacobi_tblock;
+} vof_tuning;
+vof_status vof_set_tuning(vof_solver* s, const vof_tuning* t);
+vof_status vof_get_tuning(const vof_solver* s, vof_tuning* t);
+
+/* Test hook: run one stage on the current state.  stage: 0 predictor,
+ * 1 rhs, 2 one Jacobi sweep, 3 correction + velocity BC, 4/5/6 FCT sweep
+ * x/y/z, 7 F ghost fill.  Not part of the reference surface. */
+vof_status vof_debug_stage(vof_solver* s, int stage);
+
+/* Number of interior cells n_x n_y n_z and of stored cells (n+2)^3. */
+int64_t vof_interior_cells(const vof_solver* s);
+int64_t vof_total_cells(const vof_solver* s);
+
+const char* vof_status_str(vof_status st);
+const char* vof_last_error(void);
+
+/* ---------------------------------------------------------------------------
+ * Batched ensemble: many instances of uneven grid size and parameters advance
+ * together, one launch per stage for the whole ensemble (no reference
+ * counterpart; the reference runs one fixed 200^3 case).
+ * All instances share the precision and Jacobi sweep count of params[0].
+ */
+typedef struct vof_batch vof_batch;
+vof_status vof_batch_create(const vof_params* params, int ninst, vof_batch** out);
+void vof_batch_destroy(vof_batch* b);
+/* per-instance IC box [m], lower-face strict '<' rule; bx/by/bz arrays of ninst */
+vof_status vof_batch_init_box(vof_batch* b, const double* bx, const double* by, const double* bz);
+vof_status vof_batch_init_ic(vof_batch* b, int ic);
+vof_status vof_batch_step(vof_batch* b, int64_t first_istep, int nsteps, int use_graph);
+vof_status vof_batch_download(vof_batch* b, int inst, vof_field f, void* host, size_t count);
+vof_status vof_batch_upload(vof_batch* b, int inst, vof_field f, const void* host, size_t count);
+vof_status vof_batch_diagnostics(vof_batch* b, int inst, vof_diag* d);
+vof_status vof_batch_sync(vof_batch* b);
+vof_status vof_batch_time_steps(vof_batch* b, int64_t first_istep, int warmup, int steps,
+                                int use_graph, double* ms_total);
+vof_status vof_batch_set_tuning(vof_batch* b, const vof_tuning* t);
+int64_t vof_batch_interior_cells(const vof_batch* b);
+int vof_batch_size(const vof_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VOF_CAPI_H */

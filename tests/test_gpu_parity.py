@@ -1,0 +1,242 @@
+"""GPU parity: the HIP library against the CPU oracle, through the C ABI.
+
+Bar: exact equality (IEEE ==, so +0 == -0) of every field the reference
+exposes, stage by stage and after whole steps, in FP32 and FP64.  Both sides
+are built without FMA contraction and keep the reference's operation order.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20261203
+
+
+def random_state(shape, rng, dtype, vel=1.0):
+    """Random volume fraction (many exact 0 and 1, interface values between)
+    and random face velocities / pressure; p ghosts zero as in the reference."""
+    F = rng.random(shape)
+    F[F < 0.35] = 0.0
+    F[F > 0.8] = 1.0
+    u = rng.uniform(-vel, vel, shape)
+    v = rng.uniform(-vel, vel, shape)
+    w = rng.uniform(-vel, vel, shape)
+    p = rng.normal(0.0, 50.0, shape)
+    p[0, :, :] = p[-1, :, :] = 0
+    p[:, 0, :] = p[:, -1, :] = 0
+    p[:, :, 0] = p[:, :, -1] = 0
+    return {k: a.astype(dtype) for k, a in dict(F=F, U=u, V=v, W=w, P=p).items()}
+
+
+def interior(a):
+    return a[1:-1, 1:-1, 1:-1]
+
+
+def assert_same(a, b, what):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape, what
+    bad = ~((a == b) | (np.isnan(a) & np.isnan(b)))
+    if bad.any():
+        idx = np.argwhere(bad)[:5]
+        diff = np.max(np.abs(a.astype(np.float64) - b.astype(np.float64))[bad])
+        raise AssertionError(f"{what}: {bad.sum()} of {a.size} differ (max |d| {diff:.3e}); first {idx.tolist()}"
+                             f" gpu {a[tuple(idx[0])]!r} oracle {b[tuple(idx[0])]!r}")
+
+
+def make_pair(vofmod, orc, **kw):
+    prm = vofmod.default_params(**kw)
+    s = vofmod.Solver(prm)
+    o = orc.Oracle(vofmod.default_params(**kw))
+    return s, o
+
+
+def load_state(s, o, st):
+    for k, a in st.items():
+        s.upload(k, a)
+        o.set(k, a)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("grid", [(13, 11, 9), (16, 16, 16), (70, 5, 7)])
+def test_stages(vofmod, orc, prec, grid):
+    nx, ny, nz = grid
+    print(f"stage parity prec={prec} grid={grid}", flush=True)
+    s, o = make_pair(vofmod, orc, nx=nx, ny=ny, nz=nz, precision=prec, dt=1e-3)
+    rng = np.random.default_rng(SEED + nx)
+    st = random_state(s.shape, rng, s.dtype)
+    load_state(s, o, st)
+    # predictor
+    s.debug_stage(vofmod.STAGE_PREDICT)
+    for name in ("nu_rho", "advect_u", "advect_v", "advect_w"):
+        o.stage(name)
+    for f in ("USTAR", "VSTAR", "WSTAR"):
+        assert_same(s.download(f), o.get(f), f)
+    # rhs (formed once; the oracle forms it inside each sweep)
+    s.debug_stage(vofmod.STAGE_RHS)
+    assert_same(interior(s.download("RHS")), interior(o.get("RHS")), "RHS")
+    # Jacobi sweeps
+    for it in range(3):
+        s.debug_stage(vofmod.STAGE_JACOBI)
+        o.stage("jacobi")
+        assert_same(s.download("P"), o.get("P"), f"P sweep {it}")
+    # correction + velocity BC, all stored entries
+    s.debug_stage(vofmod.STAGE_CORRECT)
+    o.stage("update")
+    o.stage("bc_vel")
+    for f in ("U", "V", "W"):
+        assert_same(s.download(f), o.get(f), f)
+    # FCT sweeps in a rotated order, then clamp + F BC
+    for gs, on in ((vofmod.STAGE_FCT_Y, "fct_y"), (vofmod.STAGE_FCT_Z, "fct_z"), (vofmod.STAGE_FCT_X, "fct_x")):
+        s.debug_stage(gs)
+        o.stage(on)
+        assert_same(interior(s.download("F")), interior(o.get("F")), on)
+    s.debug_stage(vofmod.STAGE_GHOST_F)
+    o.stage("clamp")
+    o.stage("bc_F")
+    assert_same(s.download("F"), o.get("F"), "F with ghosts")
+    s.close()
+    o.close()
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_fct_each_direction_random(vofmod, orc, prec):
+    """Each sweep direction alone on a random interface field with a large CFL."""
+    for gs, on in ((vofmod.STAGE_FCT_X, "fct_x"), (vofmod.STAGE_FCT_Y, "fct_y"), (vofmod.STAGE_FCT_Z, "fct_z")):
+        s, o = make_pair(vofmod, orc, nx=37, ny=29, nz=23, precision=prec, dt=2e-3)
+        rng = np.random.default_rng(SEED + 7)
+        st = random_state(s.shape, rng, s.dtype, vel=1.5)
+        load_state(s, o, st)
+        s.debug_stage(gs)
+        o.stage(on)
+        assert_same(interior(s.download("F")), interior(o.get("F")), on)
+        s.close()
+        o.close()
+
+
+def run_steps_pair(vofmod, orc, steps, first=1, graph=False, ic=1, box=None, **kw):
+    s, o = make_pair(vofmod, orc, **kw)
+    if box:
+        s.init_box(*box)
+        o.init_box(*box)
+    else:
+        s.init_ic(ic)
+        o.init_ic(ic)
+    assert_same(s.download("F"), o.get("F"), "initial F")
+    s.step(first, steps, graph=graph)
+    o.step(first, steps)
+    return s, o
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_whole_steps_reference_case(vofmod, orc, prec):
+    print(f"whole steps prec={prec} n=24", flush=True)
+    s, o = run_steps_pair(vofmod, orc, 30, n=24, precision=prec)
+    for f in ("F", "U", "V", "W", "P", "USTAR", "VSTAR", "WSTAR"):
+        assert_same(s.download(f), o.get(f), f)
+    d1, d2 = s.diagnostics(), o.diagnostics()
+    for k in ("liquid_volume", "p_mean", "residual_l2", "rhs_l2"):
+        assert d1[k] == pytest.approx(d2[k], rel=1e-12, abs=1e-300), k
+    assert d1["f_min"] == d2["f_min"] and d1["f_max"] == d2["f_max"]
+    assert d1["nonfinite"] == 0
+
+
+def test_whole_steps_config_box_gravity_noncubic(vofmod, orc):
+    """Non-cubic grid, extension box IC and gravity along -z (config style)."""
+    s, o = run_steps_pair(vofmod, orc, 25, nx=30, ny=22, nz=26, precision=4, g=(0.0, 0.0, -9.81), dt=2e-5,
+                          box=(0.04, 0.04, 0.06))
+    for f in ("F", "U", "V", "W", "P"):
+        assert_same(s.download(f), o.get(f), f)
+
+
+def test_whole_steps_random_velocity_large_dt(vofmod, orc):
+    """Random initial velocities and a large time step drive every FCT branch
+    (both upwind signs, active limiter, clipping) through whole steps."""
+    s, o = make_pair(vofmod, orc, nx=28, ny=20, nz=24, precision=4, dt=2e-4)
+    rng = np.random.default_rng(SEED + 3)
+    st = random_state(s.shape, rng, s.dtype, vel=0.8)
+    st["P"][:] = 0
+    load_state(s, o, st)
+    s.step(1, 12)
+    o.step(1, 12)
+    for f in ("F", "U", "V", "W", "P"):
+        assert_same(s.download(f), o.get(f), f)
+    assert float(interior(s.download("F")).min()) >= 0.0 and float(interior(s.download("F")).max()) <= 1.0
+
+
+def test_graph_equals_eager(vofmod):
+    print("graph == eager", flush=True)
+    a = vofmod.Solver(n=20)
+    b = vofmod.Solver(n=20)
+    a.init_ic(1)
+    b.init_ic(1)
+    a.step(1, 14, graph=False)
+    b.step(1, 14, graph=True)
+    for f in ("F", "U", "V", "W", "P"):
+        assert_same(a.download(f), b.download(f), f)
+    # eager work between replays, replays again
+    a.step(15, 3)
+    b.step(15, 3, graph=True)
+    assert_same(a.download("F"), b.download("F"), "F after more replays")
+
+
+def test_remove_p_mean(vofmod, orc):
+    flags = vofmod.FLAG_REMOVE_P_MEAN
+    s, o = run_steps_pair(vofmod, orc, 9, n=18, precision=8, flags=flags)
+    for f in ("F", "U", "V", "W"):
+        np.testing.assert_allclose(s.download(f), o.get(f), rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(s.download("P"), o.get("P"), rtol=1e-9, atol=1e-9)
+    assert abs(s.diagnostics()["p_mean"]) < 1e-9
+
+
+def test_batch_equals_plain(vofmod):
+    """Instances of a ragged batch reproduce plain runs bit for bit."""
+    sizes = [(12, 12, 12), (20, 16, 18), (9, 14, 11)]
+    prms = [vofmod.default_params(nx=a, ny=b, nz=c, g=(0, 0, -9.81), rho_g=25.0 + 10 * i, dt=2e-5)
+            for i, (a, b, c) in enumerate(sizes)]
+    boxes = [(0.04, 0.05, 0.06), (0.03, 0.04, 0.07), (0.05, 0.02, 0.05)]
+    bt = vofmod.Batch(prms)
+    bt.init_box(*zip(*boxes))
+    bt.step(1, 11, graph=True)
+    for i, prm in enumerate(prms):
+        s = vofmod.Solver(vofmod.default_params(nx=prm.nx, ny=prm.ny, nz=prm.nz, g=(0, 0, -9.81),
+                                                rho_g=prm.rho_g, dt=2e-5))
+        s.init_box(*boxes[i])
+        s.step(1, 11)
+        for f in ("F", "U", "V", "W", "P"):
+            assert_same(bt.download(i, f), s.download(f), f"inst {i} {f}")
+        s.close()
+
+
+def test_full_size_200_parity_three_steps(vofmod, orc):
+    """Reference default case at full size: bitwise after one step of each rotation."""
+    print("200^3 parity, 3 steps", flush=True)
+    s, o = run_steps_pair(vofmod, orc, 3, n=200, precision=4)
+    for f in ("F", "U", "V", "W", "P"):
+        assert_same(s.download(f), o.get(f), f)
+
+
+def test_full_size_200_invariants(vofmod):
+    """200^3, 300 steps: F stays in [0,1], no NaN, volume drift bounded."""
+    s = vofmod.Solver(n=200)
+    s.init_ic(1)
+    v0 = s.diagnostics()["liquid_volume"]
+    s.step(1, 300, graph=True)
+    d = s.diagnostics()
+    assert d["nonfinite"] == 0
+    assert 0.0 <= d["f_min"] and d["f_max"] <= 1.0
+    assert abs(d["liquid_volume"] - v0) / v0 < 1e-3
+    print("200^3 volume drift", (d["liquid_volume"] - v0) / v0, flush=True)
+
+
+def test_vtk_from_gpu_matches_oracle_writer(vofmod, orc, tmp_path):
+    s = vofmod.Solver(n=10)
+    s.init_ic(1)
+    s.step(1, 5)
+    a = tmp_path / "gpu.vtk"
+    b = tmp_path / "orc.vtk"
+    s.write_vtk(str(a))
+    orc.write_vtk_float(str(b), s.download("F"))
+    assert a.read_bytes() == b.read_bytes()
