@@ -357,10 +357,14 @@ struct Ensemble : EnsBase {
         }
         VOF_HIP(hipGetLastError());
         if (flags & VOF_FLAG_WATCHDOG) {
+            // clamp01 returns the rounding error of 1+v, so it can leave [0,1]
+            // by half an ulp of 1: [-2^-25, 1+2^-23] in float, [-2^-54, 1+2^-52] in double.
+            const double lo = (sizeof(R) == 4) ? -std::ldexp(1.0, -25) : -std::ldexp(1.0, -54);
+            const double hi = 1.0 + ((sizeof(R) == 4) ? std::ldexp(1.0, -23) : std::ldexp(1.0, -52));
             for (int i = 0; i < ninst(); ++i) {
                 vof_diag d;
                 VOF_TRY(diagnostics(i, &d));
-                if (d.nonfinite != 0 || !(d.f_min >= 0.0) || !(d.f_max <= 1.0)) {
+                if (d.nonfinite != 0 || !(d.f_min >= lo) || !(d.f_max <= hi)) {
                     set_err("watchdog: instance %d has %lld non-finite values, F in [%g, %g]", i,
                             (long long)d.nonfinite, d.f_min, d.f_max);
                     return VOF_ERR_NUMERICS;

@@ -12,6 +12,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 SEED = 20261203
+# range of the reference clamp01f in FP32 (it returns the rounding error of 1+v)
+F32_LO, F32_HI = -(2.0 ** -25), 1.0 + 2.0 ** -23
 
 
 def random_state(shape, rng, dtype, vel=1.0):
@@ -163,7 +165,7 @@ def test_whole_steps_random_velocity_large_dt(vofmod, orc):
     o.step(1, 12)
     for f in ("F", "U", "V", "W", "P"):
         assert_same(s.download(f), o.get(f), f)
-    assert float(interior(s.download("F")).min()) >= 0.0 and float(interior(s.download("F")).max()) <= 1.0
+    assert float(interior(s.download("F")).min()) >= F32_LO and float(interior(s.download("F")).max()) <= F32_HI
 
 
 def test_graph_equals_eager(vofmod):
@@ -226,7 +228,7 @@ def test_full_size_200_invariants(vofmod):
     s.step(1, 300, graph=True)
     d = s.diagnostics()
     assert d["nonfinite"] == 0
-    assert 0.0 <= d["f_min"] and d["f_max"] <= 1.0
+    assert F32_LO <= d["f_min"] and d["f_max"] <= F32_HI
     assert abs(d["liquid_volume"] - v0) / v0 < 1e-3
     print("200^3 volume drift", (d["liquid_volume"] - v0) / v0, flush=True)
 
