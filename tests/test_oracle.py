@@ -35,12 +35,14 @@ def test_clamp01f_leaks_rounding_error_outside_unit_interval():
     """The result is the rounding error of fl(1+v) for v <= 0 and of fl(1+v)-v for
     v >= 1, so the 'clamp' can leave [0,1] by half an ulp of 1+v:
     v = -1.5*2^-24 -> -2^-25 and v = 1 + 3*2^-23 -> 1 + 2^-23 (ties to even).
-    F is therefore bounded by [-2^-25, 1 + 2^-23] in the reference's FP32 step."""
+    For inputs in [-1, 3) (the range the FCT feeds it) the output therefore
+    lies in [-2^-25, 1 + 2^-23]; from v >= 3 the half ulp of 1+v is 2^-22."""
     assert f32(clamp01f(float(f32(-1.5 * 2 ** -24)))) == f32(-(2 ** -25))
     assert f32(clamp01f(float(f32(1 + 3 * 2 ** -23)))) == f32(1 + 2 ** -23)
+    assert f32(clamp01f(float(f32(3 + 3 * 2 ** -22)))) == f32(1 + 2 ** -22)
     rng = np.random.default_rng(11)
     v = np.concatenate([rng.uniform(-1e-6, 1e-6, 3000), rng.uniform(1 - 1e-6, 1 + 1e-6, 3000),
-                        rng.uniform(-3, 4, 3000)]).astype(f32)
+                        rng.uniform(-1, 2.999, 3000)]).astype(f32)
     out = np.array([clamp01f(float(x)) for x in v], dtype=np.float64)
     assert out.min() >= -(2 ** -25) and out.max() <= 1 + 2 ** -23
 
