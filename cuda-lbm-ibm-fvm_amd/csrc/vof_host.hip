@@ -101,7 +101,7 @@ struct Ensemble : EnsBase {
     std::vector<vof::Inst<R>> ih;
     vof::Ens<R> e;
     vof::Geo g_pred, g_rhs, g_jac, g_corr, g_fx, g_fy, g_fz, g_ghost, g_red, g_sub;
-    vof_tuning tun{4, 8, 8, 4, 32, 1};
+    vof_tuning tun{8, 8, 8, 8, 32, 1, 4};
     int fsel = 0, psel = 0;
     int sweeps = 10;
     uint32_t flags = 0;
@@ -176,20 +176,16 @@ struct Ensemble : EnsBase {
     }
 
     vof_status build_geos() {
-        VOF_TRY(make_geo(g_pred, tun.kz_predict, [](const vof::Inst<R>& I, int kz) {
-            return vof::blocks_col(I.nx, I.ny, I.nz, kz); }));
-        VOF_TRY(make_geo(g_rhs, tun.kz_rhs, [](const vof::Inst<R>& I, int kz) {
-            return vof::blocks_col(I.nx, I.ny, I.nz, kz); }));
-        VOF_TRY(make_geo(g_jac, tun.kz_jacobi, [](const vof::Inst<R>& I, int kz) {
-            return vof::blocks_col(I.nx, I.ny, I.nz, kz); }));
-        VOF_TRY(make_geo(g_sub, 8, [](const vof::Inst<R>& I, int kz) {
-            return vof::blocks_col(I.nx, I.ny, I.nz, kz); }));
-        VOF_TRY(make_geo(g_corr, tun.kz_correct, [](const vof::Inst<R>& I, int kz) {
-            return vof::blocks_col(I.nx, I.ny, I.nz + 2, kz); }));
+        auto col1 = [](const vof::Inst<R>& I, int kz) { return vof::blocks_wcol(I.nx, I.ny, I.nz, kz, 1); };
+        VOF_TRY(make_geo(g_pred, tun.kz_predict, col1));
+        VOF_TRY(make_geo(g_rhs, tun.kz_rhs, col1));
+        VOF_TRY(make_geo(g_jac, tun.kz_jacobi, col1));
+        VOF_TRY(make_geo(g_sub, 8, col1));
+        VOF_TRY(make_geo(g_corr, tun.kz_correct, col1));
         VOF_TRY(make_geo(g_red, 8, [](const vof::Inst<R>& I, int kz) {
-            return vof::blocks_col(I.nx, I.ny, I.nz + 2, kz); }));
-        VOF_TRY(make_geo(g_fx, 1, [](const vof::Inst<R>& I, int) {
-            return vof::blocks_fct_x(I.nx, I.ny, I.nz); }));
+            return vof::blocks_wcol(I.nx, I.ny, I.nz + 2, kz, 0); }));
+        VOF_TRY(make_geo(g_fx, tun.kz_fct_x, [](const vof::Inst<R>& I, int kz) {
+            return vof::blocks_wcol(I.nx, I.ny, I.nz, kz, 3); }));
         VOF_TRY(make_geo(g_fy, tun.fct_chunk, [](const vof::Inst<R>& I, int ch) {
             return vof::blocks_march(I.nx, I.ny, I.nz, ch); }));
         VOF_TRY(make_geo(g_fz, tun.fct_chunk, [](const vof::Inst<R>& I, int ch) {
@@ -265,6 +261,7 @@ struct Ensemble : EnsBase {
         if (t->kz_correct > 0) tun.kz_correct = t->kz_correct;
         if (t->fct_chunk > 0) tun.fct_chunk = t->fct_chunk;
         if (t->jacobi_tblock > 0) tun.jacobi_tblock = t->jacobi_tblock;
+        if (t->kz_fct_x > 0) tun.kz_fct_x = t->kz_fct_x;
         VOF_HIP(hipStreamSynchronize(st));
         drop_graphs();
         return build_geos();
@@ -312,12 +309,11 @@ struct Ensemble : EnsBase {
         vof::launch_correct(e, g_corr, fsel, psel, st);       // :1377-1382
         static const int order[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};  // :1384-1396
         const int r = int(istep % 3);
-        for (int d = 0; d < 3; ++d) {
+        for (int d = 0; d < 3; ++d) {  // the last sweep also does :1398-1401
             vof::Geo& g = (order[r][d] == 0) ? g_fx : (order[r][d] == 1) ? g_fy : g_fz;
-            vof::launch_fct(e, g, order[r][d], fsel, st);
+            vof::launch_fct(e, g, order[r][d], fsel, d == 2, st);
             fsel ^= 1;
         }
-        vof::launch_ghost_F(e, g_ghost, fsel, st);            // :1398-1401
     }
 
     vof_status replay(int64_t istep) {
@@ -380,10 +376,10 @@ struct Ensemble : EnsBase {
             case 1: vof::launch_rhs(e, g_rhs, fsel, st); break;
             case 2: vof::launch_jacobi(e, g_jac, psel, st); psel ^= 1; break;
             case 3: vof::launch_correct(e, g_corr, fsel, psel, st); break;
-            case 4: vof::launch_fct(e, g_fx, 0, fsel, st); fsel ^= 1; break;
-            case 5: vof::launch_fct(e, g_fy, 1, fsel, st); fsel ^= 1; break;
-            case 6: vof::launch_fct(e, g_fz, 2, fsel, st); fsel ^= 1; break;
-            case 7: vof::launch_ghost_F(e, g_ghost, fsel, st); break;
+            case 4: vof::launch_fct(e, g_fx, 0, fsel, 0, st); fsel ^= 1; break;
+            case 5: vof::launch_fct(e, g_fy, 1, fsel, 0, st); fsel ^= 1; break;
+            case 6: vof::launch_fct(e, g_fz, 2, fsel, 0, st); fsel ^= 1; break;
+            case 7: vof::launch_clamp_F(e, g_sub, fsel, st); vof::launch_ghost_F(e, g_ghost, fsel, st); break;
             default: set_err("unknown stage %d", stage); return VOF_ERR_INVALID;
         }
         VOF_HIP(hipGetLastError());
@@ -512,12 +508,12 @@ struct Ensemble : EnsBase {
                 for (int d = 0; d < 3; ++d) {
                     const int dir = order[r][d];
                     vof::Geo& g = (dir == 0) ? g_fx : (dir == 1) ? g_fy : g_fz;
-                    vof::launch_fct(e, g, dir, fsel, st);
+                    vof::launch_fct(e, g, dir, fsel, d == 2, st);
                     fsel ^= 1;
                     dir_end[d] = ev[6 + d];
                     VOF_HIP(hipEventRecord(dir_end[d], st));
                 }
-                vof::launch_ghost_F(e, g_ghost, fsel, st);
+                // F ghosts are written by the last sweep: the last slot stays ~0
                 VOF_HIP(hipEventRecord(ev[NS], st));
                 VOF_HIP(hipEventSynchronize(ev[NS]));
                 float t = 0.f;

@@ -25,7 +25,8 @@ template <class R> void launch_predict(const Ens<R>& e, const Geo& g, int fsel, 
 template <class R> void launch_rhs(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
 template <class R> void launch_jacobi(const Ens<R>& e, const Geo& g, int psrc, hipStream_t st);
 template <class R> void launch_correct(const Ens<R>& e, const Geo& g, int fsel, int psel, hipStream_t st);
-template <class R> void launch_fct(const Ens<R>& e, const Geo& g, int dir, int src, hipStream_t st);
+template <class R> void launch_fct(const Ens<R>& e, const Geo& g, int dir, int src, int fin, hipStream_t st);
+template <class R> void launch_clamp_F(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
 template <class R> void launch_ghost_F(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
 template <class R> void launch_sub_mean(const Ens<R>& e, const Geo& g, int psel, const double* means, hipStream_t st);
 template <class R>
@@ -33,15 +34,14 @@ void launch_reduce(const Ens<R>& e, const Geo& g, int fsel, int psel, int mode, 
                    const long long* ncell, double* out, double* means, hipStream_t st);
 
 // Work-groups per instance for each launch shape (host mirror of the device
-// index arithmetic).
-inline int blocks_col(int nx, int ny, int nzrange, int kz) {
+// index arithmetic).  Overlapped-wave columns: waves of 64 lanes storing
+// 64-2h positions of the (nx+2)(ny+2) plane, 4 waves per work-group, one
+// work-group row per chunk of kz planes out of nzrange.
+inline int blocks_wcol(int nx, int ny, int nzrange, int kz, int h) {
     const int plane = (nx + 2) * (ny + 2);
-    return ((plane + 255) / 256) * ((nzrange + kz - 1) / kz);
-}
-inline int blocks_fct_x(int nx, int ny, int nz) {
-    const int P = (nx + 2) * (ny + 2);
-    const int wpp = (P + 57) / 58;
-    return ((wpp + 3) / 4) * nz;
+    const int s = 64 - 2 * h;
+    const int wpp = (plane + s - 1) / s;
+    return ((wpp + 3) / 4) * ((nzrange + kz - 1) / kz);
 }
 inline int blocks_march(int nx, int nd, int no, int ch) {
     const int ntr = no * (nx + 2);

@@ -8,9 +8,15 @@
 // Floating-point contract: built with -ffp-contract=off and IEEE division, and
 // every expression keeps the association of the reference kernel it replaces,
 // so each field is bit-identical to the reference's arithmetic (the CPU oracle
-// checks this).  What changes is data movement: stages are fused, derived
-// fields (rho, nu) are recomputed from F in registers, the Jacobi right-hand
-// side is formed once per step, and each FCT direction is one kernel.
+// checks this).  What changes is data movement:
+//  * column kernels (predictor, rhs, Jacobi, correction) march along z with the
+//    z-neighbours carried in registers; x-neighbours come from the adjacent
+//    lane (wave shuffles) in overlapped waves of 64 lanes that store the middle
+//    64-2H; y-neighbours are one extra coalesced row load;
+//  * derived fields (rho, nu) are recomputed from F in registers;
+//  * the Jacobi right-hand side is formed once per step;
+//  * each FCT direction is one fused kernel, the last one of a step also
+//    applies post_process_f and writes the F ghost layer.
 #include "vof_device.h"
 #include "vof_kernels.h"
 
@@ -24,100 +30,128 @@ namespace vof {
     const int lb = b_ - boff[inst_];                                      \
     const Inst<R>& I = tab[inst_];
 
-// Column mapping: 256 lanes over the (nx+2)(ny+2) plane, x fastest, each lane
-// walks KZ consecutive planes.  Returns false for lanes past the plane.
-struct Col {
-    int i, j, k0, k1;
+template <class R> __device__ __forceinline__ R sh_up(R x) { return __shfl_up(x, 1); }    // value of lane-1
+template <class R> __device__ __forceinline__ R sh_dn(R x) { return __shfl_down(x, 1); }  // value of lane+1
+
+// Overlapped-wave column mapping.  The (nx+2)(ny+2) plane is flattened x
+// fastest; wave w covers positions [w*S - H, w*S - H + 64) with S = 64 - 2H and
+// stores only its middle S lanes, whose +-H neighbours along x are in the wave.
+// Each wave walks the planes k0..k1 of its z chunk.  All lanes of a wave stay
+// active (clamped positions) so that every shuffle reads a live lane.
+struct WMap {
+    int lane, t, tc, i, j, k0, k1;
+    bool own;  // lane's cell is stored by this wave
 };
 
-template <class R>
-__device__ __forceinline__ bool col_map(const Inst<R>& I, int lb, int kz, int klo, int khi, Col& c) {
-    const int plane = I.sx * (I.ny + 2);
-    const int npb = (plane + 255) >> 8;
-    const int pb = lb % npb, zb = lb / npb;
-    const int t = pb * 256 + (int)threadIdx.x;
-    if (t >= plane) return false;
-    c.j = t / I.sx;
-    c.i = t - c.j * I.sx;
-    c.k0 = klo + zb * kz;
-    c.k1 = min(khi, c.k0 + kz - 1);
+template <class R, int H>
+__device__ __forceinline__ bool wmap(const Inst<R>& I, int lb, int kz, int klo, int khi, WMap& m) {
+    constexpr int S = 64 - 2 * H;
+    const int P = I.sx * (I.ny + 2);
+    const int wpp = (P + S - 1) / S;
+    const int bpp = (wpp + 3) >> 2;
+    const int zb = lb / bpp;
+    const int wv = (lb - zb * bpp) * 4 + ((int)threadIdx.x >> 6);
+    if (wv >= wpp) return false;  // wave-uniform
+    m.lane = (int)threadIdx.x & 63;
+    m.t = wv * S - H + m.lane;
+    m.tc = min(max(m.t, 0), P - 1);
+    m.j = m.tc / I.sx;
+    m.i = m.tc - m.j * I.sx;
+    m.own = (m.lane >= H) && (m.lane < 64 - H) && (m.t >= 0) && (m.t < P);
+    m.k0 = klo + zb * kz;
+    m.k1 = min(khi, m.k0 + kz - 1);
     return true;
 }
 
 // ---------------------------------------------------------------------------
-// Momentum predictor (advect_{u,v,w}_kernel, voftrial1.cu:230-399), fused: the
-// thread of cell (i,j,k) forms u* on face i, v* on face j and w* on face k, each
-// only on interior faces (i>=2, j>=2, k>=2 — faces on the walls are never
-// written, as in the reference).  nu comes from F in registers (:95-104).
+// Momentum predictor (advect_{u,v,w}_kernel, voftrial1.cu:230-399) for one
+// face: first-order upwind advection by (ax, ay, az), explicit viscosity with
+// the cell's nu, plus gravity.  For u* the x advecting velocity is u itself,
+// for v* the y one is v, for w* the z one is w; the other two are the 4-point
+// averages of the reference.  Same association as :266-284.
 template <class R>
-__device__ __forceinline__ R predict_u(const Inst<R>& I, const R* __restrict__ u, const R* __restrict__ v,
-                                       const R* __restrict__ w, int c, R n0) {
-    const int sx = I.sx, sy = I.sy;
-    const R vh = R(0.25) * (v[c - 1] + v[c - 1 + sx] + v[c] + v[c + sx]);
-    const R wh = R(0.25) * (w[c - 1] + w[c - 1 + sy] + w[c] + w[c + sy]);
-    const R uc = u[c];
-    const R uxm = u[c - 1], uxp = u[c + 1], uym = u[c - sx], uyp = u[c + sx], uzm = u[c - sy], uzp = u[c + sy];
-    const R ddx = (uc > R(0)) ? (uc - uxm) * I.dxi : (uxp - uc) * I.dxi;
-    const R ddy = (vh > R(0)) ? (uc - uym) * I.dyi : (uyp - uc) * I.dyi;
-    const R ddz = (wh > R(0)) ? (uc - uzm) * I.dzi : (uzp - uc) * I.dzi;
-    const R visc = n0 * (uxm - R(2) * uc + uxp) * I.dxi * I.dxi + n0 * (uym - R(2) * uc + uyp) * I.dyi * I.dyi +
-                   n0 * (uzm - R(2) * uc + uzp) * I.dzi * I.dzi;
-    const R adv = -uc * ddx - vh * ddy - wh * ddz;
-    return uc + I.dt * (visc + adv + I.gx);
+__device__ __forceinline__ R mom(const Inst<R>& I, R c, R xm, R xp, R ym, R yp, R zm, R zp, R ax, R ay, R az,
+                                 R n0, R g) {
+    const R ddx = (ax > R(0)) ? (c - xm) * I.dxi : (xp - c) * I.dxi;
+    const R ddy = (ay > R(0)) ? (c - ym) * I.dyi : (yp - c) * I.dyi;
+    const R ddz = (az > R(0)) ? (c - zm) * I.dzi : (zp - c) * I.dzi;
+    const R visc = n0 * (xm - R(2) * c + xp) * I.dxi * I.dxi + n0 * (ym - R(2) * c + yp) * I.dyi * I.dyi +
+                   n0 * (zm - R(2) * c + zp) * I.dzi * I.dzi;
+    const R adv = -ax * ddx - ay * ddy - az * ddz;
+    return c + I.dt * (visc + adv + g);
 }
 
-template <class R>
-__device__ __forceinline__ R predict_v(const Inst<R>& I, const R* __restrict__ u, const R* __restrict__ v,
-                                       const R* __restrict__ w, int c, R n0) {
-    const int sx = I.sx, sy = I.sy;
-    const R uh = R(0.25) * (u[c - sx] + u[c] + u[c + 1 - sx] + u[c + 1]);
-    const R wh = R(0.25) * (w[c - sx + sy] + w[c - sx] + w[c] + w[c + sy]);
-    const R vc = v[c];
-    const R vxm = v[c - 1], vxp = v[c + 1], vym = v[c - sx], vyp = v[c + sx], vzm = v[c - sy], vzp = v[c + sy];
-    const R ddx = (uh > R(0)) ? (vc - vxm) * I.dxi : (vxp - vc) * I.dxi;
-    const R ddy = (vc > R(0)) ? (vc - vym) * I.dyi : (vyp - vc) * I.dyi;
-    const R ddz = (wh > R(0)) ? (vc - vzm) * I.dzi : (vzp - vc) * I.dzi;
-    const R visc = n0 * (vxm - R(2) * vc + vxp) * I.dxi * I.dxi + n0 * (vym - R(2) * vc + vyp) * I.dyi * I.dyi +
-                   n0 * (vzm - R(2) * vc + vzp) * I.dzi * I.dzi;
-    const R adv = -uh * ddx - vc * ddy - wh * ddz;
-    return vc + I.dt * (visc + adv + I.gy);
-}
-
-template <class R>
-__device__ __forceinline__ R predict_w(const Inst<R>& I, const R* __restrict__ u, const R* __restrict__ v,
-                                       const R* __restrict__ w, int c, R n0) {
-    const int sx = I.sx, sy = I.sy;
-    const R uh = R(0.25) * (u[c + 1 - sy] + u[c - sy] + u[c + 1] + u[c]);
-    const R vh = R(0.25) * (v[c + sx - sy] + v[c - sy] + v[c] + v[c + sx]);
-    const R wc = w[c];
-    const R wxm = w[c - 1], wxp = w[c + 1], wym = w[c - sx], wyp = w[c + sx], wzm = w[c - sy], wzp = w[c + sy];
-    const R ddx = (uh > R(0)) ? (wc - wxm) * I.dxi : (wxp - wc) * I.dxi;
-    const R ddy = (vh > R(0)) ? (wc - wym) * I.dyi : (wyp - wc) * I.dyi;
-    const R ddz = (wc > R(0)) ? (wc - wzm) * I.dzi : (wzp - wc) * I.dzi;
-    const R visc = n0 * (wxm - R(2) * wc + wxp) * I.dxi * I.dxi + n0 * (wym - R(2) * wc + wyp) * I.dyi * I.dyi +
-                   n0 * (wzm - R(2) * wc + wzp) * I.dzi * I.dzi;
-    const R adv = -uh * ddx - vh * ddy - wc * ddz;
-    return wc + I.dt * (visc + adv + I.gz);
-}
-
+// Fused predictor: the lane of cell (i,j,k) forms u* on face i, v* on face j
+// and w* on face k (interior faces only: i>=2, j>=2, k>=2; wall faces are never
+// written, as in the reference).  Per plane: 10 coalesced loads (u,v,w at k+1
+// and at rows j-1, j+1 of plane k+1, F at k), 8 lane shuffles.
 template <class R>
 __global__ __launch_bounds__(256) void k_predict(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
                                                  int ninst, int fsel, int kz) {
     VOF_PROLOGUE(R)
-    Col cm;
-    if (!col_map(I, lb, kz, 1, I.nz, cm)) return;
-    const int i = cm.i, j = cm.j;
-    if (i < 1 || i > I.nx || j < 1 || j > I.ny) return;
+    WMap m;
+    if (!wmap<R, 1>(I, lb, kz, 1, I.nz, m)) return;
+    const int i = m.i, j = m.j, sx = I.sx, sy = I.sy;
+    const int P = sx * (I.ny + 2);
+    const bool out = m.own && i >= 1 && i <= I.nx && j >= 1 && j <= I.ny;
+    const int tc = m.tc;
+    const int tjm = max(tc - sx, 0), tjp = min(tc + sx, P - 1);
     const R* __restrict__ F = I.F[fsel];
     const R* __restrict__ u = I.u;
     const R* __restrict__ v = I.v;
     const R* __restrict__ w = I.w;
-    for (int k = cm.k0; k <= cm.k1; ++k) {
-        const int c = k * I.sy + j * I.sx + i;
-        const R n0 = nu_of(I, F[c]);
-        if (i >= 2) I.us[c] = predict_u(I, u, v, w, c, n0);
-        if (j >= 2) I.vs[c] = predict_v(I, u, v, w, c, n0);
-        if (k >= 2) I.ws[c] = predict_w(I, u, v, w, c, n0);
+    R* __restrict__ us = I.us;
+    R* __restrict__ vs = I.vs;
+    R* __restrict__ ws = I.ws;
+    int k = m.k0;
+    int o = k * sy;
+    R u_km = u[o - sy + tc], u_k = u[o + tc];
+    R v_km = v[o - sy + tc], v_k = v[o + tc];
+    R w_km = w[o - sy + tc], w_k = w[o + tc];
+    R ujm = u[o + tjm], ujp = u[o + tjp];
+    R vjm = v[o + tjm], vjp = v[o + tjp], vjp_km = v[o - sy + tjp];
+    R wjm = w[o + tjm], wjp = w[o + tjp];
+    R udn_km = sh_dn(u_km);  // u(i+1, j, k-1)
+    R wup_k = sh_up(w_k);    // w(i-1, j, k)
+    for (; k <= m.k1; ++k, o += sy) {
+        const int o1 = o + sy;
+        const R u_kp = u[o1 + tc], v_kp = v[o1 + tc], w_kp = w[o1 + tc];
+        const R ujm_n = u[o1 + tjm], ujp_n = u[o1 + tjp];
+        const R vjm_n = v[o1 + tjm], vjp_n = v[o1 + tjp];
+        const R wjm_n = w[o1 + tjm], wjp_n = w[o1 + tjp];
+        const R f = F[o + tc];
+        const R uup = sh_up(u_k), udn = sh_dn(u_k);
+        const R vup = sh_up(v_k), vdn = sh_dn(v_k);
+        const R wdn = sh_dn(w_k);
+        const R vjp_up = sh_up(vjp);   // v(i-1, j+1, k)
+        const R ujm_dn = sh_dn(ujm);   // u(i+1, j-1, k)
+        const R wup_kp = sh_up(w_kp);  // w(i-1, j, k+1)
+        if (out) {
+            const R n0 = nu_of(I, f);
+            const int c = o + tc;
+            if (i >= 2) {  // u* (:253-284)
+                const R vh = R(0.25) * (vup + vjp_up + v_k + vjp);
+                const R wh = R(0.25) * (wup_k + wup_kp + w_k + w_kp);
+                us[c] = mom(I, u_k, uup, udn, ujm, ujp, u_km, u_kp, u_k, vh, wh, n0, I.gx);
+            }
+            if (j >= 2) {  // v* (:310-341)
+                const R uh = R(0.25) * (ujm + u_k + ujm_dn + udn);
+                const R wh = R(0.25) * (wjm_n + wjm + w_k + w_kp);
+                vs[c] = mom(I, v_k, vup, vdn, vjm, vjp, v_km, v_kp, uh, v_k, wh, n0, I.gy);
+            }
+            if (k >= 2) {  // w* (:367-398)
+                const R uh = R(0.25) * (udn_km + u_km + udn + u_k);
+                const R vh = R(0.25) * (vjp_km + v_km + v_k + vjp);
+                ws[c] = mom(I, w_k, wup_k, wdn, wjm, wjp, w_km, w_kp, uh, vh, w_k, n0, I.gz);
+            }
+        }
+        u_km = u_k; u_k = u_kp;
+        v_km = v_k; v_k = v_kp;
+        w_km = w_k; w_k = w_kp;
+        ujm = ujm_n; ujp = ujp_n;
+        vjp_km = vjp; vjm = vjm_n; vjp = vjp_n;
+        wjm = wjm_n; wjp = wjp_n;
+        udn_km = udn; wup_k = wup_kp;
     }
 }
 
@@ -129,19 +163,26 @@ template <class R>
 __global__ __launch_bounds__(256) void k_rhs(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
                                              int ninst, int fsel, int kz) {
     VOF_PROLOGUE(R)
-    Col cm;
-    if (!col_map(I, lb, kz, 1, I.nz, cm)) return;
-    const int i = cm.i, j = cm.j;
-    if (i < 1 || i > I.nx || j < 1 || j > I.ny) return;
+    WMap m;
+    if (!wmap<R, 1>(I, lb, kz, 1, I.nz, m)) return;
+    const int i = m.i, j = m.j, sx = I.sx, sy = I.sy;
+    const int P = sx * (I.ny + 2);
+    const bool out = m.own && i >= 1 && i <= I.nx && j >= 1 && j <= I.ny;
+    const int tc = m.tc, tjp = min(tc + sx, P - 1);
     const R* __restrict__ F = I.F[fsel];
     const R* __restrict__ us = I.us;
     const R* __restrict__ vs = I.vs;
     const R* __restrict__ ws = I.ws;
-    for (int k = cm.k0; k <= cm.k1; ++k) {
-        const int c = k * I.sy + j * I.sx + i;
-        const R rho = rho_of(I, F[c]);
-        I.rhs[c] = rho / I.dt *
-                   ((us[c + 1] - us[c]) * I.dxi + (vs[c + I.sx] - vs[c]) * I.dyi + (ws[c + I.sy] - ws[c]) * I.dzi);
+    R* __restrict__ rhs = I.rhs;
+    int o = m.k0 * sy;
+    R ws_k = ws[o + tc];
+    for (int k = m.k0; k <= m.k1; ++k, o += sy) {
+        const R usk = us[o + tc], vsk = vs[o + tc], vsp = vs[o + tjp], ws_kp = ws[o + sy + tc];
+        const R f = F[o + tc];
+        const R usp = sh_dn(usk);
+        if (out)
+            rhs[o + tc] = rho_of(I, f) / I.dt * ((usp - usk) * I.dxi + (vsp - vsk) * I.dyi + (ws_kp - ws_k) * I.dzi);
+        ws_k = ws_kp;
     }
 }
 
@@ -150,33 +191,35 @@ __global__ __launch_bounds__(256) void k_rhs(const Inst<R>* __restrict__ tab, co
 // Boundary neighbours carry a zero coefficient times the (zero) ghost p, kept
 // in the expression so the rounding sequence is the reference's.
 template <class R>
-__device__ __forceinline__ R jacobi_cell(const Inst<R>& I, const R* __restrict__ p, R rhs, int c, int i, int j,
-                                         int k) {
+__global__ __launch_bounds__(256) void k_jacobi(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
+                                                int ninst, int psrc, int kz) {
+    VOF_PROLOGUE(R)
+    WMap m;
+    if (!wmap<R, 1>(I, lb, kz, 1, I.nz, m)) return;
+    const int i = m.i, j = m.j, sx = I.sx, sy = I.sy;
+    const int P = sx * (I.ny + 2);
+    const bool out = m.own && i >= 1 && i <= I.nx && j >= 1 && j <= I.ny;
+    const int tc = m.tc, tjm = max(tc - sx, 0), tjp = min(tc + sx, P - 1);
+    const R* __restrict__ p = I.p[psrc];
+    R* __restrict__ pt = I.p[psrc ^ 1];
+    const R* __restrict__ rhs = I.rhs;
     const R ae = (i != I.nx) ? I.axx : R(0);
     const R aw = (i != 1) ? I.axx : R(0);
     const R an = (j != I.ny) ? I.ayy : R(0);
     const R as = (j != 1) ? I.ayy : R(0);
-    const R af = (k != I.nz) ? I.azz : R(0);
-    const R ab = (k != 1) ? I.azz : R(0);
-    const R ap = -(ae + aw + an + as + af + ab);
-    return (rhs - ae * p[c + 1] - aw * p[c - 1] - an * p[c + I.sx] - as * p[c - I.sx] - af * p[c + I.sy] -
-            ab * p[c - I.sy]) / ap;
-}
-
-template <class R>
-__global__ __launch_bounds__(256) void k_jacobi(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
-                                                int ninst, int psrc, int kz) {
-    VOF_PROLOGUE(R)
-    Col cm;
-    if (!col_map(I, lb, kz, 1, I.nz, cm)) return;
-    const int i = cm.i, j = cm.j;
-    if (i < 1 || i > I.nx || j < 1 || j > I.ny) return;
-    const R* __restrict__ p = I.p[psrc];
-    R* __restrict__ pt = I.p[psrc ^ 1];
-    const R* __restrict__ rhs = I.rhs;
-    for (int k = cm.k0; k <= cm.k1; ++k) {
-        const int c = k * I.sy + j * I.sx + i;
-        pt[c] = jacobi_cell(I, p, rhs[c], c, i, j, k);
+    int o = m.k0 * sy;
+    R p_km = p[o - sy + tc], p_k = p[o + tc];
+    for (int k = m.k0; k <= m.k1; ++k, o += sy) {
+        const R p_kp = p[o + sy + tc], pjm = p[o + tjm], pjp = p[o + tjp], b = rhs[o + tc];
+        const R pim = sh_up(p_k), pip = sh_dn(p_k);
+        if (out) {
+            const R af = (k != I.nz) ? I.azz : R(0);
+            const R ab = (k != 1) ? I.azz : R(0);
+            const R ap = -(ae + aw + an + as + af + ab);
+            pt[o + tc] = (b - ae * pip - aw * pim - an * pjp - as * pjm - af * p_kp - ab * p_km) / ap;
+        }
+        p_km = p_k;
+        p_k = p_kp;
     }
 }
 
@@ -189,55 +232,76 @@ __global__ __launch_bounds__(256) void k_jacobi(const Inst<R>* __restrict__ tab,
 //            = u_corr(i, cj, ck)        for 2 <= i <= nx,
 // with cj = clamp(j,1,ny), ck = clamp(k,1,nz) — the tangential ghosts copy
 // the nearest interior face (zero gradient).  v and w are the same with the
-// roles of the axes exchanged.  The work-item of every stored cell writes its
-// own final value, so no pass ordering is needed.
+// roles of the axes exchanged.  Each lane loads at its row clamped into the
+// interior, x-ghost lanes take their neighbour lane's value, and the lanes of
+// the first/last interior plane also write ghost planes 0 / nz+1.
 template <class R>
-__device__ __forceinline__ R corr(const Inst<R>& I, const R* __restrict__ F, const R* __restrict__ st,
-                                  const R* __restrict__ p, int c, int off, R hi) {
-    const R r = R(0.5) * (rho_of(I, F[c]) + rho_of(I, F[c - off]));
-    return st[c] - I.dt / r * (p[c] - p[c - off]) * hi;
+__device__ __forceinline__ R corr(const Inst<R>& I, R st, R r0, R r1, R p0, R p1, R hi) {
+    const R r = R(0.5) * (r0 + r1);
+    return st - I.dt / r * (p0 - p1) * hi;
 }
 
 template <class R>
 __global__ __launch_bounds__(256) void k_correct(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
                                                  int ninst, int fsel, int psel, int kz) {
     VOF_PROLOGUE(R)
-    Col cm;
-    if (!col_map(I, lb, kz, 0, I.nz + 1, cm)) return;
-    const int i = cm.i, j = cm.j;
-    const int nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
+    WMap m;
+    if (!wmap<R, 1>(I, lb, kz, 1, I.nz, m)) return;
+    const int i = m.i, j = m.j, nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
+    const int cj = min(max(j, 1), ny), ci = min(max(i, 1), nx);
+    const bool iin = (i >= 1 && i <= nx), jin = (j >= 1 && j <= ny);
+    const int ts = cj * sx + i;  // load position: row clamped into the interior
+    const int tsm = ts - sx;
     const R* __restrict__ F = I.F[fsel];
     const R* __restrict__ p = I.p[psel];
-    const int ci = min(max(i, 1), nx), cj = min(max(j, 1), ny);
-    const bool jin = (j >= 1 && j <= ny), iin = (i >= 1 && i <= nx);
-    for (int k = cm.k0; k <= cm.k1; ++k) {
-        const int ck = min(max(k, 1), nz);
-        const bool kin = (k >= 1 && k <= nz);
-        const int c = k * sy + j * sx + i;
-        // u
-        if (i == 1 || i == nx + 1) {
-            I.u[c] = R(0);
-        } else if (i == 0) {
-            if (!(jin && kin)) I.u[c] = I.u[ck * sy + cj * sx];
-        } else {
-            I.u[c] = corr(I, F, I.us, p, ck * sy + cj * sx + i, 1, I.dxi);
+    const R* __restrict__ us = I.us;
+    const R* __restrict__ vs = I.vs;
+    const R* __restrict__ ws = I.ws;
+    R* __restrict__ u = I.u;
+    R* __restrict__ v = I.v;
+    R* __restrict__ w = I.w;
+    const int t = m.t;
+    int o = m.k0 * sy;
+    R p_km = p[o - sy + ts];
+    R r_km = rho_of(I, F[o - sy + ts]);
+    for (int k = m.k0; k <= m.k1; ++k, o += sy) {
+        const R pk = p[o + ts], pjm = p[o + tsm];
+        const R rk = rho_of(I, F[o + ts]), rjm = rho_of(I, F[o + tsm]);
+        const R usk = us[o + ts], vsk = vs[o + ts], wsk = ws[o + ts];
+        const R pim = sh_up(pk), rim = sh_up(rk);
+        const R uc = corr(I, usk, rk, rim, pk, pim, I.dxi);
+        const R vc = corr(I, vsk, rk, rjm, pk, pjm, I.dyi);
+        const R wc = corr(I, wsk, rk, r_km, pk, p_km, I.dzi);
+        const R v_r = sh_dn(vc), v_l = sh_up(vc), w_r = sh_dn(wc), w_l = sh_up(wc);
+        if (m.own) {
+            // u: walls at i = 1, nx+1; plane i = 0 keeps/copies its own values
+            R uval = R(0);
+            bool uw = true;
+            if (i >= 2 && i <= nx) uval = uc;
+            else if (i == 0) { uw = !jin; if (uw) uval = u[o + ts]; }
+            if (uw) u[o + t] = uval;
+            // v: walls at j = 1, ny+1; row j = 0 keeps/copies; x ghosts copy the neighbour
+            R vval = R(0);
+            bool vw = true;
+            if (j >= 2 && j <= ny) vval = iin ? vc : (i == 0 ? v_r : v_l);
+            else if (j == 0) { vw = !iin; if (vw) vval = v[o + ci]; }
+            if (vw) v[o + t] = vval;
+            // w: wall at k = 1
+            const R wval = (k == 1) ? R(0) : (iin ? wc : (i == 0 ? w_r : w_l));
+            w[o + t] = wval;
+            if (k == 1) {  // ghost plane 0: u, v copy plane 1; w copies its own plane-0 interior
+                u[t] = (i == 0) ? u[sy + ts] : uval;
+                v[t] = (j == 0) ? v[sy + ci] : vval;
+                if (!(iin && jin)) w[t] = w[cj * sx + ci];
+            }
+            if (k == nz) {  // ghost plane nz+1: u, v copy plane nz; w wall
+                u[o + sy + t] = (i == 0) ? u[o + ts] : uval;
+                v[o + sy + t] = (j == 0) ? v[o + ci] : vval;
+                w[o + sy + t] = R(0);
+            }
         }
-        // v
-        if (j == 1 || j == ny + 1) {
-            I.v[c] = R(0);
-        } else if (j == 0) {
-            if (!(iin && kin)) I.v[c] = I.v[ck * sy + ci];
-        } else {
-            I.v[c] = corr(I, F, I.vs, p, ck * sy + j * sx + ci, sx, I.dyi);
-        }
-        // w
-        if (k == 1 || k == nz + 1) {
-            I.w[c] = R(0);
-        } else if (k == 0) {
-            if (!(iin && jin)) I.w[c] = I.w[cj * sx + ci];
-        } else {
-            I.w[c] = corr(I, F, I.ws, p, k * sy + cj * sx + ci, sy, I.dzi);
-        }
+        p_km = pk;
+        r_km = rk;
     }
 }
 
@@ -260,6 +324,12 @@ __global__ __launch_bounds__(256) void k_correct(const Inst<R>* __restrict__ tab
 //   stage 1   x: (((d*dy)*dz)/vol)     y: ((d*dy)/(dx*dy))    z: (((d*dy)*dx)/vol)
 //   Q factor  x: dx   y: dx   z: dz   (:579, :718, :853)
 //   stage 4   x: /dy  y: /dy  z: /dz  (:643, :782, :902)
+//
+// The last sweep of a step (fin != 0) also performs post_process_f (:502-507),
+// a second clamp01 (not idempotent: clamp01 maps the tiny negatives it can
+// leave, -x, to +x), and apply_bc_F (:1122-1135), whose three passes leave every
+// ghost equal to the nearest interior cell: the lane of a boundary cell writes
+// its value to every ghost cell that clamps onto it.
 
 template <class R, int DIR>
 __device__ __forceinline__ R fct_stage1_scale(const Inst<R>& I, R d) {
@@ -268,72 +338,87 @@ __device__ __forceinline__ R fct_stage1_scale(const Inst<R>& I, R d) {
     return d * I.dy * I.dx / I.vol;
 }
 
+template <class R>
+__device__ __forceinline__ void store_F(const Inst<R>& I, R* __restrict__ Fo, int i, int j, int k, R val,
+                                        bool ghosts) {
+    const int sx = I.sx, sy = I.sy;
+    Fo[k * sy + j * sx + i] = val;
+    if (!ghosts) return;
+    if (!(i == 1 || i == I.nx || j == 1 || j == I.ny || k == 1 || k == I.nz)) return;
+    int ii[3], jj[3], kk[3], ni = 0, nj = 0, nk = 0;
+    ii[ni++] = i; if (i == 1) ii[ni++] = 0; if (i == I.nx) ii[ni++] = I.nx + 1;
+    jj[nj++] = j; if (j == 1) jj[nj++] = 0; if (j == I.ny) jj[nj++] = I.ny + 1;
+    kk[nk++] = k; if (k == 1) kk[nk++] = 0; if (k == I.nz) kk[nk++] = I.nz + 1;
+    for (int a = 0; a < ni; ++a)
+        for (int b = 0; b < nj; ++b)
+            for (int c = 0; c < nk; ++c)
+                if (a | b | c) Fo[kk[c] * sy + jj[b] * sx + ii[a]] = val;
+}
+
 // Gas cells stay exactly zero: if F vanishes on the whole 7-cell stencil of a
 // cell, every flux is a signed zero, Ftd = +0, rp = rm = 0 and Fnew = clamp01(+-0)
 // = +0, so the divisions are skipped (wave-uniformly) with identical results.
 
-// x sweep: lanes run along x over the flattened (x,y) plane of one k slab.
-// Neighbour values move across lanes (ds_bpermute shuffles); a wave of 64 lanes
-// covers 64 consecutive cells and stores the middle 58, which have their whole
-// +-3 stencil inside the wave.  Row ends need no care: ghost lanes carry the
-// reference's zero conventions, which decouple consecutive rows.
+// x sweep: lanes run along x over the flattened (x,y) plane (overlapped waves,
+// halo 3: a wave stores its middle 58 lanes, whose whole +-3 stencil is inside
+// the wave); each wave walks kz planes.  Row ends need no care: ghost lanes
+// carry the reference's zero conventions, which decouple consecutive rows.
 template <class R>
 __global__ __launch_bounds__(256) void k_fct_x(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
-                                               int ninst, int src) {
+                                               int ninst, int src, int kz, int fin) {
     VOF_PROLOGUE(R)
-    const int P = I.sx * (I.ny + 2);
-    const int wpp = (P + 57) / 58;
-    const int bpp = (wpp + 3) >> 2;
-    const int k = 1 + lb / bpp;
-    const int wv = (lb % bpp) * 4 + ((int)threadIdx.x >> 6);
-    const int lane = (int)threadIdx.x & 63;
-    const int t = wv * 58 - 3 + lane;
-    const bool inpl = (t >= 0 && t < P);
-    const int tc = t < 0 ? 0 : (t >= P ? P - 1 : t);
-    const int j = tc / I.sx, i = tc - j * I.sx;
-    const bool cell = inpl && i >= 1 && i <= I.nx && j >= 1 && j <= I.ny;
-    const bool store = cell && lane >= 3 && lane <= 60;
-    const int c = k * I.sy + tc;
-    const R f0 = I.F[src][c];
-    const R u0 = I.u[c];
+    WMap m;
+    if (!wmap<R, 3>(I, lb, kz, 1, I.nz, m)) return;
+    const int i = m.i, j = m.j;
+    const bool cell = (m.t == m.tc) && i >= 1 && i <= I.nx && j >= 1 && j <= I.ny;
+    const bool store = cell && m.own;
+    const R* __restrict__ Fi = I.F[src];
     R* __restrict__ Fo = I.F[src ^ 1];
-    if (__all(f0 == R(0))) {
-        if (store) Fo[c] = R(0);
-        return;
+    const R* __restrict__ u = I.u;
+    for (int k = m.k0; k <= m.k1; ++k) {
+        const int c = k * I.sy + m.tc;
+        const R f0 = Fi[c];
+        const R u0 = u[c];
+        if (__all(f0 == R(0))) {
+            if (store) store_F(I, Fo, i, j, k, R(0), fin != 0);
+            continue;
+        }
+        const R fm = sh_up(f0);
+        const R up = sh_dn(u0);
+        const R vdt = u0 * I.dt;
+        const R L0 = (u0 >= R(0)) ? vdt * fm : vdt * f0;
+        const R H0 = (u0 <= R(0)) ? vdt * fm : vdt * f0;
+        const R a0 = H0 - L0;
+        const R L1 = sh_dn(L0);
+        const R a1 = sh_dn(a0);
+        const R dv = I.vol - I.dtdydz * (up - u0);
+        R ftd = R(0);
+        if (cell) {
+            R oo = (f0 + fct_stage1_scale<R, 0>(I, L0 - L1)) * I.vol / dv;
+            if (oo > R(1) || oo < R(0)) oo = clamp01(oo);
+            ftd = oo;
+        }
+        const R ftm = sh_up(ftd);
+        const R ftp = sh_dn(ftd);
+        R rp = R(0), rm = R(0);
+        if (cell) {
+            const R fmx = vmax(ftd, vmax(ftm, ftp));
+            const R fmn = vmin(ftd, vmin(ftm, ftp));
+            const R pp = vmax(R(0), a0) - vmin(R(0), a1);
+            if (pp > R(0)) rp = vmin(R(1), (fmx - ftd) * I.dx / pp);
+            const R pm = vmax(R(0), a1) - vmin(R(0), a0);
+            if (pm > R(0)) rm = vmin(R(1), (ftd - fmn) * I.dx / pm);
+        }
+        const R rp1 = sh_dn(rp);
+        const R rm1 = sh_dn(rm);
+        const R c1 = cell ? ((a1 >= R(0)) ? vmin(rp1, rm) : vmin(rp, rm1)) : R(0);
+        const R ac1 = a1 * c1;
+        const R ac0 = sh_up(ac1);
+        if (store) {
+            const R fn = clamp01(ftd - ((ac1 - ac0) / I.dy) * I.vol / dv);
+            store_F(I, Fo, i, j, k, fin ? clamp01(fn) : fn, fin != 0);
+        }
     }
-    const R fm = __shfl_up(f0, 1);
-    const R fp = __shfl_down(f0, 1);
-    const R up = __shfl_down(u0, 1);
-    const R vdt = u0 * I.dt;
-    const R L0 = (u0 >= R(0)) ? vdt * fm : vdt * f0;
-    const R H0 = (u0 <= R(0)) ? vdt * fm : vdt * f0;
-    const R a0 = H0 - L0;
-    const R L1 = __shfl_down(L0, 1);
-    const R a1 = __shfl_down(a0, 1);
-    const R dv = I.vol - I.dtdydz * (up - u0);
-    R ftd = R(0);
-    if (cell) {
-        R o = (f0 + fct_stage1_scale<R, 0>(I, L0 - L1)) * I.vol / dv;
-        if (o > R(1) || o < R(0)) o = clamp01(o);
-        ftd = o;
-    }
-    const R ftm = __shfl_up(ftd, 1);
-    const R ftp = __shfl_down(ftd, 1);
-    R rp = R(0), rm = R(0);
-    if (cell) {
-        const R fmx = vmax(ftd, vmax(ftm, ftp));
-        const R fmn = vmin(ftd, vmin(ftm, ftp));
-        const R pp = vmax(R(0), a0) - vmin(R(0), a1);
-        if (pp > R(0)) rp = vmin(R(1), (fmx - ftd) * I.dx / pp);
-        const R pm = vmax(R(0), a1) - vmin(R(0), a0);
-        if (pm > R(0)) rm = vmin(R(1), (ftd - fmn) * I.dx / pm);
-    }
-    const R rp1 = __shfl_down(rp, 1);
-    const R rm1 = __shfl_down(rm, 1);
-    const R c1 = cell ? ((a1 >= R(0)) ? vmin(rp1, rm) : vmin(rp, rm1)) : R(0);
-    const R ac1 = a1 * c1;
-    const R ac0 = __shfl_up(ac1, 1);
-    if (store) Fo[c] = clamp01(ftd - ((ac1 - ac0) / I.dy) * I.vol / dv);
 }
 
 // y and z sweeps: lanes run along x (coalesced), each lane marches along the
@@ -342,7 +427,7 @@ __global__ __launch_bounds__(256) void k_fct_x(const Inst<R>* __restrict__ tab, 
 // warm-up, so that there are enough waves to cover memory latency.
 template <class R, int DIR>
 __global__ __launch_bounds__(256) void k_fct_march(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
-                                                   int ninst, int src, int ch) {
+                                                   int ninst, int src, int ch, int fin) {
     VOF_PROLOGUE(R)
     const int nd = (DIR == 1) ? I.ny : I.nz;
     const int no = (DIR == 1) ? I.nz : I.ny;
@@ -357,15 +442,20 @@ __global__ __launch_bounds__(256) void k_fct_march(const Inst<R>* __restrict__ t
     const int S = (DIR == 1) ? I.sx : I.sy;
     const int base = (DIR == 1) ? (o * I.sy + i) : (o * I.sx + i);
     const R* __restrict__ F = I.F[src] + base;
-    R* __restrict__ Fo = I.F[src ^ 1] + base;
+    R* __restrict__ Fo = I.F[src ^ 1];
     const R* __restrict__ V = ((DIR == 1) ? I.v : I.w) + base;
     const R kdv = (DIR == 1) ? I.dtdxdz : I.dtdxdy;
     const R qf = (DIR == 1) ? I.dx : I.dz;
     const R s4 = (DIR == 1) ? I.dy : I.dz;
     const R dt = I.dt, vol = I.vol;
+    const bool gh = fin != 0;
     const int m0 = 1 + chn * ch;
     const int m1 = min(nd, m0 + ch - 1);
-    auto cl = [nd](int m) { return m < 0 ? 0 : (m > nd + 1 ? nd + 1 : m); };
+    auto cl = [nd](int mm) { return mm < 0 ? 0 : (mm > nd + 1 ? nd + 1 : mm); };
+    auto put = [&](int mm, R val) {
+        if (DIR == 1) store_F(I, Fo, i, mm, o, val, gh);
+        else store_F(I, Fo, i, o, mm, val, gh);
+    };
 
     int c = m0 - 2;
     const R Fcm = F[cl(c - 1) * S];
@@ -383,10 +473,14 @@ __global__ __launch_bounds__(256) void k_fct_march(const Inst<R>* __restrict__ t
     R rpm1 = R(0), rmm1 = R(0), rpm2 = R(0), rmm2 = R(0), cm2 = R(0);
     R Fn = F[cl(c + 1) * S];
     R Vn = V[cl(c + 1) * S];
+    R Fn2 = F[cl(c + 2) * S];
+    R Vn2 = V[cl(c + 2) * S];
     for (; c <= m1 + 2; ++c) {
         const R Fcp = Fn, Vcp = Vn;
-        Fn = F[cl(c + 2) * S];
-        Vn = V[cl(c + 2) * S];
+        Fn = Fn2;
+        Vn = Vn2;
+        Fn2 = F[cl(c + 3) * S];
+        Vn2 = V[cl(c + 3) * S];
         zr = (Fcp == R(0)) ? zr + 1 : 0;
         const R dvc = vol - kdv * (Vcp - Vc);
         R Lcp, acp, ftc, cm1;
@@ -394,7 +488,7 @@ __global__ __launch_bounds__(256) void k_fct_march(const Inst<R>* __restrict__ t
             // F(c-5..c+1) all zero: every value formed this step is (signed) zero.
             Lcp = R(0); acp = R(0); ftc = R(0);
             rpm1 = R(0); rmm1 = R(0); cm1 = R(0);
-            if (c - 2 >= m0) Fo[(c - 2) * S] = R(0);
+            if (c - 2 >= m0) put(c - 2, R(0));
         } else {
             const R vdt = Vcp * dt;
             Lcp = (Vcp >= R(0)) ? vdt * Fc : vdt * Fcp;
@@ -420,7 +514,10 @@ __global__ __launch_bounds__(256) void k_fct_march(const Inst<R>* __restrict__ t
             cm1 = R(0);
             if (c - 1 >= 2 && c - 1 <= nd + 1) cm1 = (am1 >= R(0)) ? vmin(rpm1, rmm2) : vmin(rpm2, rmm1);
             // corrected value of cell c-2
-            if (c - 2 >= m0) Fo[(c - 2) * S] = clamp01(ftm2 - ((am1 * cm1 - am2 * cm2) / s4) * vol / dvm2);
+            if (c - 2 >= m0) {
+                const R fn = clamp01(ftm2 - ((am1 * cm1 - am2 * cm2) / s4) * vol / dvm2);
+                put(c - 2, gh ? clamp01(fn) : fn);
+            }
         }
         // shift the window
         Fc = Fcp; Vc = Vcp; Lc = Lcp;
@@ -433,13 +530,10 @@ __global__ __launch_bounds__(256) void k_fct_march(const Inst<R>* __restrict__ t
 }
 
 // ---------------------------------------------------------------------------
-// F ghost layer (apply_bc_F, :1122-1135 -> set_bc_F_{j,i,k}, :1053-1105).  The
-// three passes leave every ghost equal to the nearest interior cell,
-// F(clamp(i), clamp(j), clamp(k)); this kernel writes that directly, each
-// ghost once: k faces (full planes), then j faces and i faces of the interior
-// slabs.  post_process_f (:502-507) is the identity here: every interior value
-// already came out of clamp01 in the last sweep, and clamp01 is idempotent on
-// its own outputs; ghosts are overwritten.
+// F ghost layer on its own (after an IC upload or a single test stage): every
+// ghost equals the nearest interior cell, F(clamp(i), clamp(j), clamp(k)),
+// written once per ghost: k faces (full planes), then j faces and i faces of
+// the interior slabs.
 template <class R>
 __global__ __launch_bounds__(256) void k_ghost_F(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
                                                  int ninst, int fsel) {
@@ -468,6 +562,22 @@ __global__ __launch_bounds__(256) void k_ghost_F(const Inst<R>* __restrict__ tab
     F[k * sy + j * sx + i] = F[ck * sy + cj * sx + ci];
 }
 
+// post_process_f (:502-507) on interior cells, for stage-by-stage tests; in a
+// step it is folded into the store of the last FCT sweep.
+template <class R>
+__global__ __launch_bounds__(256) void k_clamp_F(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
+                                                 int ninst, int fsel, int kz) {
+    VOF_PROLOGUE(R)
+    WMap m;
+    if (!wmap<R, 1>(I, lb, kz, 1, I.nz, m)) return;
+    if (!(m.own && m.i >= 1 && m.i <= I.nx && m.j >= 1 && m.j <= I.ny)) return;
+    R* __restrict__ F = I.F[fsel];
+    for (int k = m.k0; k <= m.k1; ++k) {
+        const int c = k * I.sy + m.tc;
+        F[c] = clamp01(F[c]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Mean-pressure removal (extension; no reference counterpart): subtract the
 // FP64 interior mean, stored by the reduction, from interior p.
@@ -475,15 +585,14 @@ template <class R>
 __global__ __launch_bounds__(256) void k_sub_mean(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
                                                   int ninst, int psel, const double* __restrict__ means, int kz) {
     VOF_PROLOGUE(R)
-    Col cm;
-    if (!col_map(I, lb, kz, 1, I.nz, cm)) return;
-    const int i = cm.i, j = cm.j;
-    if (i < 1 || i > I.nx || j < 1 || j > I.ny) return;
-    const R m = R(means[inst_]);
+    WMap m;
+    if (!wmap<R, 1>(I, lb, kz, 1, I.nz, m)) return;
+    if (!(m.own && m.i >= 1 && m.i <= I.nx && m.j >= 1 && m.j <= I.ny)) return;
+    const R mean = R(means[inst_]);
     R* __restrict__ p = I.p[psel];
-    for (int k = cm.k0; k <= cm.k1; ++k) {
-        const int c = k * I.sy + j * I.sx + i;
-        p[c] = p[c] - m;
+    for (int k = m.k0; k <= m.k1; ++k) {
+        const int c = k * I.sy + m.tc;
+        p[c] = p[c] - mean;
     }
 }
 
@@ -517,15 +626,15 @@ __global__ __launch_bounds__(256) void k_reduce1(const Inst<R>* __restrict__ tab
     const int lb = b_ - boff[inst_];
     const Inst<R>& I = tab[inst_];
     double s[NRED] = {0.0, 0.0, 0.0, 0.0, 1e300, -1e300, 0.0};
-    Col cm;
-    if (col_map(I, lb, kz, 0, I.nz + 1, cm)) {
-        const int i = cm.i, j = cm.j;
+    WMap m;
+    if (wmap<R, 0>(I, lb, kz, 0, I.nz + 1, m) && m.own) {
+        const int i = m.i, j = m.j;
         const R* __restrict__ F = I.F[fsel];
         const R* __restrict__ p = I.p[psel];
         const bool ij_in = (i >= 1 && i <= I.nx && j >= 1 && j <= I.ny);
         const double ax = double(I.axx), ay = double(I.ayy), az = double(I.azz);
-        for (int k = cm.k0; k <= cm.k1; ++k) {
-            const int c = k * I.sy + j * I.sx + i;
+        for (int k = m.k0; k <= m.k1; ++k) {
+            const int c = k * I.sy + m.tc;
             if (mode == 0) {
                 s[6] += (!isfinite(double(F[c]))) + (!isfinite(double(p[c]))) + (!isfinite(double(I.u[c]))) +
                         (!isfinite(double(I.v[c]))) + (!isfinite(double(I.w[c])));
@@ -618,13 +727,19 @@ void launch_correct(const Ens<R>& e, const Geo& g, int fsel, int psel, hipStream
     hipLaunchKernelGGL(k_correct<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, fsel, psel, g.kz);
 }
 template <class R>
-void launch_fct(const Ens<R>& e, const Geo& g, int dir, int src, hipStream_t st) {
+void launch_fct(const Ens<R>& e, const Geo& g, int dir, int src, int fin, hipStream_t st) {
     if (dir == 0)
-        hipLaunchKernelGGL(k_fct_x<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, src);
+        hipLaunchKernelGGL(k_fct_x<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, src, g.kz, fin);
     else if (dir == 1)
-        hipLaunchKernelGGL((k_fct_march<R, 1>), dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, src, g.kz);
+        hipLaunchKernelGGL((k_fct_march<R, 1>), dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, src, g.kz,
+                           fin);
     else
-        hipLaunchKernelGGL((k_fct_march<R, 2>), dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, src, g.kz);
+        hipLaunchKernelGGL((k_fct_march<R, 2>), dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, src, g.kz,
+                           fin);
+}
+template <class R>
+void launch_clamp_F(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st) {
+    hipLaunchKernelGGL(k_clamp_F<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, fsel, g.kz);
 }
 template <class R>
 void launch_ghost_F(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st) {
@@ -647,7 +762,8 @@ void launch_reduce(const Ens<R>& e, const Geo& g, int fsel, int psel, int mode, 
     template void launch_rhs<R>(const Ens<R>&, const Geo&, int, hipStream_t);                            \
     template void launch_jacobi<R>(const Ens<R>&, const Geo&, int, hipStream_t);                         \
     template void launch_correct<R>(const Ens<R>&, const Geo&, int, int, hipStream_t);                   \
-    template void launch_fct<R>(const Ens<R>&, const Geo&, int, int, hipStream_t);                       \
+    template void launch_fct<R>(const Ens<R>&, const Geo&, int, int, int, hipStream_t);                  \
+    template void launch_clamp_F<R>(const Ens<R>&, const Geo&, int, hipStream_t);                        \
     template void launch_ghost_F<R>(const Ens<R>&, const Geo&, int, hipStream_t);                        \
     template void launch_sub_mean<R>(const Ens<R>&, const Geo&, int, const double*, hipStream_t);        \
     template void launch_reduce<R>(const Ens<R>&, const Geo&, int, int, int, double*, const long long*, \

@@ -56,7 +56,7 @@ class VofDiag(ctypes.Structure):
 
 class VofTuning(ctypes.Structure):
     _fields_ = [("kz_predict", c_int32), ("kz_rhs", c_int32), ("kz_jacobi", c_int32), ("kz_correct", c_int32),
-                ("fct_chunk", c_int32), ("jacobi_tblock", c_int32)]
+                ("fct_chunk", c_int32), ("jacobi_tblock", c_int32), ("kz_fct_x", c_int32)]
 
 
 def default_params(**kw) -> VofParams:

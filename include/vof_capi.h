@@ -153,12 +153,12 @@ vof_status vof_sync(vof_solver* s);
 vof_status vof_time_steps(vof_solver* s, int64_t first_istep, int warmup, int steps,
                           int use_graph, double* ms_total, double* stage_ms);
 
-/* Launch-shape tuning (no reference counterpart).  kz_* = planes each lane
- * walks in the column kernels, fct_chunk = cells per lane in the y/z FCT
- * marches, jacobi_tblock = Jacobi sweeps fused per launch (1 or 2).  Values
- * <= 0 keep the current setting.  Changing them drops captured graphs. */
+/* Launch-shape tuning (no reference counterpart).  kz_* = planes each wave
+ * walks in the column kernels and the x sweep, fct_chunk = cells per lane in
+ * the y/z FCT marches, jacobi_tblock = Jacobi sweeps fused per launch.
+ * Values <= 0 keep the current setting.  Changing them drops captured graphs. */
 typedef struct vof_tuning {
-    int32_t kz_predict, kz_rhs, kz_jacobi, kz_correct, fct_chunk, jacobi_tblock;
+    int32_t kz_predict, kz_rhs, kz_jacobi, kz_correct, fct_chunk, jacobi_tblock, kz_fct_x;
 } vof_tuning;
 vof_status vof_set_tuning(vof_solver* s, const vof_tuning* t);
 vof_status vof_get_tuning(const vof_solver* s, vof_tuning* t);

@@ -1,0 +1,120 @@
+"""CPU-only checks of the drop-in boundary: the C ABI library loads and exports
+every function include/vof_capi.h declares, argument validation happens before
+any device call, the VTK writer is byte-identical to the reference's ostream
+writer, and the CLI keeps the reference's flags, usage text and exit codes."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "vof_capi.h")
+
+
+def declared_functions():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:vof_status|void|int64_t|int|const char\s*\*)\s+(vof_\w+)\s*\(", txt,
+                                 re.M)))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for must in ("vof_create", "vof_destroy", "vof_init_ic", "vof_step", "vof_download", "vof_upload",
+                 "vof_diagnostics", "vof_write_vtk", "vof_batch_create", "vof_batch_step"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol(vofmod):
+    lib = vofmod.load_library()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_default_params_are_the_reference_defaults(vofmod):
+    # voftrial1.cu:1258-1283
+    p = vofmod.VofParams()
+    vofmod.load_library().vof_params_default(p)
+    assert (p.nx, p.ny, p.nz, p.precision) == (200, 200, 200, 4)
+    assert (p.Lx, p.Ly, p.Lz) == (0.1, 0.1, 0.1)
+    assert (p.rho_l, p.rho_g, p.nu_l, p.nu_g, p.sigma) == (1000.0, 50.0, 1e-6, 1.5e-5, 0.007)
+    assert (p.gx, p.gy, p.gz, p.dt, p.jacobi_sweeps) == (0.0, -5.0, 0.0, 4e-6, 10)
+    q = vofmod.default_params()
+    assert all(getattr(p, k) == getattr(q, k) for k, _ in vofmod.VofParams._fields_)
+
+
+@pytest.mark.parametrize("bad", [dict(nx=0), dict(precision=3), dict(dt=-1.0), dict(jacobi_sweeps=-1),
+                                 dict(nx=2000, ny=2000, nz=2000)])
+def test_invalid_parameters_rejected_before_device_use(vofmod, bad):
+    with pytest.raises(vofmod.VofError) as e:
+        vofmod.Solver(vofmod.default_params(**bad))
+    assert e.value.status == 1
+
+
+def test_unsupported_flags_are_reported(vofmod):
+    with pytest.raises(vofmod.VofError) as e:
+        vofmod.Solver(vofmod.default_params(flags=vofmod.FLAG_NO_SLIP))
+    assert e.value.status == 4
+
+
+def test_status_strings(vofmod):
+    lib = vofmod.load_library()
+    assert lib.vof_status_str(0) == b"ok"
+    assert lib.vof_status_str(1) == b"invalid argument"
+
+
+def test_vtk_writer_bytes_match_reference_stream_writer(vofmod, orc, tmp_path):
+    rng = np.random.default_rng(0)
+    F = rng.random((6, 5, 4)).astype(np.float32)
+    F[0, 0, :] = [0.0, 1.0, 1e-10, 0.29999995]
+    a, b = tmp_path / "a.vtk", tmp_path / "b.vtk"
+    vofmod.write_vtk_host(str(a), F)
+    orc.write_vtk_float(str(b), F)
+    assert a.read_bytes() == b.read_bytes()
+
+
+def test_vtk_header_known_answer(vofmod, tmp_path):
+    # 202 points per axis: SPACING 1/201 printed at the default precision 6 (:918-928)
+    F = np.zeros((202, 202, 202), dtype=np.float32)
+    a = tmp_path / "h.vtk"
+    vofmod.write_vtk_host(str(a), F)
+    with open(a, "rb") as fh:
+        head = fh.read(300).decode()
+    lines = head.split("\n")
+    assert lines[:10] == ["# vtk DataFile Version 3.0", "3D VOF field", "ASCII", "DATASET STRUCTURED_POINTS",
+                          "DIMENSIONS 202 202 202", "ORIGIN 0 0 0", "SPACING 0.00497512 0.00497512 0.00497512",
+                          "POINT_DATA 8242408", "SCALARS VOF float 1", "LOOKUP_TABLE default"]
+    assert os.path.getsize(a) == len("\n".join(lines[:10]) + "\n") + 2 * 202 ** 3
+
+
+REF_USAGE = ("Usage: ./3dvof_cuda [-ic 1|2|3] [--nstep N] [--max_steps N]\n"
+             "  -ic         Initial condition selector. Only 1 is implemented, exactly as in the source.\n"
+             "  --nstep     Dump interval.\n"
+             "  --max_steps Number of steps to run. 0 means run until Ctrl+C.\n")
+
+
+def cli(*args):
+    import vof
+
+    return subprocess.run([vof.CLI_PATH, *args], capture_output=True, text=True, timeout=60)
+
+
+def test_cli_help_is_the_reference_usage():
+    for flag in ("--help", "-h"):
+        r = cli(flag)
+        assert r.returncode == 0
+        assert r.stdout == REF_USAGE
+
+
+def test_cli_rejects_bad_ic():
+    r = cli("-ic", "4")
+    assert r.returncode == 1
+    assert r.stderr.strip() == "Invalid initial condition. Allowed values: 1, 2, 3"
+
+
+def test_cli_rejects_zero_dump_interval():
+    # the reference divides by zero here (istep % nstep); the drop-in reports it
+    r = cli("--nstep", "0")
+    assert r.returncode == 1
+    assert "nstep" in r.stderr
