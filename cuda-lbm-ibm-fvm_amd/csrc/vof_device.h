@@ -37,6 +37,11 @@ struct Inst {
     R* rhs;   // rho/dt * div(u*), formed once per step
 };
 
+// Pick one of a ping-pong pair with constant indices only, so that a copied
+// Inst stays in registers (a runtime array index would force it to memory).
+template <class T>
+__device__ __forceinline__ T sel2(const T (&a)[2], int s) { return s ? a[1] : a[0]; }
+
 // Work-group -> instance lookup.  boff has ninst+1 entries: boff[i] is the
 // first work-group of instance i for this launch shape, boff[ninst] the total.
 __device__ __forceinline__ int find_inst(const int* __restrict__ boff, int ninst, int b) {
@@ -54,6 +59,60 @@ __device__ __forceinline__ int find_inst(const int* __restrict__ boff, int ninst
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
     const int q = nb >> 3, r = nb & 7, x = b & 7;
     return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+// ---------------------------------------------------------------------------
+// Raw buffer access.  A field is addressed through a 128-bit buffer resource
+// (base, size in bytes): the per-lane offset is a 32-bit VGPR that stays
+// fixed while a lane marches, and the plane offset is a scalar (SGPR)
+// operand, so a load or store costs no per-lane 64-bit address arithmetic.
+// Accesses outside [0, size) return 0 / are dropped by the hardware.
+constexpr int kRsrcWord3 = 0x00020000;  // raw, untyped 32-bit records (gfx9 family)
+
+template <class R>
+struct Buf {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ Buf(const R* p, int n)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<R*>(p), (short)0, n * (int)sizeof(R), kRsrcWord3)) {}
+    // vo: per-lane element offset, so: wave-uniform element offset
+    __device__ __forceinline__ R ld(int vo, int so) const;
+    __device__ __forceinline__ void st(R v, int vo, int so) const;
+};
+
+template <>
+__device__ __forceinline__ float Buf<float>::ld(int vo, int so) const {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, vo * 4, so * 4, 0));
+}
+template <>
+__device__ __forceinline__ void Buf<float>::st(float v, int vo, int so) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, vo * 4, so * 4, 0);
+}
+template <>
+__device__ __forceinline__ double Buf<double>::ld(int vo, int so) const {
+    const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, vo * 8, so * 8, 0);
+    return __hiloint2double((int)x[1], (int)x[0]);
+}
+template <>
+__device__ __forceinline__ void Buf<double>::st(double v, int vo, int so) const {
+    decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)) x;
+    x[0] = (unsigned)__double2loint(v);
+    x[1] = (unsigned)__double2hiint(v);
+    __builtin_amdgcn_raw_buffer_store_b64(x, r, vo * 8, so * 8, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Whole-wave lane shifts by one lane (DPP wave_shr:1 / wave_shl:1, one VALU
+// op each; the value entering at the wave's edge is 0).
+__device__ __forceinline__ int dpp_up_i(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_dn_i(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false); }
+
+__device__ __forceinline__ float lane_up(float x) { return __int_as_float(dpp_up_i(__float_as_int(x))); }  // lane-1
+__device__ __forceinline__ float lane_dn(float x) { return __int_as_float(dpp_dn_i(__float_as_int(x))); }  // lane+1
+__device__ __forceinline__ double lane_up(double x) {
+    return __hiloint2double(dpp_up_i(__double2hiint(x)), dpp_up_i(__double2loint(x)));
+}
+__device__ __forceinline__ double lane_dn(double x) {
+    return __hiloint2double(dpp_dn_i(__double2hiint(x)), dpp_dn_i(__double2loint(x)));
 }
 
 template <class R> __device__ __forceinline__ R vmax(R a, R b);

@@ -24,14 +24,16 @@ namespace vof {
 
 // ---------------------------------------------------------------------------
 // Shared launch prologue: logical work-group, instance and local work-group.
+// The instance record is copied into (scalar) registers: buffer stores are
+// opaque to alias analysis, and a reference would be re-read after each one.
 #define VOF_PROLOGUE(R)                                                   \
     const int b_ = xcd_remap(blockIdx.x, gridDim.x);                      \
     const int inst_ = (ninst == 1) ? 0 : find_inst(boff, ninst, b_);      \
     const int lb = b_ - boff[inst_];                                      \
-    const Inst<R>& I = tab[inst_];
+    const Inst<R> I = tab[inst_];
 
-template <class R> __device__ __forceinline__ R sh_up(R x) { return __shfl_up(x, 1); }    // value of lane-1
-template <class R> __device__ __forceinline__ R sh_dn(R x) { return __shfl_down(x, 1); }  // value of lane+1
+template <class R> __device__ __forceinline__ R sh_up(R x) { return lane_up(x); }  // value of lane-1
+template <class R> __device__ __forceinline__ R sh_dn(R x) { return lane_dn(x); }  // value of lane+1
 
 // Overlapped-wave column mapping.  The (nx+2)(ny+2) plane is flattened x
 // fastest; wave w covers positions [w*S - H, w*S - H + 64) with S = 64 - 2H and
@@ -96,30 +98,35 @@ __global__ __launch_bounds__(256) void k_predict(const Inst<R>* __restrict__ tab
     const bool out = m.own && i >= 1 && i <= I.nx && j >= 1 && j <= I.ny;
     const int tc = m.tc;
     const int tjm = max(tc - sx, 0), tjp = min(tc + sx, P - 1);
-    const R* __restrict__ F = I.F[fsel];
-    const R* __restrict__ u = I.u;
-    const R* __restrict__ v = I.v;
-    const R* __restrict__ w = I.w;
-    R* __restrict__ us = I.us;
-    R* __restrict__ vs = I.vs;
-    R* __restrict__ ws = I.ws;
+    const Buf<R> F(sel2(I.F, fsel), I.ntot), u(I.u, I.ntot), v(I.v, I.ntot), w(I.w, I.ntot);
+    const Buf<R> us(I.us, I.ntot), vs(I.vs, I.ntot), ws(I.ws, I.ntot);
     int k = m.k0;
     int o = k * sy;
-    R u_km = u[o - sy + tc], u_k = u[o + tc];
-    R v_km = v[o - sy + tc], v_k = v[o + tc];
-    R w_km = w[o - sy + tc], w_k = w[o + tc];
-    R ujm = u[o + tjm], ujp = u[o + tjp];
-    R vjm = v[o + tjm], vjp = v[o + tjp], vjp_km = v[o - sy + tjp];
-    R wjm = w[o + tjm], wjp = w[o + tjp];
+    R u_km = u.ld(tc, o - sy), u_k = u.ld(tc, o);
+    R v_km = v.ld(tc, o - sy), v_k = v.ld(tc, o);
+    R w_km = w.ld(tc, o - sy), w_k = w.ld(tc, o);
+    R ujm = u.ld(tjm, o), ujp = u.ld(tjp, o);
+    R vjm = v.ld(tjm, o), vjp = v.ld(tjp, o), vjp_km = v.ld(tjp, o - sy);
+    R wjm = w.ld(tjm, o), wjp = w.ld(tjp, o);
     R udn_km = sh_dn(u_km);  // u(i+1, j, k-1)
     R wup_k = sh_up(w_k);    // w(i-1, j, k)
+    // plane k+1 operands (and F of plane k), fetched one plane ahead
+    const int olim = I.nz * sy;
+    R nu = u.ld(tc, o + sy), nv = v.ld(tc, o + sy), nw = w.ld(tc, o + sy);
+    R nujm = u.ld(tjm, o + sy), nujp = u.ld(tjp, o + sy), nvjm = v.ld(tjm, o + sy), nvjp = v.ld(tjp, o + sy);
+    R nwjm = w.ld(tjm, o + sy), nwjp = w.ld(tjp, o + sy), nf = F.ld(tc, o);
     for (; k <= m.k1; ++k, o += sy) {
-        const int o1 = o + sy;
-        const R u_kp = u[o1 + tc], v_kp = v[o1 + tc], w_kp = w[o1 + tc];
-        const R ujm_n = u[o1 + tjm], ujp_n = u[o1 + tjp];
-        const R vjm_n = v[o1 + tjm], vjp_n = v[o1 + tjp];
-        const R wjm_n = w[o1 + tjm], wjp_n = w[o1 + tjp];
-        const R f = F[o + tc];
+        const R u_kp = nu, v_kp = nv, w_kp = nw;
+        const R ujm_n = nujm, ujp_n = nujp, vjm_n = nvjm, vjp_n = nvjp, wjm_n = nwjm, wjp_n = nwjp;
+        const R f = nf;
+        {
+            const int o2 = min(o + sy, olim), o3 = o2 + sy;
+            nu = u.ld(tc, o3); nv = v.ld(tc, o3); nw = w.ld(tc, o3);
+            nujm = u.ld(tjm, o3); nujp = u.ld(tjp, o3);
+            nvjm = v.ld(tjm, o3); nvjp = v.ld(tjp, o3);
+            nwjm = w.ld(tjm, o3); nwjp = w.ld(tjp, o3);
+            nf = F.ld(tc, o2);
+        }
         const R uup = sh_up(u_k), udn = sh_dn(u_k);
         const R vup = sh_up(v_k), vdn = sh_dn(v_k);
         const R wdn = sh_dn(w_k);
@@ -128,21 +135,20 @@ __global__ __launch_bounds__(256) void k_predict(const Inst<R>* __restrict__ tab
         const R wup_kp = sh_up(w_kp);  // w(i-1, j, k+1)
         if (out) {
             const R n0 = nu_of(I, f);
-            const int c = o + tc;
             if (i >= 2) {  // u* (:253-284)
                 const R vh = R(0.25) * (vup + vjp_up + v_k + vjp);
                 const R wh = R(0.25) * (wup_k + wup_kp + w_k + w_kp);
-                us[c] = mom(I, u_k, uup, udn, ujm, ujp, u_km, u_kp, u_k, vh, wh, n0, I.gx);
+                us.st(mom(I, u_k, uup, udn, ujm, ujp, u_km, u_kp, u_k, vh, wh, n0, I.gx), tc, o);
             }
             if (j >= 2) {  // v* (:310-341)
                 const R uh = R(0.25) * (ujm + u_k + ujm_dn + udn);
                 const R wh = R(0.25) * (wjm_n + wjm + w_k + w_kp);
-                vs[c] = mom(I, v_k, vup, vdn, vjm, vjp, v_km, v_kp, uh, v_k, wh, n0, I.gy);
+                vs.st(mom(I, v_k, vup, vdn, vjm, vjp, v_km, v_kp, uh, v_k, wh, n0, I.gy), tc, o);
             }
             if (k >= 2) {  // w* (:367-398)
                 const R uh = R(0.25) * (udn_km + u_km + udn + u_k);
                 const R vh = R(0.25) * (vjp_km + v_km + v_k + vjp);
-                ws[c] = mom(I, w_k, wup_k, wdn, wjm, wjp, w_km, w_kp, uh, vh, w_k, n0, I.gz);
+                ws.st(mom(I, w_k, wup_k, wdn, wjm, wjp, w_km, w_kp, uh, vh, w_k, n0, I.gz), tc, o);
             }
         }
         u_km = u_k; u_k = u_kp;
@@ -169,19 +175,22 @@ __global__ __launch_bounds__(256) void k_rhs(const Inst<R>* __restrict__ tab, co
     const int P = sx * (I.ny + 2);
     const bool out = m.own && i >= 1 && i <= I.nx && j >= 1 && j <= I.ny;
     const int tc = m.tc, tjp = min(tc + sx, P - 1);
-    const R* __restrict__ F = I.F[fsel];
-    const R* __restrict__ us = I.us;
-    const R* __restrict__ vs = I.vs;
-    const R* __restrict__ ws = I.ws;
-    R* __restrict__ rhs = I.rhs;
+    const Buf<R> F(sel2(I.F, fsel), I.ntot), us(I.us, I.ntot), vs(I.vs, I.ntot), ws(I.ws, I.ntot), rhs(I.rhs, I.ntot);
     int o = m.k0 * sy;
-    R ws_k = ws[o + tc];
+    const int olim = I.nz * sy;
+    R ws_k = ws.ld(tc, o);
+    R n_us = us.ld(tc, o), n_vs = vs.ld(tc, o), n_vsp = vs.ld(tjp, o), n_ws = ws.ld(tc, o + sy), n_f = F.ld(tc, o);
     for (int k = m.k0; k <= m.k1; ++k, o += sy) {
-        const R usk = us[o + tc], vsk = vs[o + tc], vsp = vs[o + tjp], ws_kp = ws[o + sy + tc];
-        const R f = F[o + tc];
+        const R usk = n_us, vsk = n_vs, vsp = n_vsp, ws_kp = n_ws, f = n_f;
+        const int o2 = min(o + sy, olim);
+        n_us = us.ld(tc, o2);
+        n_vs = vs.ld(tc, o2);
+        n_vsp = vs.ld(tjp, o2);
+        n_ws = ws.ld(tc, o2 + sy);
+        n_f = F.ld(tc, o2);
         const R usp = sh_dn(usk);
         if (out)
-            rhs[o + tc] = rho_of(I, f) / I.dt * ((usp - usk) * I.dxi + (vsp - vsk) * I.dyi + (ws_kp - ws_k) * I.dzi);
+            rhs.st(rho_of(I, f) / I.dt * ((usp - usk) * I.dxi + (vsp - vsk) * I.dyi + (ws_kp - ws_k) * I.dzi), tc, o);
         ws_k = ws_kp;
     }
 }
@@ -200,23 +209,29 @@ __global__ __launch_bounds__(256) void k_jacobi(const Inst<R>* __restrict__ tab,
     const int P = sx * (I.ny + 2);
     const bool out = m.own && i >= 1 && i <= I.nx && j >= 1 && j <= I.ny;
     const int tc = m.tc, tjm = max(tc - sx, 0), tjp = min(tc + sx, P - 1);
-    const R* __restrict__ p = I.p[psrc];
-    R* __restrict__ pt = I.p[psrc ^ 1];
-    const R* __restrict__ rhs = I.rhs;
+    const Buf<R> p(sel2(I.p, psrc), I.ntot), pt(sel2(I.p, psrc ^ 1), I.ntot), rhs(I.rhs, I.ntot);
     const R ae = (i != I.nx) ? I.axx : R(0);
     const R aw = (i != 1) ? I.axx : R(0);
     const R an = (j != I.ny) ? I.ayy : R(0);
     const R as = (j != 1) ? I.ayy : R(0);
     int o = m.k0 * sy;
-    R p_km = p[o - sy + tc], p_k = p[o + tc];
+    const int olim = I.nz * sy;  // last plane offset whose +1 neighbour is stored
+    R p_km = p.ld(tc, o - sy), p_k = p.ld(tc, o);
+    // operands of the current plane, fetched one plane ahead
+    R n_kp = p.ld(tc, o + sy), n_jm = p.ld(tjm, o), n_jp = p.ld(tjp, o), n_b = rhs.ld(tc, o);
     for (int k = m.k0; k <= m.k1; ++k, o += sy) {
-        const R p_kp = p[o + sy + tc], pjm = p[o + tjm], pjp = p[o + tjp], b = rhs[o + tc];
+        const R p_kp = n_kp, pjm = n_jm, pjp = n_jp, b = n_b;
+        const int o2 = min(o + sy, olim);
+        n_kp = p.ld(tc, o2 + sy);
+        n_jm = p.ld(tjm, o2);
+        n_jp = p.ld(tjp, o2);
+        n_b = rhs.ld(tc, o2);
         const R pim = sh_up(p_k), pip = sh_dn(p_k);
         if (out) {
             const R af = (k != I.nz) ? I.azz : R(0);
             const R ab = (k != 1) ? I.azz : R(0);
             const R ap = -(ae + aw + an + as + af + ab);
-            pt[o + tc] = (b - ae * pip - aw * pim - an * pjp - as * pjm - af * p_kp - ab * p_km) / ap;
+            pt.st((b - ae * pip - aw * pim - an * pjp - as * pjm - af * p_kp - ab * p_km) / ap, tc, o);
         }
         p_km = p_k;
         p_k = p_kp;
@@ -252,22 +267,27 @@ __global__ __launch_bounds__(256) void k_correct(const Inst<R>* __restrict__ tab
     const bool iin = (i >= 1 && i <= nx), jin = (j >= 1 && j <= ny);
     const int ts = cj * sx + i;  // load position: row clamped into the interior
     const int tsm = ts - sx;
-    const R* __restrict__ F = I.F[fsel];
-    const R* __restrict__ p = I.p[psel];
-    const R* __restrict__ us = I.us;
-    const R* __restrict__ vs = I.vs;
-    const R* __restrict__ ws = I.ws;
-    R* __restrict__ u = I.u;
-    R* __restrict__ v = I.v;
-    R* __restrict__ w = I.w;
+    const Buf<R> F(sel2(I.F, fsel), I.ntot), p(sel2(I.p, psel), I.ntot);
+    const Buf<R> us(I.us, I.ntot), vs(I.vs, I.ntot), ws(I.ws, I.ntot);
+    const Buf<R> u(I.u, I.ntot), v(I.v, I.ntot), w(I.w, I.ntot);
     const int t = m.t;
     int o = m.k0 * sy;
-    R p_km = p[o - sy + ts];
-    R r_km = rho_of(I, F[o - sy + ts]);
+    const int olim = nz * sy;
+    R p_km = p.ld(ts, o - sy);
+    R r_km = rho_of(I, F.ld(ts, o - sy));
+    // operands of plane k, fetched one plane ahead
+    R n_p = p.ld(ts, o), n_pjm = p.ld(tsm, o), n_f = F.ld(ts, o), n_fjm = F.ld(tsm, o);
+    R n_us = us.ld(ts, o), n_vs = vs.ld(ts, o), n_ws = ws.ld(ts, o);
     for (int k = m.k0; k <= m.k1; ++k, o += sy) {
-        const R pk = p[o + ts], pjm = p[o + tsm];
-        const R rk = rho_of(I, F[o + ts]), rjm = rho_of(I, F[o + tsm]);
-        const R usk = us[o + ts], vsk = vs[o + ts], wsk = ws[o + ts];
+        const R pk = n_p, pjm = n_pjm;
+        const R rk = rho_of(I, n_f), rjm = rho_of(I, n_fjm);
+        const R usk = n_us, vsk = n_vs, wsk = n_ws;
+        {
+            const int o2 = min(o + sy, olim);
+            n_p = p.ld(ts, o2); n_pjm = p.ld(tsm, o2);
+            n_f = F.ld(ts, o2); n_fjm = F.ld(tsm, o2);
+            n_us = us.ld(ts, o2); n_vs = vs.ld(ts, o2); n_ws = ws.ld(ts, o2);
+        }
         const R pim = sh_up(pk), rim = sh_up(rk);
         const R uc = corr(I, usk, rk, rim, pk, pim, I.dxi);
         const R vc = corr(I, vsk, rk, rjm, pk, pjm, I.dyi);
@@ -278,26 +298,26 @@ __global__ __launch_bounds__(256) void k_correct(const Inst<R>* __restrict__ tab
             R uval = R(0);
             bool uw = true;
             if (i >= 2 && i <= nx) uval = uc;
-            else if (i == 0) { uw = !jin; if (uw) uval = u[o + ts]; }
-            if (uw) u[o + t] = uval;
+            else if (i == 0) { uw = !jin; if (uw) uval = u.ld(ts, o); }
+            if (uw) u.st(uval, t, o);
             // v: walls at j = 1, ny+1; row j = 0 keeps/copies; x ghosts copy the neighbour
             R vval = R(0);
             bool vw = true;
             if (j >= 2 && j <= ny) vval = iin ? vc : (i == 0 ? v_r : v_l);
-            else if (j == 0) { vw = !iin; if (vw) vval = v[o + ci]; }
-            if (vw) v[o + t] = vval;
+            else if (j == 0) { vw = !iin; if (vw) vval = v.ld(ci, o); }
+            if (vw) v.st(vval, t, o);
             // w: wall at k = 1
             const R wval = (k == 1) ? R(0) : (iin ? wc : (i == 0 ? w_r : w_l));
-            w[o + t] = wval;
+            w.st(wval, t, o);
             if (k == 1) {  // ghost plane 0: u, v copy plane 1; w copies its own plane-0 interior
-                u[t] = (i == 0) ? u[sy + ts] : uval;
-                v[t] = (j == 0) ? v[sy + ci] : vval;
-                if (!(iin && jin)) w[t] = w[cj * sx + ci];
+                u.st((i == 0) ? u.ld(ts, sy) : uval, t, 0);
+                v.st((j == 0) ? v.ld(ci, sy) : vval, t, 0);
+                if (!(iin && jin)) w.st(w.ld(cj * sx + ci, 0), t, 0);
             }
             if (k == nz) {  // ghost plane nz+1: u, v copy plane nz; w wall
-                u[o + sy + t] = (i == 0) ? u[o + ts] : uval;
-                v[o + sy + t] = (j == 0) ? v[o + ci] : vval;
-                w[o + sy + t] = R(0);
+                u.st((i == 0) ? u.ld(ts, o) : uval, t, o + sy);
+                v.st((j == 0) ? v.ld(ci, o) : vval, t, o + sy);
+                w.st(R(0), t, o + sy);
             }
         }
         p_km = pk;
@@ -339,10 +359,10 @@ __device__ __forceinline__ R fct_stage1_scale(const Inst<R>& I, R d) {
 }
 
 template <class R>
-__device__ __forceinline__ void store_F(const Inst<R>& I, R* __restrict__ Fo, int i, int j, int k, R val,
+__device__ __forceinline__ void store_F(const Inst<R>& I, const Buf<R>& Fo, int i, int j, int k, R val,
                                         bool ghosts) {
     const int sx = I.sx, sy = I.sy;
-    Fo[k * sy + j * sx + i] = val;
+    Fo.st(val, j * sx + i, k * sy);
     if (!ghosts) return;
     if (!(i == 1 || i == I.nx || j == 1 || j == I.ny || k == 1 || k == I.nz)) return;
     int ii[3], jj[3], kk[3], ni = 0, nj = 0, nk = 0;
@@ -352,7 +372,7 @@ __device__ __forceinline__ void store_F(const Inst<R>& I, R* __restrict__ Fo, in
     for (int a = 0; a < ni; ++a)
         for (int b = 0; b < nj; ++b)
             for (int c = 0; c < nk; ++c)
-                if (a | b | c) Fo[kk[c] * sy + jj[b] * sx + ii[a]] = val;
+                if (a | b | c) Fo.st(val, jj[b] * sx + ii[a], kk[c] * sy);
 }
 
 // Gas cells stay exactly zero: if F vanishes on the whole 7-cell stencil of a
@@ -372,13 +392,10 @@ __global__ __launch_bounds__(256) void k_fct_x(const Inst<R>* __restrict__ tab, 
     const int i = m.i, j = m.j;
     const bool cell = (m.t == m.tc) && i >= 1 && i <= I.nx && j >= 1 && j <= I.ny;
     const bool store = cell && m.own;
-    const R* __restrict__ Fi = I.F[src];
-    R* __restrict__ Fo = I.F[src ^ 1];
-    const R* __restrict__ u = I.u;
+    const Buf<R> Fi(sel2(I.F, src), I.ntot), Fo(sel2(I.F, src ^ 1), I.ntot), u(I.u, I.ntot);
     for (int k = m.k0; k <= m.k1; ++k) {
-        const int c = k * I.sy + m.tc;
-        const R f0 = Fi[c];
-        const R u0 = u[c];
+        const R f0 = Fi.ld(m.tc, k * I.sy);
+        const R u0 = u.ld(m.tc, k * I.sy);
         if (__all(f0 == R(0))) {
             if (store) store_F(I, Fo, i, j, k, R(0), fin != 0);
             continue;
@@ -441,9 +458,10 @@ __global__ __launch_bounds__(256) void k_fct_march(const Inst<R>* __restrict__ t
     if (i < 1 || i > I.nx) return;
     const int S = (DIR == 1) ? I.sx : I.sy;
     const int base = (DIR == 1) ? (o * I.sy + i) : (o * I.sx + i);
-    const R* __restrict__ F = I.F[src] + base;
-    R* __restrict__ Fo = I.F[src ^ 1];
-    const R* __restrict__ V = ((DIR == 1) ? I.v : I.w) + base;
+    const Buf<R> Fb(sel2(I.F, src), I.ntot), Fo(sel2(I.F, src ^ 1), I.ntot), Vb((DIR == 1) ? I.v : I.w, I.ntot);
+    // line element m of the lane's line: per-lane offset base, scalar offset m*S
+    auto F = [&](int mm) { return Fb.ld(base, mm * S); };
+    auto V = [&](int mm) { return Vb.ld(base, mm * S); };
     const R kdv = (DIR == 1) ? I.dtdxdz : I.dtdxdy;
     const R qf = (DIR == 1) ? I.dx : I.dz;
     const R s4 = (DIR == 1) ? I.dy : I.dz;
@@ -458,9 +476,9 @@ __global__ __launch_bounds__(256) void k_fct_march(const Inst<R>* __restrict__ t
     };
 
     int c = m0 - 2;
-    const R Fcm = F[cl(c - 1) * S];
-    R Fc = F[cl(c) * S];
-    R Vc = V[cl(c) * S];
+    const R Fcm = F(cl(c - 1));
+    R Fc = F(cl(c));
+    R Vc = V(cl(c));
     R Lc, ac;
     {
         const R vdt = Vc * dt;
@@ -471,16 +489,16 @@ __global__ __launch_bounds__(256) void k_fct_march(const Inst<R>* __restrict__ t
     int zr = (Fc == R(0)) ? ((Fcm == R(0)) ? 2 : 1) : 0;
     R am1 = R(0), am2 = R(0), ftm1 = R(0), ftm2 = R(0), dvm1 = R(1), dvm2 = R(1);
     R rpm1 = R(0), rmm1 = R(0), rpm2 = R(0), rmm2 = R(0), cm2 = R(0);
-    R Fn = F[cl(c + 1) * S];
-    R Vn = V[cl(c + 1) * S];
-    R Fn2 = F[cl(c + 2) * S];
-    R Vn2 = V[cl(c + 2) * S];
+    R Fn = F(cl(c + 1));
+    R Vn = V(cl(c + 1));
+    R Fn2 = F(cl(c + 2));
+    R Vn2 = V(cl(c + 2));
     for (; c <= m1 + 2; ++c) {
         const R Fcp = Fn, Vcp = Vn;
         Fn = Fn2;
         Vn = Vn2;
-        Fn2 = F[cl(c + 3) * S];
-        Vn2 = V[cl(c + 3) * S];
+        Fn2 = F(cl(c + 3));
+        Vn2 = V(cl(c + 3));
         zr = (Fcp == R(0)) ? zr + 1 : 0;
         const R dvc = vol - kdv * (Vcp - Vc);
         R Lcp, acp, ftc, cm1;
@@ -557,7 +575,7 @@ __global__ __launch_bounds__(256) void k_ghost_F(const Inst<R>* __restrict__ tab
     } else {
         return;
     }
-    R* __restrict__ F = I.F[fsel];
+    R* __restrict__ F = sel2(I.F, fsel);
     const int ci = min(max(i, 1), nx), cj = min(max(j, 1), ny), ck = min(max(k, 1), nz);
     F[k * sy + j * sx + i] = F[ck * sy + cj * sx + ci];
 }
@@ -571,7 +589,7 @@ __global__ __launch_bounds__(256) void k_clamp_F(const Inst<R>* __restrict__ tab
     WMap m;
     if (!wmap<R, 1>(I, lb, kz, 1, I.nz, m)) return;
     if (!(m.own && m.i >= 1 && m.i <= I.nx && m.j >= 1 && m.j <= I.ny)) return;
-    R* __restrict__ F = I.F[fsel];
+    R* __restrict__ F = sel2(I.F, fsel);
     for (int k = m.k0; k <= m.k1; ++k) {
         const int c = k * I.sy + m.tc;
         F[c] = clamp01(F[c]);
@@ -589,7 +607,7 @@ __global__ __launch_bounds__(256) void k_sub_mean(const Inst<R>* __restrict__ ta
     if (!wmap<R, 1>(I, lb, kz, 1, I.nz, m)) return;
     if (!(m.own && m.i >= 1 && m.i <= I.nx && m.j >= 1 && m.j <= I.ny)) return;
     const R mean = R(means[inst_]);
-    R* __restrict__ p = I.p[psel];
+    R* __restrict__ p = sel2(I.p, psel);
     for (int k = m.k0; k <= m.k1; ++k) {
         const int c = k * I.sy + m.tc;
         p[c] = p[c] - mean;
@@ -629,8 +647,8 @@ __global__ __launch_bounds__(256) void k_reduce1(const Inst<R>* __restrict__ tab
     WMap m;
     if (wmap<R, 0>(I, lb, kz, 0, I.nz + 1, m) && m.own) {
         const int i = m.i, j = m.j;
-        const R* __restrict__ F = I.F[fsel];
-        const R* __restrict__ p = I.p[psel];
+        const R* __restrict__ F = sel2(I.F, fsel);
+        const R* __restrict__ p = sel2(I.p, psel);
         const bool ij_in = (i >= 1 && i <= I.nx && j >= 1 && j <= I.ny);
         const double ax = double(I.axx), ay = double(I.ayy), az = double(I.azz);
         for (int k = m.k0; k <= m.k1; ++k) {
