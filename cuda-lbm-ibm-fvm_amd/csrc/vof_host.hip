@@ -100,8 +100,8 @@ struct Ensemble : EnsBase {
     std::vector<vof_params> prm;
     std::vector<vof::Inst<R>> ih;
     vof::Ens<R> e;
-    vof::Geo g_pred, g_rhs, g_jac, g_corr, g_fx, g_fy, g_fz, g_ghost, g_red, g_sub;
-    vof_tuning tun{8, 8, 8, 8, 32, 1, 4};
+    vof::Geo g_pred, g_rhs, g_jac, g_jac2, g_corr, g_fx, g_fy, g_fz, g_ghost, g_red, g_sub;
+    vof_tuning tun{4, 4, 4, 4, 8, 1, 4};
     int fsel = 0, psel = 0;
     int sweeps = 10;
     uint32_t flags = 0;
@@ -180,6 +180,8 @@ struct Ensemble : EnsBase {
         VOF_TRY(make_geo(g_pred, tun.kz_predict, col1));
         VOF_TRY(make_geo(g_rhs, tun.kz_rhs, col1));
         VOF_TRY(make_geo(g_jac, tun.kz_jacobi, col1));
+        VOF_TRY(make_geo(g_jac2, tun.kz_jacobi, [](const vof::Inst<R>& I, int kz) {
+            return vof::blocks_jacobi2(I.nx, I.ny, I.nz, kz); }));
         VOF_TRY(make_geo(g_sub, 8, col1));
         VOF_TRY(make_geo(g_corr, tun.kz_correct, col1));
         VOF_TRY(make_geo(g_red, 8, [](const vof::Inst<R>& I, int kz) {
@@ -294,14 +296,30 @@ struct Ensemble : EnsBase {
         return VOF_OK;
     }
 
+    // ---- the Jacobi sweeps: pairs fused per launch when jacobi_tblock == 2,
+    // one launch per sweep otherwise; psel follows the launch ping-pong.
+    int jacobi_launches() const {
+        return (tun.jacobi_tblock >= 2) ? (sweeps / 2 + (sweeps & 1)) : sweeps;
+    }
+    void enqueue_jacobi() {
+        int left = sweeps;
+        if (tun.jacobi_tblock >= 2) {
+            for (; left >= 2; left -= 2) {
+                vof::launch_jacobi2(e, g_jac2, psel, st);
+                psel ^= 1;
+            }
+        }
+        for (; left > 0; --left) {
+            vof::launch_jacobi(e, g_jac, psel, st);
+            psel ^= 1;
+        }
+    }
+
     // ---- one step, the reference loop body (voftrial1.cu:1362-1401).
     void enqueue_step(int64_t istep) {
         vof::launch_predict(e, g_pred, fsel, st);             // :1364-1368
         vof::launch_rhs(e, g_rhs, fsel, st);                  // rhs of :418-422
-        for (int it = 0; it < sweeps; ++it) {                  // :1371-1375
-            vof::launch_jacobi(e, g_jac, psel, st);
-            psel ^= 1;
-        }
+        enqueue_jacobi();                                     // :1371-1375
         if (flags & VOF_FLAG_REMOVE_P_MEAN) {
             vof::launch_reduce(e, g_red, fsel, psel, 1, part, ncell, red_out, means, st);
             vof::launch_sub_mean(e, g_sub, psel, means, st);
@@ -337,7 +355,7 @@ struct Ensemble : EnsBase {
         }
         VOF_HIP(hipGraphLaunch(it->second, st));
         // advance the buffer parities exactly as enqueue_step does
-        psel ^= (sweeps & 1);
+        psel ^= (jacobi_launches() & 1);
         fsel ^= 1;
         return VOF_OK;
     }
@@ -375,6 +393,7 @@ struct Ensemble : EnsBase {
             case 0: vof::launch_predict(e, g_pred, fsel, st); break;
             case 1: vof::launch_rhs(e, g_rhs, fsel, st); break;
             case 2: vof::launch_jacobi(e, g_jac, psel, st); psel ^= 1; break;
+            case 8: vof::launch_jacobi2(e, g_jac2, psel, st); psel ^= 1; break;
             case 3: vof::launch_correct(e, g_corr, fsel, psel, st); break;
             case 4: vof::launch_fct(e, g_fx, 0, fsel, 0, st); fsel ^= 1; break;
             case 5: vof::launch_fct(e, g_fy, 1, fsel, 0, st); fsel ^= 1; break;
@@ -493,7 +512,7 @@ struct Ensemble : EnsBase {
                 VOF_HIP(hipEventRecord(ev[1], st));
                 vof::launch_rhs(e, g_rhs, fsel, st);
                 VOF_HIP(hipEventRecord(ev[2], st));
-                for (int it = 0; it < sweeps; ++it) { vof::launch_jacobi(e, g_jac, psel, st); psel ^= 1; }
+                enqueue_jacobi();
                 VOF_HIP(hipEventRecord(ev[3], st));
                 if (flags & VOF_FLAG_REMOVE_P_MEAN) {
                     vof::launch_reduce(e, g_red, fsel, psel, 1, part, ncell, red_out, means, st);
@@ -727,6 +746,46 @@ vof_status vof_write_vtk(vof_solver* s, const char* path) {
     }
     if (rc != 0) { set_err("Failed to open output file: %s", path); return VOF_ERR_INVALID; }
     return VOF_OK;
+}
+
+vof_status vof_copy_bandwidth(int64_t bytes, int reps, double* gbs) {
+    if (bytes < (1 << 20) || reps < 1 || !gbs) { set_err("copy_bandwidth: bad argument"); return VOF_ERR_INVALID; }
+    bytes &= ~int64_t(15);
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    vof_status rc = VOF_OK;
+    auto fail = [&](hipError_t e) {
+        set_err("HIP failure in copy_bandwidth: %s", hipGetErrorString(e));
+        rc = VOF_ERR_HIP;
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&a, size_t(bytes))) != hipSuccess || (e = hipMalloc(&b, size_t(bytes))) != hipSuccess) {
+        fail(e);
+    } else if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess ||
+               (e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess ||
+               (e = hipMemsetAsync(a, 0, size_t(bytes), st)) != hipSuccess) {
+        fail(e);
+    } else {
+        const long n16 = long(bytes / 16);
+        vof::launch_copy(a, b, n16, st);  // warm-up
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < reps; ++r) vof::launch_copy(a, b, n16, st);
+        (void)hipEventRecord(e1, st);
+        if ((e = hipEventSynchronize(e1)) != hipSuccess) {
+            fail(e);
+        } else {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            *gbs = 2.0 * double(bytes) * reps / (double(ms) * 1e-3) / 1e9;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
 }
 
 vof_status vof_write_vtk_host(const char* path, const void* data, int precision, int nxTot, int nyTot,

@@ -21,9 +21,11 @@ struct Ens {
 
 constexpr int kNumReduce = 7;
 
+void launch_copy(const void* a, void* b, long n16, hipStream_t st);
 template <class R> void launch_predict(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
 template <class R> void launch_rhs(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
 template <class R> void launch_jacobi(const Ens<R>& e, const Geo& g, int psrc, hipStream_t st);
+template <class R> void launch_jacobi2(const Ens<R>& e, const Geo& g, int psrc, hipStream_t st);
 template <class R> void launch_correct(const Ens<R>& e, const Geo& g, int fsel, int psel, hipStream_t st);
 template <class R> void launch_fct(const Ens<R>& e, const Geo& g, int dir, int src, int fin, hipStream_t st);
 template <class R> void launch_clamp_F(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
@@ -42,6 +44,11 @@ inline int blocks_wcol(int nx, int ny, int nzrange, int kz, int h) {
     const int s = 64 - 2 * h;
     const int wpp = (plane + s - 1) / s;
     return ((wpp + 3) / 4) * ((nzrange + kz - 1) / kz);
+}
+// Two-sweep Jacobi tiles: 60 stored lanes x 14 stored rows per work-group
+// (1024 lanes), one work-group row per chunk of kz planes.
+inline int blocks_jacobi2(int nx, int ny, int nz, int kz) {
+    return ((nx + 59) / 60) * ((ny + 13) / 14) * ((nz + kz - 1) / kz);
 }
 inline int blocks_march(int nx, int nd, int no, int ch) {
     const int ntr = no * (nx + 2);

@@ -214,6 +214,13 @@ __global__ __launch_bounds__(256) void k_jacobi(const Inst<R>* __restrict__ tab,
     const R aw = (i != 1) ? I.axx : R(0);
     const R an = (j != I.ny) ? I.ayy : R(0);
     const R as = (j != 1) ? I.ayy : R(0);
+    // ap = -(ae+aw+an+as+af+ab) for the three kinds of plane (k = 1, interior, k = nz),
+    // summed in the reference's order (:430)
+    const R azz = I.azz;
+    const R ap_in = -(ae + aw + an + as + azz + azz);
+    const R ap_lo = -(ae + aw + an + as + ((I.nz != 1) ? azz : R(0)) + R(0));
+    const R ap_hi = -(ae + aw + an + as + R(0) + ((I.nz != 1) ? azz : R(0)));
+    const R ap_one = -(ae + aw + an + as + R(0) + R(0));
     int o = m.k0 * sy;
     const int olim = I.nz * sy;  // last plane offset whose +1 neighbour is stored
     R p_km = p.ld(tc, o - sy), p_k = p.ld(tc, o);
@@ -228,13 +235,94 @@ __global__ __launch_bounds__(256) void k_jacobi(const Inst<R>* __restrict__ tab,
         n_b = rhs.ld(tc, o2);
         const R pim = sh_up(p_k), pip = sh_dn(p_k);
         if (out) {
-            const R af = (k != I.nz) ? I.azz : R(0);
-            const R ab = (k != 1) ? I.azz : R(0);
-            const R ap = -(ae + aw + an + as + af + ab);
+            const R af = (k != I.nz) ? azz : R(0);
+            const R ab = (k != 1) ? azz : R(0);
+            const R ap = (k != 1 && k != I.nz) ? ap_in : (I.nz == 1 ? ap_one : (k == 1 ? ap_lo : ap_hi));
             pt.st((b - ae * pip - aw * pim - an * pjp - as * pjm - af * p_kp - ab * p_km) / ap, tc, o);
         }
         p_km = p_k;
         p_k = p_kp;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Two Jacobi sweeps per launch (temporal blocking; the reference runs one
+// sweep per launch, :1371-1375).  A work-group of 16 waves covers a tile of
+// 16 rows x 64 lanes: wave w holds row j = 14*band + w, lane l holds
+// i = 60*xtile - 1 + l.  Sweep 1 is formed on every lane of every row (zero
+// outside the interior, as the never-written ghost p of the reference), and
+// shared across rows through a 3-plane LDS ring; sweep 2 is stored on lanes
+// 2..61 of rows 1..14, which have all their sweep-1 neighbours.  The tile
+// marches along z with sweep 2 one plane behind sweep 1, the z neighbours of
+// both sweeps held in registers.  Each sweep keeps the reference's operation
+// order, so two launches of this kernel equal four single sweeps bit for bit.
+constexpr int kJ2Rows = 16, kJ2OutRows = 14, kJ2OutLanes = 60;
+
+template <class R>
+__device__ __forceinline__ R jac_point(R b, R ae, R aw, R an, R as, R af, R ab, R pe, R pw, R pn, R ps, R pf,
+                                       R pb) {
+    const R ap = -(ae + aw + an + as + af + ab);
+    return (b - ae * pe - aw * pw - an * pn - as * ps - af * pf - ab * pb) / ap;
+}
+
+template <class R>
+__global__ __launch_bounds__(1024) void k_jacobi2(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
+                                                  int ninst, int psrc, int kz) {
+    VOF_PROLOGUE(R)
+    __shared__ R ring[3][kJ2Rows][64];
+    const int nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
+    const int nxt = (nx + kJ2OutLanes - 1) / kJ2OutLanes;
+    const int nbd = (ny + kJ2OutRows - 1) / kJ2OutRows;
+    const int per_z = nxt * nbd;
+    const int zc = lb / per_z, rem = lb - zc * per_z;
+    const int bd = rem / nxt, xt = rem - bd * nxt;
+    const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+    const int i = kJ2OutLanes * xt - 1 + lane;
+    const int j = kJ2OutRows * bd + w;
+    const int k0 = 1 + zc * kz, k1 = min(nz, k0 + kz - 1);
+    const bool in1 = (i >= 1 && i <= nx && j >= 1 && j <= ny);
+    const bool st2 = in1 && lane >= 2 && lane <= 61 && w >= 1 && w <= kJ2OutRows;
+    const int ic = min(max(i, 0), nx + 1), jc = min(max(j, 0), ny + 1);
+    const int tc = jc * sx + ic;
+    const int tjm = max(jc - 1, 0) * sx + ic, tjp = min(jc + 1, ny + 1) * sx + ic;
+    const Buf<R> p(sel2(I.p, psrc), I.ntot), pt(sel2(I.p, psrc ^ 1), I.ntot), rhs(I.rhs, I.ntot);
+    const R ae = (i != nx) ? I.axx : R(0), aw = (i != 1) ? I.axx : R(0);
+    const R an = (j != ny) ? I.ayy : R(0), as = (j != 1) ? I.ayy : R(0);
+    const R azz = I.azz;
+    const int olast = (nz + 1) * sy;
+    auto po = [&](int s) { return min(max(s, 0) * sy, olast); };  // clamped plane offset
+    // march state: sweep-0 column at s-1, s; sweep-1 column at s-2, s-1; rhs at s-1
+    int s = k0 - 1;
+    R q0m = p.ld(tc, po(s - 1)), q0 = p.ld(tc, po(s));
+    R q1mm = R(0), q1m = R(0), bm = R(0);
+    R n_q0p = p.ld(tc, po(s + 1)), n_jm = p.ld(tjm, po(s)), n_jp = p.ld(tjp, po(s)), n_b = rhs.ld(tc, po(s));
+    for (; s <= k1 + 1; ++s) {
+        const R q0p = n_q0p, q0jm = n_jm, q0jp = n_jp, b = n_b;
+        n_q0p = p.ld(tc, po(s + 2));
+        n_jm = p.ld(tjm, po(s + 1));
+        n_jp = p.ld(tjp, po(s + 1));
+        n_b = rhs.ld(tc, po(s + 1));
+        // sweep 1 at plane s
+        const R q0w = sh_up(q0), q0e = sh_dn(q0);
+        R q1 = R(0);
+        if (in1 && s >= 1 && s <= nz) {
+            const R af = (s != nz) ? azz : R(0), ab = (s != 1) ? azz : R(0);
+            q1 = jac_point(b, ae, aw, an, as, af, ab, q0e, q0w, q0jp, q0jm, q0p, q0m);
+        }
+        const int slot = s % 3, pslot = (s + 2) % 3;
+        ring[slot][w][lane] = q1;
+        __syncthreads();
+        // sweep 2 at plane s-1
+        const int s2 = s - 1;
+        const R q1w = sh_up(q1m), q1e = sh_dn(q1m);
+        if (st2 && s2 >= k0) {
+            const R q1s = ring[pslot][w - 1][lane], q1n = ring[pslot][w + 1][lane];
+            const R af = (s2 != nz) ? azz : R(0), ab = (s2 != 1) ? azz : R(0);
+            pt.st(jac_point(bm, ae, aw, an, as, af, ab, q1e, q1w, q1n, q1s, q1, q1mm), tc, s2 * sy);
+        }
+        q0m = q0; q0 = q0p;
+        q1mm = q1m; q1m = q1;
+        bm = b;
     }
 }
 
@@ -393,9 +481,12 @@ __global__ __launch_bounds__(256) void k_fct_x(const Inst<R>* __restrict__ tab, 
     const bool cell = (m.t == m.tc) && i >= 1 && i <= I.nx && j >= 1 && j <= I.ny;
     const bool store = cell && m.own;
     const Buf<R> Fi(sel2(I.F, src), I.ntot), Fo(sel2(I.F, src ^ 1), I.ntot), u(I.u, I.ntot);
+    R nf = Fi.ld(m.tc, m.k0 * I.sy), nu = u.ld(m.tc, m.k0 * I.sy);  // plane k, fetched one plane ahead
     for (int k = m.k0; k <= m.k1; ++k) {
-        const R f0 = Fi.ld(m.tc, k * I.sy);
-        const R u0 = u.ld(m.tc, k * I.sy);
+        const R f0 = nf, u0 = nu;
+        const int on = min(k + 1, m.k1) * I.sy;
+        nf = Fi.ld(m.tc, on);
+        nu = u.ld(m.tc, on);
         if (__all(f0 == R(0))) {
             if (store) store_F(I, Fo, i, j, k, R(0), fin != 0);
             continue;
@@ -727,6 +818,18 @@ __global__ __launch_bounds__(256) void k_reduce2(const int* __restrict__ boff, c
 }
 
 // ---------------------------------------------------------------------------
+// Plain streaming copy (16 B per lane per access, grid-stride), used to
+// measure the attainable HBM bandwidth and to calibrate the byte counters.
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ a, float4* __restrict__ b, long n) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) b[t] = a[t];
+}
+
+void launch_copy(const void* a, void* b, long n16, hipStream_t st) {
+    hipLaunchKernelGGL(k_copy, dim3(256 * 32), dim3(256), 0, st, (const float4*)a, (float4*)b, n16);
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers.
 template <class R>
 void launch_predict(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st) {
@@ -739,6 +842,10 @@ void launch_rhs(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st) {
 template <class R>
 void launch_jacobi(const Ens<R>& e, const Geo& g, int psrc, hipStream_t st) {
     hipLaunchKernelGGL(k_jacobi<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, psrc, g.kz);
+}
+template <class R>
+void launch_jacobi2(const Ens<R>& e, const Geo& g, int psrc, hipStream_t st) {
+    hipLaunchKernelGGL(k_jacobi2<R>, dim3(g.nblk), dim3(1024), 0, st, e.tab, g.boff, e.ninst, psrc, g.kz);
 }
 template <class R>
 void launch_correct(const Ens<R>& e, const Geo& g, int fsel, int psel, hipStream_t st) {
@@ -779,6 +886,7 @@ void launch_reduce(const Ens<R>& e, const Geo& g, int fsel, int psel, int mode, 
     template void launch_predict<R>(const Ens<R>&, const Geo&, int, hipStream_t);                        \
     template void launch_rhs<R>(const Ens<R>&, const Geo&, int, hipStream_t);                            \
     template void launch_jacobi<R>(const Ens<R>&, const Geo&, int, hipStream_t);                         \
+    template void launch_jacobi2<R>(const Ens<R>&, const Geo&, int, hipStream_t);                        \
     template void launch_correct<R>(const Ens<R>&, const Geo&, int, int, hipStream_t);                   \
     template void launch_fct<R>(const Ens<R>&, const Geo&, int, int, int, hipStream_t);                  \
     template void launch_clamp_F<R>(const Ens<R>&, const Geo&, int, hipStream_t);                        \

@@ -34,7 +34,8 @@ FLAG_WATCHDOG = 0x10
 STAGE_TIME_NAMES = ("predictor", "rhs", "jacobi", "p_mean", "correction", "fct_x", "fct_y", "fct_z", "ghost_F")
 
 # Stages for Solver.debug_stage (test hook).
-STAGE_PREDICT, STAGE_RHS, STAGE_JACOBI, STAGE_CORRECT, STAGE_FCT_X, STAGE_FCT_Y, STAGE_FCT_Z, STAGE_GHOST_F = range(8)
+(STAGE_PREDICT, STAGE_RHS, STAGE_JACOBI, STAGE_CORRECT, STAGE_FCT_X, STAGE_FCT_Y, STAGE_FCT_Z, STAGE_GHOST_F,
+ STAGE_JACOBI2) = range(9)
 
 
 class VofParams(ctypes.Structure):
@@ -117,6 +118,7 @@ def load_library(path: str = LIB_PATH):
         "vof_diagnostics": (c_int, [S, POINTER(VofDiag)]),
         "vof_write_vtk": (c_int, [S, c_char_p]),
         "vof_write_vtk_host": (c_int, [c_char_p, c_void_p, c_int, c_int, c_int, c_int]),
+        "vof_copy_bandwidth": (c_int, [c_int64, c_int, POINTER(c_double)]),
         "vof_sync": (c_int, [S]),
         "vof_time_steps": (c_int, [S, c_int64, c_int, c_int, c_int, POINTER(c_double), POINTER(c_double)]),
         "vof_set_tuning": (c_int, [S, POINTER(VofTuning)]),
@@ -320,6 +322,14 @@ class Batch:
     @property
     def interior_cells(self) -> int:
         return int(self.lib.vof_batch_interior_cells(self.h))
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20) -> float:
+    """Attainable device bandwidth in GB/s (read + write) of a plain copy kernel."""
+    lib = load_library()
+    g = c_double()
+    _check(lib.vof_copy_bandwidth(int(nbytes), int(reps), byref(g)))
+    return g.value
 
 
 def write_vtk_host(path: str, field: np.ndarray) -> None:

@@ -139,6 +139,10 @@ vof_status vof_write_vtk(vof_solver* s, const char* path);
 vof_status vof_write_vtk_host(const char* path, const void* data, int precision, int nxTot, int nyTot,
                               int nzTot);
 
+/* Attainable device bandwidth: a plain 16-byte-per-lane copy of `bytes`
+ * bytes repeated `reps` times; *gbs = (read + written bytes) / time. */
+vof_status vof_copy_bandwidth(int64_t bytes, int reps, double* gbs);
+
 /* Synchronise the handle's stream. */
 vof_status vof_sync(vof_solver* s);
 
@@ -165,7 +169,8 @@ vof_status vof_get_tuning(const vof_solver* s, vof_tuning* t);
 
 /* Test hook: run one stage on the current state.  stage: 0 predictor,
  * 1 rhs, 2 one Jacobi sweep, 3 correction + velocity BC, 4/5/6 FCT sweep
- * x/y/z, 7 F ghost fill.  Not part of the reference surface. */
+ * x/y/z, 7 clamp + F ghost fill, 8 two Jacobi sweeps in one launch.  Not
+ * part of the reference surface. */
 vof_status vof_debug_stage(vof_solver* s, int stage);
 
 /* Number of interior cells n_x n_y n_z and of stored cells (n+2)^3. */

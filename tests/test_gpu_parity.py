@@ -79,11 +79,16 @@ def test_stages(vofmod, orc, prec, grid):
     # rhs (formed once; the oracle forms it inside each sweep)
     s.debug_stage(vofmod.STAGE_RHS)
     assert_same(interior(s.download("RHS")), interior(o.get("RHS")), "RHS")
-    # Jacobi sweeps
+    # Jacobi sweeps: single-sweep launches, then two-sweep (temporally blocked) launches
     for it in range(3):
         s.debug_stage(vofmod.STAGE_JACOBI)
         o.stage("jacobi")
         assert_same(s.download("P"), o.get("P"), f"P sweep {it}")
+    for it in range(2):
+        s.debug_stage(vofmod.STAGE_JACOBI2)
+        o.stage("jacobi")
+        o.stage("jacobi")
+        assert_same(s.download("P"), o.get("P"), f"P double sweep {it}")
     # correction + velocity BC, all stored entries
     s.debug_stage(vofmod.STAGE_CORRECT)
     o.stage("update")
@@ -132,10 +137,29 @@ def run_steps_pair(vofmod, orc, steps, first=1, graph=False, ic=1, box=None, **k
     return s, o
 
 
+def test_jacobi2_tiles_and_chunks(vofmod, orc):
+    """Two-sweep launches on a grid with several x tiles, row bands and z chunks."""
+    s, o = make_pair(vofmod, orc, nx=131, ny=33, nz=21, precision=4, dt=1e-3)
+    s.set_tuning(kz_jacobi=4)
+    rng = np.random.default_rng(SEED + 11)
+    st = random_state(s.shape, rng, s.dtype)
+    load_state(s, o, st)
+    s.debug_stage(vofmod.STAGE_PREDICT)
+    s.debug_stage(vofmod.STAGE_RHS)
+    for name in ("nu_rho", "advect_u", "advect_v", "advect_w"):
+        o.stage(name)
+    for it in range(3):
+        s.debug_stage(vofmod.STAGE_JACOBI2)
+        o.stage("jacobi")
+        o.stage("jacobi")
+        assert_same(s.download("P"), o.get("P"), f"P double sweep {it}")
+
+
+@pytest.mark.parametrize("tblock", [1, 2])
 @pytest.mark.parametrize("prec", [4, 8])
-def test_whole_steps_reference_case(vofmod, orc, prec):
-    print(f"whole steps prec={prec} n=24", flush=True)
-    s, o = run_steps_pair(vofmod, orc, 30, n=24, precision=prec)
+def test_whole_steps_reference_case(vofmod, orc, prec, tblock):
+    print(f"whole steps prec={prec} n=24 tblock={tblock}", flush=True)
+    s, o = run_steps_pair(vofmod, orc, 30, n=24, precision=prec, jacobi_tblock=tblock)
     for f in ("F", "U", "V", "W", "P", "USTAR", "VSTAR", "WSTAR"):
         assert_same(s.download(f), o.get(f), f)
     d1, d2 = s.diagnostics(), o.diagnostics()
@@ -212,10 +236,11 @@ def test_batch_equals_plain(vofmod):
         s.close()
 
 
-def test_full_size_200_parity_three_steps(vofmod, orc):
+@pytest.mark.parametrize("tblock", [1, 2])
+def test_full_size_200_parity_three_steps(vofmod, orc, tblock):
     """Reference default case at full size: bitwise after one step of each rotation."""
-    print("200^3 parity, 3 steps", flush=True)
-    s, o = run_steps_pair(vofmod, orc, 3, n=200, precision=4)
+    print(f"200^3 parity, 3 steps, tblock={tblock}", flush=True)
+    s, o = run_steps_pair(vofmod, orc, 3, n=200, precision=4, jacobi_tblock=tblock)
     for f in ("F", "U", "V", "W", "P"):
         assert_same(s.download(f), o.get(f), f)
 
