@@ -52,7 +52,23 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--stage-steps", type=int, default=30)
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="run one plain copy of this many bytes first (byte-counter calibration)")
+    ap.add_argument("--jacobi-tblock", type=int, default=0)
     return ap.parse_args()
+
+
+def load_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/traffic_200.json, written from rocprofv3 FETCH_SIZE / WRITE_SIZE
+    runs scaled by a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_200.json")
+    try:
+        with open(path) as fh:
+            t = json.load(fh)
+        return t["kernels"][kernel]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def make_solver(vof, n, prec):
@@ -75,7 +91,7 @@ def cpu_baseline(n, prec, budget_s):
     o.step(1, 1)
     t1 = time.perf_counter()
     per = max(t1 - t0, 1e-6)
-    k = max(1, min(50, int(budget_s / per)))
+    k = max(1, min(1000, int(budget_s / per)))
     t0 = time.perf_counter()
     o.step(2, k)
     el = time.perf_counter() - t0
@@ -100,10 +116,14 @@ def main():
     import vof
 
     vof.load_library()
+    if a.calibrate_copy:
+        vof.copy_bandwidth(a.calibrate_copy, 1)
     n, prec = a.n, a.precision
     esz = 8 if prec == 8 else 4
     cells = n ** 3
     s = make_solver(vof, n, prec)
+    if a.jacobi_tblock:
+        s.set_tuning(jacobi_tblock=a.jacobi_tblock)
     graph = not a.eager
 
     # warm-up (also captures the three graphs of the sweep rotation)
@@ -135,7 +155,10 @@ def main():
     d = kern[dom]
     step_bytes = sum(STAGE_BYTES[k] * LAUNCHES_PER_STEP[k] for k in STAGE_BYTES) * (esz // 4) * cells
     diag = s.diagnostics()
+    tuning = s.get_tuning()
     s.close()
+    copy_hbm = vof.copy_bandwidth(1 << 30, 10)
+    copy_mall = vof.copy_bandwidth(1 << 26, 50)
 
     out = {
         "metric": METRIC,
@@ -165,10 +188,12 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": d["frac"],
-            "traffic": None,
+            "traffic": (load_traffic(dom) if (n == 200 and prec == 4) else None),
             "bytes_per_launch": d["bytes_per_launch"],
             "ms_per_launch": d["ms_per_launch"],
+            "attainable_copy_gbs": {"hbm_1GiB": round(copy_hbm, 1), "mall_64MiB": round(copy_mall, 1)},
         },
+        "tuning": tuning,
         "step_hbm_frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "stage_ms_per_step": {k: round(v, 5) for k, v in per_step.items()},
         "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
