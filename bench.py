@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--precision", type=int, default=4)
     ap.add_argument("--eager", action="store_true", help="launch kernels per step instead of graph replay")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-instance ensemble leg")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--stage-steps", type=int, default=30)
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -76,6 +78,61 @@ def make_solver(vof, n, prec):
     s = vof.Solver(prm)
     s.init_box(0.04, 0.04, 0.06)
     return s
+
+
+SEED = 20261203
+
+
+def ensemble_config(vof, ninst=64, seed=SEED):
+    """64-instance dam-break ensemble: instance i draws, from seed+i, a cubic grid
+    N in {32,48,64,80,96}, liquid block fractions of the 0.1 m edge in
+    U[0.2,0.6] (x, y) and U[0.3,0.8] (z), gas density U[25,100], sigma
+    U[0.003,0.02]; liquid 1000 kg/m^3, g = (0,0,-9.81), dt = 4e-6 s."""
+    import numpy as np
+
+    prms, boxes = [], []
+    for i in range(ninst):
+        r = np.random.default_rng(seed + i)
+        n = int(r.choice([32, 48, 64, 80, 96]))
+        fx, fy, fz = r.uniform(0.2, 0.6), r.uniform(0.2, 0.6), r.uniform(0.3, 0.8)
+        rho_g = float(r.uniform(25.0, 100.0))
+        sigma = float(r.uniform(0.003, 0.02))
+        prms.append(vof.default_params(n=n, g=(0.0, 0.0, -9.81), rho_g=rho_g, sigma=sigma))
+        boxes.append((0.1 * fx, 0.1 * fy, 0.1 * fz))
+    return prms, boxes
+
+
+def batched_leg(vof, steps=100, warmup=10):
+    """Aggregate cell-updates/s of the ensemble advanced as one batch (one launch
+    per stage, hipGraph per step) and as 64 plain solvers run one after the other."""
+    prms, boxes = ensemble_config(vof)
+    cells = sum(p.nx * p.ny * p.nz for p in prms)
+    b = vof.Batch(prms)
+    b.init_box(*zip(*boxes))
+    b.autotune(1, 3, None)
+    b.step(1, warmup, graph=True)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(1 + warmup, steps, graph=True)
+    b.sync()
+    tb = time.perf_counter() - t0
+    b.close()
+    tp = 0.0
+    for p, bx in zip(prms, boxes):
+        s = vof.Solver(p)
+        s.init_box(*bx)
+        s.step(1, warmup, graph=True)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(1 + warmup, steps, graph=True)
+        s.sync()
+        tp += time.perf_counter() - t0
+        s.close()
+    batched = cells * steps / tb
+    plain = cells * steps / tp
+    return {"value": batched, "unit": "cell-updates/s", "plain_value": plain, "ratio_vs_plain": batched / plain,
+            "instances": len(prms), "interior_cells": cells, "steps": steps,
+            "ms_per_step_batched": 1e3 * tb / steps, "ms_per_step_plain_sum": 1e3 * tp / steps}
 
 
 def cpu_baseline(n, prec, budget_s):
@@ -124,6 +181,9 @@ def main():
     s = make_solver(vof, n, prec)
     if a.jacobi_tblock:
         s.set_tuning(jacobi_tblock=a.jacobi_tblock)
+    if not a.no_autotune:
+        # launch shapes chosen by timing candidates on this GPU (state restored)
+        s.autotune(1, 4, None)
     graph = not a.eager
 
     # warm-up (also captures the three graphs of the sweep rotation)
@@ -200,6 +260,8 @@ def main():
                     for k, v in kern.items()},
         "final_liquid_volume": diag["liquid_volume"],
     }
+    if not a.no_batched:
+        out["batched"] = batched_leg(vof)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, prec, a.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -74,6 +74,75 @@ int box_extent(int n, double L, double b) {
     return std::max(0, std::min(n, c));
 }
 
+// Tuning cache: a JSON object, one entry per line,
+//   "gfx950|200x200x200|p4|n1|c8000000|s10": {"kz_predict": 4, ..., "kz_fct_x": 4},
+const char* const kTuneFields[7] = {"kz_predict", "kz_rhs", "kz_jacobi", "kz_correct",
+                                    "fct_chunk", "jacobi_tblock", "kz_fct_x"};
+
+int32_t* tune_slot(vof_tuning* t, int f) {
+    int32_t* s[7] = {&t->kz_predict, &t->kz_rhs, &t->kz_jacobi, &t->kz_correct,
+                     &t->fct_chunk, &t->jacobi_tblock, &t->kz_fct_x};
+    return s[f];
+}
+
+std::vector<std::string> read_lines(const char* path) {
+    std::vector<std::string> lines;
+    std::FILE* fp = std::fopen(path, "r");
+    if (!fp) return lines;
+    char buf[1024];
+    while (std::fgets(buf, sizeof(buf), fp)) lines.emplace_back(buf);
+    std::fclose(fp);
+    return lines;
+}
+
+bool load_tuning(const char* path, const std::string& key, vof_tuning* t) {
+    const std::string tag = "\"" + key + "\":";
+    for (const auto& l : read_lines(path)) {
+        if (l.find(tag) == std::string::npos) continue;
+        vof_tuning r = *t;
+        for (int f = 0; f < 7; ++f) {
+            const std::string name = std::string("\"") + kTuneFields[f] + "\":";
+            const size_t at = l.find(name);
+            if (at == std::string::npos) return false;
+            const int v = std::atoi(l.c_str() + at + name.size());
+            if (v <= 0) return false;
+            *tune_slot(&r, f) = v;
+        }
+        *t = r;
+        return true;
+    }
+    return false;
+}
+
+void store_tuning(const char* path, const std::string& key, vof_tuning t) {
+    std::vector<std::string> keep;
+    const std::string tag = "\"" + key + "\":";
+    for (auto l : read_lines(path)) {
+        if (l.find("\": {") == std::string::npos || l.find(tag) != std::string::npos) continue;
+        while (!l.empty() && (l.back() == '\n' || l.back() == ',' || l.back() == ' ')) l.pop_back();
+        keep.push_back(l);
+    }
+    std::string entry = " " + tag + " {";
+    for (int f = 0; f < 7; ++f) {
+        char b[64];
+        std::snprintf(b, sizeof(b), "%s\"%s\": %d", f ? ", " : "", kTuneFields[f], int(*tune_slot(&t, f)));
+        entry += b;
+    }
+    entry += "}";
+    keep.push_back(entry);
+    const std::string tmp = std::string(path) + ".tmp";
+    std::FILE* fp = std::fopen(tmp.c_str(), "w");
+    if (!fp) return;
+    std::fputs("{\n", fp);
+    for (size_t i = 0; i < keep.size(); ++i) {
+        std::fputs(keep[i].c_str(), fp);
+        std::fputs(i + 1 < keep.size() ? ",\n" : "\n", fp);
+    }
+    std::fputs("}\n", fp);
+    std::fclose(fp);
+    std::rename(tmp.c_str(), path);
+}
+
 struct EnsBase {
     virtual ~EnsBase() {}
     int prec = 4;
@@ -92,6 +161,7 @@ struct EnsBase {
     virtual int64_t interior_cells() const = 0;
     virtual int64_t total_cells(int inst) const = 0;
     virtual int ninst() const = 0;
+    virtual vof_status autotune(int64_t first, int steps, const char* cache, vof_tuning* out) = 0;
     virtual const vof_params& params(int inst) const = 0;
 };
 
@@ -107,6 +177,7 @@ struct Ensemble : EnsBase {
     uint32_t flags = 0;
     hipStream_t st = nullptr;
     void* arena = nullptr;
+    size_t arena_bytes = 0;
     std::vector<int*> boffs;
     double* part = nullptr;
     double* red_out = nullptr;
@@ -218,6 +289,7 @@ struct Ensemble : EnsBase {
         }
         VOF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         const size_t bytes = total * sizeof(R);
+        arena_bytes = bytes;
         if (hipMalloc(&arena, bytes) != hipSuccess) {
             arena = nullptr;
             set_err("device allocation of %zu bytes failed", bytes);
@@ -484,6 +556,100 @@ struct Ensemble : EnsBase {
         return VOF_OK;
     }
 
+    // ---- autotuner: time candidate launch shapes stage by stage on the live
+    // state (snapshotted and restored), keep the fastest per stage, cache it.
+    std::string tune_key() const {
+        hipDeviceProp_t prop;
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        std::string arch = (hipGetDeviceProperties(&prop, dev) == hipSuccess) ? prop.gcnArchName : "unknown";
+        const size_t colon = arch.find(':');
+        if (colon != std::string::npos) arch = arch.substr(0, colon);
+        char buf[160];
+        int64_t cells = interior_cells();
+        std::snprintf(buf, sizeof(buf), "%s|%dx%dx%d|p%d|n%d|c%lld|s%d", arch.c_str(), ih[0].nx, ih[0].ny, ih[0].nz,
+                      int(sizeof(R)), ninst(), (long long)cells, sweeps);
+        return buf;
+    }
+
+    vof_status autotune(int64_t first, int steps, const char* cache, vof_tuning* out) override {
+        const std::string key = tune_key();
+        if (cache && load_tuning(cache, key, &tun)) {
+            VOF_TRY(set_tuning(&tun));
+            if (out) *out = tun;
+            return VOF_OK;
+        }
+        if (steps < 1) steps = 1;
+        VOF_HIP(hipStreamSynchronize(st));
+        void* snap = nullptr;
+        if (hipMalloc(&snap, arena_bytes) != hipSuccess) { set_err("autotune: no memory for the state snapshot"); return VOF_ERR_ALLOC; }
+        hipError_t ce = hipMemcpyAsync(snap, arena, arena_bytes, hipMemcpyDeviceToDevice, st);
+        const int f0 = fsel, p0 = psel;
+        vof_status rc = (ce == hipSuccess) ? VOF_OK : VOF_ERR_HIP;
+        vof_tuning best = tun;
+        int64_t is = first;
+        auto trial = [&](const vof_tuning& t, double* stage) -> vof_status {
+            vof_tuning tt = t;
+            vof_status s = set_tuning(&tt);
+            if (s != VOF_OK) return s;
+            double ms = 0.0;
+            s = time_steps(is, 1, steps, false, &ms, stage);
+            is += steps + 1;
+            return s;
+        };
+        // column kernels and the x sweep: one z-chunk length for all, best per stage
+        const int kzs[] = {2, 4, 8, 16};
+        double bestt[VOF_NSTAGE_TIMES];
+        for (double& x : bestt) x = 1e300;
+        for (int c : kzs) {
+            if (rc != VOF_OK) break;
+            vof_tuning t = tun;
+            t.kz_predict = t.kz_rhs = t.kz_jacobi = t.kz_correct = t.kz_fct_x = c;
+            double stage[VOF_NSTAGE_TIMES];
+            rc = trial(t, stage);
+            if (rc != VOF_OK) break;
+            if (stage[0] < bestt[0]) { bestt[0] = stage[0]; best.kz_predict = c; }
+            if (stage[1] < bestt[1]) { bestt[1] = stage[1]; best.kz_rhs = c; }
+            if (stage[2] < bestt[2]) { bestt[2] = stage[2]; best.kz_jacobi = c; best.jacobi_tblock = tun.jacobi_tblock; }
+            if (stage[4] < bestt[4]) { bestt[4] = stage[4]; best.kz_correct = c; }
+            if (stage[5] < bestt[5]) { bestt[5] = stage[5]; best.kz_fct_x = c; }
+        }
+        // Jacobi temporal-block depth with its own chunk lengths
+        const int other_tb = (tun.jacobi_tblock >= 2) ? 1 : 2;
+        for (int c : kzs) {
+            if (rc != VOF_OK) break;
+            vof_tuning t = best;
+            t.jacobi_tblock = other_tb;
+            t.kz_jacobi = c;
+            double stage[VOF_NSTAGE_TIMES];
+            rc = trial(t, stage);
+            if (rc == VOF_OK && stage[2] < bestt[2]) { bestt[2] = stage[2]; best.kz_jacobi = c; best.jacobi_tblock = other_tb; }
+        }
+        // y/z march chunk
+        const int chs[] = {4, 8, 16, 32};
+        double bestm = 1e300;
+        for (int c : chs) {
+            if (rc != VOF_OK) break;
+            vof_tuning t = best;
+            t.fct_chunk = c;
+            double stage[VOF_NSTAGE_TIMES];
+            rc = trial(t, stage);
+            if (rc == VOF_OK && stage[6] + stage[7] < bestm) { bestm = stage[6] + stage[7]; best.fct_chunk = c; }
+        }
+        // restore the state and apply the winner
+        const hipError_t re = hipMemcpyAsync(arena, snap, arena_bytes, hipMemcpyDeviceToDevice, st);
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(snap);
+        fsel = f0;
+        psel = p0;
+        if (re != hipSuccess && rc == VOF_OK) { set_err("autotune: state restore failed"); rc = VOF_ERR_HIP; }
+        if (rc != VOF_OK) return rc;
+        VOF_TRY(set_tuning(&best));
+        if (cache) store_tuning(cache, key, best);
+        if (out) *out = best;
+        return VOF_OK;
+    }
+
     vof_status sync() override {
         VOF_HIP(hipStreamSynchronize(st));
         return VOF_OK;
@@ -723,6 +889,16 @@ vof_status vof_get_tuning(const vof_solver* s, vof_tuning* t) {
     if (!s || !s->e || !t) { set_err("null argument"); return VOF_ERR_INVALID; }
     s->e->get_tuning(t);
     return VOF_OK;
+}
+vof_status vof_autotune(vof_solver* s, int64_t first_istep, int steps_per_trial, const char* cache_path,
+                        vof_tuning* out) {
+    VOF_NEED(s);
+    return s->e->autotune(first_istep, steps_per_trial, cache_path, out);
+}
+vof_status vof_batch_autotune(vof_batch* b, int64_t first_istep, int steps_per_trial, const char* cache_path,
+                              vof_tuning* out) {
+    VOF_NEED(b);
+    return b->e->autotune(first_istep, steps_per_trial, cache_path, out);
 }
 vof_status vof_debug_stage(vof_solver* s, int stage) {
     VOF_NEED(s);

@@ -124,6 +124,8 @@ def load_library(path: str = LIB_PATH):
         "vof_set_tuning": (c_int, [S, POINTER(VofTuning)]),
         "vof_get_tuning": (c_int, [S, POINTER(VofTuning)]),
         "vof_debug_stage": (c_int, [S, c_int]),
+        "vof_autotune": (c_int, [S, c_int64, c_int, c_char_p, POINTER(VofTuning)]),
+        "vof_batch_autotune": (c_int, [B, c_int64, c_int, c_char_p, POINTER(VofTuning)]),
         "vof_interior_cells": (c_int64, [S]),
         "vof_total_cells": (c_int64, [S]),
         "vof_status_str": (c_char_p, [c_int]),
@@ -249,6 +251,14 @@ class Solver:
         _check(self.lib.vof_get_tuning(self.h, byref(t)))
         return {k: getattr(t, k) for k, _ in VofTuning._fields_}
 
+    def autotune(self, first_istep: int = 1, steps_per_trial: int = 5, cache: str | None = None) -> dict:
+        """Pick launch shapes by timing candidates on the live state (restored
+        afterwards); cached per (arch, grid, precision) in the JSON file `cache`."""
+        t = VofTuning()
+        _check(self.lib.vof_autotune(self.h, first_istep, steps_per_trial, cache.encode() if cache else None,
+                                     byref(t)))
+        return {k: getattr(t, k) for k, _ in VofTuning._fields_}
+
     @property
     def interior_cells(self) -> int:
         return int(self.lib.vof_interior_cells(self.h))
@@ -318,6 +328,12 @@ class Batch:
     def set_tuning(self, **kw) -> None:
         t = VofTuning(*(int(kw.get(k, 0)) for k, _ in VofTuning._fields_))
         _check(self.lib.vof_batch_set_tuning(self.h, byref(t)))
+
+    def autotune(self, first_istep: int = 1, steps_per_trial: int = 3, cache: str | None = None) -> dict:
+        t = VofTuning()
+        _check(self.lib.vof_batch_autotune(self.h, first_istep, steps_per_trial,
+                                           cache.encode() if cache else None, byref(t)))
+        return {k: getattr(t, k) for k, _ in VofTuning._fields_}
 
     @property
     def interior_cells(self) -> int:

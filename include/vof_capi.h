@@ -167,6 +167,15 @@ typedef struct vof_tuning {
 vof_status vof_set_tuning(vof_solver* s, const vof_tuning* t);
 vof_status vof_get_tuning(const vof_solver* s, vof_tuning* t);
 
+/* Autotuner (no reference counterpart): if cache_path holds an entry for
+ * (GPU arch, grid, precision, instance count, sweeps) it is applied; otherwise
+ * candidate z-chunk lengths, march chunk lengths and Jacobi temporal-block
+ * depths are timed stage by stage over steps_per_trial steps each (state
+ * snapshotted and restored on the device, so the run is unaffected), the
+ * fastest per stage is applied and written to cache_path (JSON; NULL: no cache). */
+vof_status vof_autotune(vof_solver* s, int64_t first_istep, int steps_per_trial, const char* cache_path,
+                        vof_tuning* out);
+
 /* Test hook: run one stage on the current state.  stage: 0 predictor,
  * 1 rhs, 2 one Jacobi sweep, 3 correction + velocity BC, 4/5/6 FCT sweep
  * x/y/z, 7 clamp + F ghost fill, 8 two Jacobi sweeps in one launch.  Not
@@ -200,6 +209,8 @@ vof_status vof_batch_sync(vof_batch* b);
 vof_status vof_batch_time_steps(vof_batch* b, int64_t first_istep, int warmup, int steps,
                                 int use_graph, double* ms_total);
 vof_status vof_batch_set_tuning(vof_batch* b, const vof_tuning* t);
+vof_status vof_batch_autotune(vof_batch* b, int64_t first_istep, int steps_per_trial, const char* cache_path,
+                              vof_tuning* out);
 int64_t vof_batch_interior_cells(const vof_batch* b);
 int vof_batch_size(const vof_batch* b);
 

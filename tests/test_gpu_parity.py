@@ -245,6 +245,30 @@ def test_full_size_200_parity_three_steps(vofmod, orc, tblock):
         assert_same(s.download(f), o.get(f), f)
 
 
+def test_autotune_keeps_state_and_caches(vofmod, tmp_path):
+    """Tuning trials run on a snapshot: the run continues bit-identically, and a
+    second solver of the same shape picks the cached choice."""
+    a = vofmod.Solver(n=24)
+    b = vofmod.Solver(n=24)
+    a.init_ic(1)
+    b.init_ic(1)
+    a.step(1, 5)
+    b.step(1, 5)
+    cache = tmp_path / "tune.json"
+    t = a.autotune(6, 2, str(cache))
+    assert cache.exists() and "24x24x24" in cache.read_text()
+    a.step(6, 7, graph=True)
+    b.step(6, 7)
+    for f in ("F", "U", "V", "W", "P"):
+        assert_same(a.download(f), b.download(f), f)
+    c = vofmod.Solver(n=24)
+    assert c.autotune(1, 2, str(cache)) == t
+    bt = vofmod.Batch([vofmod.default_params(n=12), vofmod.default_params(n=17)])
+    bt.init_ic(1)
+    tb = bt.autotune(1, 2, None)
+    assert tb["fct_chunk"] > 0
+
+
 def test_full_size_200_invariants(vofmod):
     """200^3, 300 steps: F stays in [0,1], no NaN, volume drift bounded."""
     s = vofmod.Solver(n=200)
