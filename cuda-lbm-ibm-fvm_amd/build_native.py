@@ -41,7 +41,7 @@ def build_hip() -> str:
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libvof_hip.so")
     tmp = out + ".tmp"
-    srcs = [os.path.join(CSRC, f) for f in ("vof_kernels.hip", "vof_host.hip", "vof_vtk.cpp")]
+    srcs = [os.path.join(CSRC, f) for f in ("vof_kernels.hip", "vof_adjoint.hip", "vof_host.hip", "vof_vtk.cpp")]
     _run([HIPCC, *HIP_FLAGS, "-shared", "-pthread", *srcs, "-o", tmp])
     os.replace(tmp, out)
     return out

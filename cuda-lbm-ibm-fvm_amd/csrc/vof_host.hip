@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/vof_capi.h"
+#include "vof_adjoint.h"
 #include "vof_kernels.h"
 #include "vof_vtk.h"
 
@@ -163,6 +164,12 @@ struct EnsBase {
     virtual int ninst() const = 0;
     virtual vof_status autotune(int64_t first, int steps, const char* cache, vof_tuning* out) = 0;
     virtual const vof_params& params(int inst) const = 0;
+    virtual vof_status adj_forward(int64_t first, int n, int every, double* J) = 0;
+    virtual vof_status adj_seed(vof_field f, const double* host, size_t count) = 0;
+    virtual vof_status adj_backward(double gJ, vof_adj_scalars* out) = 0;
+    virtual vof_status adj_download(vof_field f, double* host, size_t count) = 0;
+    virtual void adj_times(double* fwd, double* bwd) const = 0;
+    virtual void adj_release() = 0;
 };
 
 template <class R>
@@ -187,7 +194,41 @@ struct Ensemble : EnsBase {
     std::map<int, hipGraphExec_t> graphs;
     std::vector<hipGraph_t> graph_defs;
 
+    // ---- reverse mode (single instance).  State = F, u, v, w, p, each ntot R,
+    // stored back to back.  store: nck checkpoints, then the K states of the
+    // interval being reversed, then the final state of the forward.
+    struct Adj {
+        int64_t first = 0;
+        int n = 0, K = 0, nck = 0;
+        size_t nt = 0;
+        R* store = nullptr;
+        size_t store_states = 0;
+        int f_fin = 0, p_fin = 0;
+        double J = 0.0, sumF = 0.0;
+        double* d = nullptr;  // FP64 arrays, ntot each, see adj_alloc
+        size_t d_arrays = 0;
+        double *aF = nullptr, *au[2] = {}, *av[2] = {}, *aw[2] = {}, *ap[2] = {}, *gt = nullptr;
+        double *aus = nullptr, *avs = nullptr, *aws = nullptr, *arhs = nullptr, *arho = nullptr, *anu = nullptr;
+        double* seed[5] = {};  // F, u, v, w, p seeds of the final state (nullptr: zero)
+        double* seed_mem = nullptr;
+        R* tapeR = nullptr;
+        vof::FctTape<R> tp{};
+        double* part = nullptr;
+        double* acc = nullptr;
+        int cur = 0;
+        bool fwd_done = false, bwd_done = false;
+        double fwd_ms = 0.0, bwd_ms = 0.0;
+    } adj;
+
+    void adj_release() override {
+        Adj& a = adj;
+        for (void* p : {(void*)a.store, (void*)a.d, (void*)a.seed_mem, (void*)a.tapeR, (void*)a.part, (void*)a.acc})
+            if (p) (void)hipFree(p);
+        a = Adj();
+    }
+
     ~Ensemble() override {
+        adj_release();
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
         for (auto g : graph_defs) (void)hipGraphDestroy(g);
         for (int* p : boffs) (void)hipFree(p);
@@ -731,6 +772,269 @@ struct Ensemble : EnsBase {
         (void)hipEventDestroy(b);
         return VOF_OK;
     }
+
+    // ---------------------------------------------------------------- reverse mode
+    R* adj_state(size_t idx) const { return adj.store + idx * 5 * adj.nt; }
+
+    vof_status save_state(R* dst) {
+        const size_t b = adj.nt * sizeof(R);
+        const auto& I = ih[0];
+        const R* src[5] = {I.F[fsel], I.u, I.v, I.w, I.p[psel]};
+        for (int q = 0; q < 5; ++q)
+            VOF_HIP(hipMemcpyAsync(dst + q * adj.nt, src[q], b, hipMemcpyDeviceToDevice, st));
+        return VOF_OK;
+    }
+    // Load a stored state into F[fs], u, v, w, p[ps] and select those buffers.
+    vof_status load_state(const R* s, int fs, int ps) {
+        const size_t b = adj.nt * sizeof(R);
+        const auto& I = ih[0];
+        R* dst[5] = {I.F[fs], I.u, I.v, I.w, I.p[ps]};
+        for (int q = 0; q < 5; ++q)
+            VOF_HIP(hipMemcpyAsync(dst[q], s + q * adj.nt, b, hipMemcpyDeviceToDevice, st));
+        fsel = fs;
+        psel = ps;
+        return VOF_OK;
+    }
+
+    vof_status adj_forward(int64_t first, int n, int every, double* J) override {
+        if (ninst() != 1) { set_err("reverse mode runs on a single instance"); return VOF_ERR_UNSUPPORTED; }
+        if (flags & ~uint32_t(VOF_FLAG_WATCHDOG)) {
+            set_err("reverse mode supports flags 0 and VOF_FLAG_WATCHDOG only (got 0x%x)", flags);
+            return VOF_ERR_UNSUPPORTED;
+        }
+        if (n < 1 || every < 1 || !J) { set_err("adj_forward: need nsteps >= 1, checkpoint_every >= 1, J"); return VOF_ERR_INVALID; }
+        const int K = std::min(every, n);
+        const int nck = (n + K - 1) / K;
+        const size_t nt = size_t(ih[0].ntot);
+        const size_t states = size_t(nck) + size_t(K) + 1;
+        if (adj.store && (adj.nt != nt || adj.store_states < states)) {
+            (void)hipFree(adj.store);
+            adj.store = nullptr;
+        }
+        adj.nt = nt;
+        if (!adj.store) {
+            const size_t bytes = states * 5 * nt * sizeof(R);
+            if (hipMalloc(&adj.store, bytes) != hipSuccess) {
+                adj.store = nullptr;
+                set_err("reverse mode: checkpoint store of %zu bytes failed", bytes);
+                return VOF_ERR_ALLOC;
+            }
+            adj.store_states = states;
+        }
+        if (!adj.part) {
+            const size_t np = 4 * size_t(vof::adj_blocks(long(nt)));
+            VOF_HIP(hipMalloc(&adj.part, np * sizeof(double)));
+            VOF_HIP(hipMalloc(&adj.acc, vof::ADJ_NSCALAR * sizeof(double)));
+        }
+        for (double*& s : adj.seed) s = nullptr;
+        adj.first = first;
+        adj.n = n;
+        adj.K = K;
+        adj.nck = nck;
+        adj.bwd_done = false;
+        hipEvent_t e0, e1;
+        VOF_HIP(hipEventCreate(&e0));
+        VOF_HIP(hipEventCreate(&e1));
+        VOF_HIP(hipEventRecord(e0, st));
+        for (int s = 0; s < n; ++s) {
+            if (s % K == 0) VOF_TRY(save_state(adj_state(size_t(s / K))));
+            VOF_TRY(step(first + s, 1, true));
+        }
+        VOF_TRY(save_state(adj_state(size_t(nck) + size_t(K))));
+        adj.f_fin = fsel;
+        adj.p_fin = psel;
+        const auto& I = ih[0];
+        vof::adj_centroid_parts(e.tab, I, I.F[fsel], adj.part, st);
+        VOF_HIP(hipGetLastError());
+        VOF_HIP(hipEventRecord(e1, st));
+        const int nb = vof::adj_blocks(long(I.nx) * I.ny * I.nz);
+        std::vector<double> h(2 * size_t(nb));
+        VOF_HIP(hipMemcpyAsync(h.data(), adj.part, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        VOF_HIP(hipStreamSynchronize(st));
+        float ms = 0.f;
+        VOF_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        double s0 = 0.0, s1 = 0.0;
+        for (int b = 0; b < nb; ++b) {
+            s0 += h[2 * size_t(b)];
+            s1 += h[2 * size_t(b) + 1];
+        }
+        if (!(s0 > 0.0)) { set_err("reverse mode: no liquid left, the centroid is undefined"); return VOF_ERR_NUMERICS; }
+        adj.sumF = s0;
+        adj.J = s1 / s0;
+        adj.fwd_ms = ms;
+        adj.fwd_done = true;
+        *J = adj.J;
+        return VOF_OK;
+    }
+
+    static int field_slot(vof_field f) {
+        switch (f) {
+            case VOF_FIELD_F: return 0;
+            case VOF_FIELD_U: return 1;
+            case VOF_FIELD_V: return 2;
+            case VOF_FIELD_W: return 3;
+            case VOF_FIELD_P: return 4;
+            default: return -1;
+        }
+    }
+
+    vof_status adj_seed(vof_field f, const double* host, size_t count) override {
+        if (!adj.fwd_done) { set_err("adj_seed: call vof_adj_forward first"); return VOF_ERR_INVALID; }
+        const int q = field_slot(f);
+        if (q < 0 || !host || count != adj.nt) { set_err("adj_seed: field must be F/U/V/W/P with %zu values", adj.nt); return VOF_ERR_INVALID; }
+        if (!adj.seed_mem) VOF_HIP(hipMalloc(&adj.seed_mem, 5 * adj.nt * sizeof(double)));
+        adj.seed[q] = adj.seed_mem + size_t(q) * adj.nt;
+        VOF_HIP(hipMemcpyAsync(adj.seed[q], host, count * sizeof(double), hipMemcpyHostToDevice, st));
+        VOF_HIP(hipStreamSynchronize(st));
+        return VOF_OK;
+    }
+
+    vof_status adj_alloc() {
+        const size_t nt = adj.nt;
+        constexpr size_t kArrays = 16 + 6;
+        if (!adj.d) {
+            VOF_HIP(hipMalloc(&adj.d, kArrays * nt * sizeof(double)));
+            VOF_HIP(hipMalloc(&adj.tapeR, 8 * nt * sizeof(R)));
+            adj.d_arrays = kArrays;
+        }
+        double* x = adj.d;
+        auto next = [&]() { double* r = x; x += nt; return r; };
+        adj.aF = next();
+        for (int s = 0; s < 2; ++s) { adj.au[s] = next(); adj.av[s] = next(); adj.aw[s] = next(); adj.ap[s] = next(); }
+        adj.gt = next(); adj.aus = next(); adj.avs = next(); adj.aws = next();
+        adj.arhs = next(); adj.arho = next(); adj.anu = next();
+        adj.tp.gftd = next(); adj.tp.ga = next(); adj.tp.gL = next();
+        adj.tp.grp = next(); adj.tp.grm = next(); adj.tp.gdv = next();
+        R* y = adj.tapeR;
+        R** rs[8] = {&adj.tp.ftd, &adj.tp.a, &adj.tp.L, &adj.tp.rp, &adj.tp.rm, &adj.tp.C, &adj.tp.dv, &adj.tp.y};
+        for (R** r : rs) { *r = y; y += nt; }
+        return VOF_OK;
+    }
+
+    vof_status zero(double* p) {
+        VOF_HIP(hipMemsetAsync(p, 0, adj.nt * sizeof(double), st));
+        return VOF_OK;
+    }
+
+    // Reverse of step `istep` whose input state is S (stored): recompute the
+    // step's intermediates with the forward kernels, then apply the stage
+    // adjoints in reverse order.  On entry adj.{aF, au/av/aw/ap[cur]} hold the
+    // adjoint of the step's output; on exit those of its input (cur flipped).
+    vof_status reverse_step(int64_t istep, const R* S) {
+        auto& I = ih[0];
+        const size_t nt = adj.nt;
+        const R *SF = S, *Su = S + nt, *Sv = S + 2 * nt, *Sw = S + 3 * nt;
+        VOF_TRY(load_state(S, 0, 0));
+        vof::launch_predict(e, g_pred, 0, st);
+        vof::launch_rhs(e, g_rhs, 0, st);
+        enqueue_jacobi();
+        vof::launch_correct(e, g_corr, 0, psel, st);
+        static const int order[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
+        const int* od = order[int(istep % 3)];
+        vof::Geo* gd[3] = {&g_fx, &g_fy, &g_fz};
+        vof::launch_fct(e, *gd[od[0]], od[0], 0, 0, st);  // F' -> F[1]
+        vof::launch_fct(e, *gd[od[1]], od[1], 1, 0, st);  // F'' -> F[0]
+        const R* Fin[3] = {SF, I.F[1], I.F[0]};
+        const R* vel[3] = {I.u, I.v, I.w};
+        const int c = adj.cur, x = c ^ 1;
+        double* avel[3] = {adj.au[c], adj.av[c], adj.aw[c]};
+        vof::adj_ghostF(e.tab, I, adj.aF, st);
+        for (int d = 2; d >= 0; --d)
+            vof::adj_fct(e.tab, I, od[d], Fin[d], vel[od[d]], adj.aF, avel[od[d]], adj.tp, d == 2, st);
+        for (double* z : {adj.aus, adj.avs, adj.aws, adj.arhs, adj.arho, adj.anu, adj.au[x], adj.av[x], adj.aw[x]})
+            VOF_TRY(zero(z));
+        vof::adj_correct(e.tab, I, SF, I.p[psel], adj.au[c], adj.av[c], adj.aw[c], adj.aus, adj.avs, adj.aws,
+                         adj.ap[c], adj.arho, adj.au[x], adj.av[x], adj.aw[x], st);
+        if (sweeps == 0) {
+            VOF_HIP(hipMemcpyAsync(adj.ap[x], adj.ap[c], nt * sizeof(double), hipMemcpyDeviceToDevice, st));
+        } else {
+            const double* gin = adj.ap[c];
+            for (int s = 0; s < sweeps; ++s) {
+                double* out = ((sweeps - 1 - s) % 2 == 0) ? adj.ap[x] : adj.gt;
+                vof::adj_jacobi(e.tab, I, gin, out, adj.arhs, st);
+                gin = out;
+            }
+        }
+        vof::adj_rhs(e.tab, I, SF, I.us, I.vs, I.ws, adj.arhs, adj.aus, adj.avs, adj.aws, adj.arho, st);
+        vof::adj_predict(e.tab, I, SF, Su, Sv, Sw, adj.aus, adj.avs, adj.aws, adj.au[x], adj.av[x], adj.aw[x],
+                         adj.anu, adj.part, adj.acc, st);
+        vof::adj_rhonu(e.tab, I, SF, adj.arho, adj.anu, adj.aF, adj.part, adj.acc, st);
+        adj.cur = x;
+        VOF_HIP(hipGetLastError());
+        return VOF_OK;
+    }
+
+    vof_status adj_backward(double gJ, vof_adj_scalars* out) override {
+        if (!adj.fwd_done) { set_err("adj_backward: call vof_adj_forward first"); return VOF_ERR_INVALID; }
+        VOF_TRY(adj_alloc());
+        auto& I = ih[0];
+        const size_t nt = adj.nt;
+        hipEvent_t e0, e1;
+        VOF_HIP(hipEventCreate(&e0));
+        VOF_HIP(hipEventCreate(&e1));
+        VOF_HIP(hipEventRecord(e0, st));
+        VOF_HIP(hipMemsetAsync(adj.acc, 0, vof::ADJ_NSCALAR * sizeof(double), st));
+        adj.cur = 0;
+        vof::adj_centroid(e.tab, I, adj.J, adj.sumF, gJ, adj.seed[0], adj.aF, st);
+        double* init[4] = {adj.au[0], adj.av[0], adj.aw[0], adj.ap[0]};
+        for (int q = 0; q < 4; ++q) {
+            if (adj.seed[q + 1])
+                VOF_HIP(hipMemcpyAsync(init[q], adj.seed[q + 1], nt * sizeof(double), hipMemcpyDeviceToDevice, st));
+            else
+                VOF_TRY(zero(init[q]));
+        }
+        const R* seg = adj_state(size_t(adj.nck));
+        for (int k = adj.nck - 1; k >= 0; --k) {
+            const int s0 = k * adj.K, len = std::min(adj.n, s0 + adj.K) - s0;
+            VOF_TRY(load_state(adj_state(size_t(k)), 0, 0));
+            for (int m = 0; m < len; ++m) {
+                VOF_TRY(save_state(const_cast<R*>(seg) + size_t(m) * 5 * nt));
+                if (m + 1 < len) VOF_TRY(replay(adj.first + s0 + m));
+            }
+            for (int m = len - 1; m >= 0; --m)
+                VOF_TRY(reverse_step(adj.first + s0 + m, seg + size_t(m) * 5 * nt));
+        }
+        VOF_TRY(load_state(adj_state(size_t(adj.nck) + size_t(adj.K)), adj.f_fin, adj.p_fin));
+        VOF_HIP(hipEventRecord(e1, st));
+        double acc[vof::ADJ_NSCALAR];
+        VOF_HIP(hipMemcpyAsync(acc, adj.acc, sizeof(acc), hipMemcpyDeviceToHost, st));
+        VOF_HIP(hipStreamSynchronize(st));
+        float ms = 0.f;
+        VOF_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        adj.bwd_ms = ms;
+        adj.bwd_done = true;
+        if (out) {
+            out->d_rho_l = acc[vof::ADJ_RHO_L];
+            out->d_rho_g = acc[vof::ADJ_RHO_G];
+            out->d_nu_l = acc[vof::ADJ_NU_L];
+            out->d_nu_g = acc[vof::ADJ_NU_G];
+            out->d_gx = acc[vof::ADJ_GX];
+            out->d_gy = acc[vof::ADJ_GY];
+            out->d_gz = acc[vof::ADJ_GZ];
+            out->d_sigma = 0.0;
+        }
+        return VOF_OK;
+    }
+
+    vof_status adj_download(vof_field f, double* host, size_t count) override {
+        if (!adj.bwd_done) { set_err("adj_download: call vof_adj_backward first"); return VOF_ERR_INVALID; }
+        const int q = field_slot(f);
+        if (q < 0 || !host || count != adj.nt) { set_err("adj_download: field must be F/U/V/W/P with %zu values", adj.nt); return VOF_ERR_INVALID; }
+        const int c = adj.cur;
+        const double* src[5] = {adj.aF, adj.au[c], adj.av[c], adj.aw[c], adj.ap[c]};
+        VOF_HIP(hipMemcpyAsync(host, src[q], count * sizeof(double), hipMemcpyDeviceToHost, st));
+        VOF_HIP(hipStreamSynchronize(st));
+        return VOF_OK;
+    }
+
+    void adj_times(double* fwd, double* bwd) const override {
+        if (fwd) *fwd = adj.fwd_ms;
+        if (bwd) *bwd = adj.bwd_ms;
+    }
 };
 
 vof_status validate(const vof_params& p) {
@@ -1059,5 +1363,34 @@ vof_status vof_batch_set_tuning(vof_batch* b, const vof_tuning* t) {
 }
 int64_t vof_batch_interior_cells(const vof_batch* b) { return (b && b->e) ? b->e->interior_cells() : 0; }
 int vof_batch_size(const vof_batch* b) { return (b && b->e) ? b->e->ninst() : 0; }
+
+// ---------------------------------------------------------------- reverse mode
+vof_status vof_adj_forward(vof_solver* s, int64_t first_istep, int nsteps, int checkpoint_every, double* J) {
+    VOF_NEED(s);
+    return s->e->adj_forward(first_istep, nsteps, checkpoint_every, J);
+}
+vof_status vof_adj_seed(vof_solver* s, vof_field f, const double* host, size_t count) {
+    VOF_NEED(s);
+    return s->e->adj_seed(f, host, count);
+}
+vof_status vof_adj_backward(vof_solver* s, double gJ, vof_adj_scalars* out) {
+    VOF_NEED(s);
+    return s->e->adj_backward(gJ, out);
+}
+vof_status vof_adj_download(vof_solver* s, vof_field f, double* host, size_t count) {
+    VOF_NEED(s);
+    return s->e->adj_download(f, host, count);
+}
+vof_status vof_adj_times(const vof_solver* s, double* fwd_ms, double* bwd_ms) {
+    if (!s || !s->e) { set_err("null handle"); return VOF_ERR_INVALID; }
+    s->e->adj_times(fwd_ms, bwd_ms);
+    return VOF_OK;
+}
+vof_status vof_adj_release(vof_solver* s) {
+    VOF_NEED(s);
+    VOF_TRY(s->e->sync());
+    s->e->adj_release();
+    return VOF_OK;
+}
 
 }  // extern "C"

@@ -55,6 +55,11 @@ class VofDiag(ctypes.Structure):
     ]
 
 
+class VofAdjScalars(ctypes.Structure):
+    _fields_ = [("d_rho_l", c_double), ("d_rho_g", c_double), ("d_nu_l", c_double), ("d_nu_g", c_double),
+                ("d_gx", c_double), ("d_gy", c_double), ("d_gz", c_double), ("d_sigma", c_double)]
+
+
 class VofTuning(ctypes.Structure):
     _fields_ = [("kz_predict", c_int32), ("kz_rhs", c_int32), ("kz_jacobi", c_int32), ("kz_correct", c_int32),
                 ("fct_chunk", c_int32), ("jacobi_tblock", c_int32), ("kz_fct_x", c_int32)]
@@ -143,6 +148,12 @@ def load_library(path: str = LIB_PATH):
         "vof_batch_set_tuning": (c_int, [B, POINTER(VofTuning)]),
         "vof_batch_interior_cells": (c_int64, [B]),
         "vof_batch_size": (c_int, [B]),
+        "vof_adj_forward": (c_int, [S, c_int64, c_int, c_int, POINTER(c_double)]),
+        "vof_adj_seed": (c_int, [S, c_int, c_void_p, c_size_t]),
+        "vof_adj_backward": (c_int, [S, c_double, POINTER(VofAdjScalars)]),
+        "vof_adj_download": (c_int, [S, c_int, c_void_p, c_size_t]),
+        "vof_adj_times": (c_int, [S, POINTER(c_double), POINTER(c_double)]),
+        "vof_adj_release": (c_int, [S]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -262,6 +273,41 @@ class Solver:
     @property
     def interior_cells(self) -> int:
         return int(self.lib.vof_interior_cells(self.h))
+
+    # ---- reverse mode: objective J = liquid-centroid height after the last step
+    def adj_forward(self, first_istep: int, nsteps: int, checkpoint_every: int = 20) -> float:
+        """Advance nsteps from the current state keeping checkpoints; returns J."""
+        J = c_double()
+        _check(self.lib.vof_adj_forward(self.h, first_istep, nsteps, checkpoint_every, byref(J)))
+        return J.value
+
+    def adj_seed(self, field: str, arr: np.ndarray) -> None:
+        """Extra FP64 seed of a final-state field (cleared by adj_forward)."""
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError(f"shape {a.shape} != {self.shape}")
+        _check(self.lib.vof_adj_seed(self.h, FIELDS[field], a.ctypes.data, a.size))
+
+    def adj_backward(self, gJ: float = 1.0) -> dict:
+        """Reverse sweep; returns the parameter gradients (times gJ)."""
+        s = VofAdjScalars()
+        _check(self.lib.vof_adj_backward(self.h, float(gJ), byref(s)))
+        return {k[2:]: getattr(s, k) for k, _ in VofAdjScalars._fields_}
+
+    def adj_download(self, field: str) -> np.ndarray:
+        """Adjoint (FP64) of an initial-state field: F, U, V, W or P."""
+        a = np.empty(self.shape, dtype=np.float64)
+        _check(self.lib.vof_adj_download(self.h, FIELDS[field], a.ctypes.data, a.size))
+        return a
+
+    def adj_times(self) -> tuple[float, float]:
+        """Device milliseconds of the last adj_forward and adj_backward."""
+        f, b = c_double(), c_double()
+        _check(self.lib.vof_adj_times(self.h, byref(f), byref(b)))
+        return f.value, b.value
+
+    def adj_release(self) -> None:
+        _check(self.lib.vof_adj_release(self.h))
 
 
 class Batch:

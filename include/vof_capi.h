@@ -214,6 +214,38 @@ vof_status vof_batch_autotune(vof_batch* b, int64_t first_istep, int steps_per_t
 int64_t vof_batch_interior_cells(const vof_batch* b);
 int vof_batch_size(const vof_batch* b);
 
+/* ---------------------------------------------------------------------------
+ * Reverse mode (no reference counterpart): the forward/backward pair behind a
+ * torch.autograd.Function.  The objective is the liquid-centroid height
+ * J = sum(F z) / sum(F) over interior cells after the last step (z = cell
+ * centre).  Adjoint arithmetic is FP64 whatever the field precision.
+ *
+ * vof_adj_forward advances nsteps steps from the solver's current state
+ * (exactly as vof_step does) keeping a copy of the state every
+ * checkpoint_every steps, and returns J.  vof_adj_backward then runs the
+ * steps in reverse, recomputing each checkpoint interval forward once and
+ * each step's intermediates from its stored state, and leaves the solver at
+ * the state vof_adj_forward ended on.  The seed of the final state is
+ * gJ * dJ/d(state) plus any field set with vof_adj_seed between the two calls
+ * (vof_adj_forward clears them), so gJ = 0 with seeds gives a general
+ * vector-Jacobian product.
+ * Supported: flags 0 or VOF_FLAG_WATCHDOG. */
+typedef struct vof_adj_scalars {
+    double d_rho_l, d_rho_g, d_nu_l, d_nu_g; /* dJ/d parameter */
+    double d_gx, d_gy, d_gz;
+    double d_sigma; /* 0: sigma does not enter the reference physics */
+} vof_adj_scalars;
+vof_status vof_adj_forward(vof_solver* s, int64_t first_istep, int nsteps, int checkpoint_every, double* J);
+/* Seed of a final-state field (F, U, V, W or P), count = stored cells, FP64. */
+vof_status vof_adj_seed(vof_solver* s, vof_field f, const double* host, size_t count);
+vof_status vof_adj_backward(vof_solver* s, double gJ, vof_adj_scalars* out);
+/* Adjoint of the initial state after vof_adj_backward: F, U, V, W or P. */
+vof_status vof_adj_download(vof_solver* s, vof_field f, double* host, size_t count);
+/* Device milliseconds of the last vof_adj_forward / vof_adj_backward. */
+vof_status vof_adj_times(const vof_solver* s, double* fwd_ms, double* bwd_ms);
+/* Free the checkpoints and adjoint arrays. */
+vof_status vof_adj_release(vof_solver* s);
+
 #ifdef __cplusplus
 }
 #endif

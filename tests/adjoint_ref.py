@@ -98,14 +98,17 @@ def jacobi_coeffs(P):
     i = torch.arange(1, nx + 1).view(1, 1, -1)
     j = torch.arange(1, ny + 1).view(1, -1, 1)
     k = torch.arange(1, nz + 1).view(-1, 1, 1)
-    ax, ay, az = P.dxi * P.dxi, P.dyi * P.dyi, P.dzi * P.dzi
+    # FP64 scalars as tensors: torch.where on two Python floats would build a
+    # float32 tensor and round the coefficients.
+    ax, ay, az = (torch.tensor(x, dtype=T) for x in (P.dxi * P.dxi, P.dyi * P.dyi, P.dzi * P.dzi))
+    z = torch.zeros((), dtype=T)
     sh = (nz, ny, nx)
-    ae = torch.where(i != nx, ax, 0.0).expand(sh).to(T)
-    aw = torch.where(i != 1, ax, 0.0).expand(sh).to(T)
-    an = torch.where(j != ny, ay, 0.0).expand(sh).to(T)
-    as_ = torch.where(j != 1, ay, 0.0).expand(sh).to(T)
-    af = torch.where(k != nz, az, 0.0).expand(sh).to(T)
-    ab = torch.where(k != 1, az, 0.0).expand(sh).to(T)
+    ae = torch.where(i != nx, ax, z).expand(sh)
+    aw = torch.where(i != 1, ax, z).expand(sh)
+    an = torch.where(j != ny, ay, z).expand(sh)
+    as_ = torch.where(j != 1, ay, z).expand(sh)
+    af = torch.where(k != nz, az, z).expand(sh)
+    ab = torch.where(k != 1, az, z).expand(sh)
     ap = -(ae + aw + an + as_ + af + ab)
     return ae, aw, an, as_, af, ab, ap
 
@@ -121,12 +124,13 @@ def jacobi(P, p, rhs, coeffs):
 def correction(P, us, vs, ws, p, rho):
     """update_{u,v,w}_kernel (:441-493) then the velocity BC closed forms."""
     K = J = I = slice(1, -1)
+    dt = torch.tensor(P.dt, dtype=T)  # python_float / tensor would multiply by a rounded reciprocal
     r = 0.5 * (rho[K, J, 2:-1] + rho[K, J, 1:-2])
-    u_in = us[K, J, 2:-1] - P.dt / r * (p[K, J, 2:-1] - p[K, J, 1:-2]) * P.dxi
+    u_in = us[K, J, 2:-1] - dt / r * (p[K, J, 2:-1] - p[K, J, 1:-2]) * P.dxi
     r = 0.5 * (rho[K, 2:-1, I] + rho[K, 1:-2, I])
-    v_in = vs[K, 2:-1, I] - P.dt / r * (p[K, 2:-1, I] - p[K, 1:-2, I]) * P.dyi
+    v_in = vs[K, 2:-1, I] - dt / r * (p[K, 2:-1, I] - p[K, 1:-2, I]) * P.dyi
     r = 0.5 * (rho[2:-1, J, I] + rho[1:-2, J, I])
-    w_in = ws[2:-1, J, I] - P.dt / r * (p[2:-1, J, I] - p[1:-2, J, I]) * P.dzi
+    w_in = ws[2:-1, J, I] - dt / r * (p[2:-1, J, I] - p[1:-2, J, I]) * P.dzi
     return u_in, v_in, w_in
 
 

@@ -36,7 +36,8 @@ def test_torch_step_matches_oracle_fp64(first):
     o.step(first, 3)
     for name, t in zip(("F", "U", "V", "W", "P"), ts):
         ref = o.get(name)
-        np.testing.assert_allclose(t.numpy(), ref, rtol=1e-10, atol=1e-10, err_msg=name)
+        # bit for bit (IEEE ==): same operations, same association, FP64 scalars
+        np.testing.assert_array_equal(t.numpy(), ref, err_msg=name)
 
 
 def test_torch_gradient_matches_central_differences():
