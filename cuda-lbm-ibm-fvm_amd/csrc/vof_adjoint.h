@@ -7,42 +7,52 @@
 
 namespace vof {
 
-// Per-line scratch of the FCT reverse sweep: forward intermediates (R) and
-// line-private adjoint accumulators (FP64), each one field in size.
-template <class R>
-struct FctTape {
-    R *ftd, *a, *L, *rp, *rm, *C, *dv, *y;
-    double *gftd, *ga, *gL, *grp, *grm, *gdv;
-};
-
 // Scalar gradients accumulated on the device, in this order.
 enum AdjScalar { ADJ_RHO_L = 0, ADJ_RHO_G, ADJ_NU_L, ADJ_NU_G, ADJ_GX, ADJ_GY, ADJ_GZ, ADJ_NSCALAR };
 
 inline int adj_blocks(long n) { return int((n + 255) / 256); }
+// doubles needed by the `part` scratch of the launchers below
+inline size_t adj_part_size(int nx, int ny, int nz) {
+    const size_t a = size_t(adj_blocks(long(nx) * ny)), b = size_t((nx + 31) / 32) * size_t((ny + 7) / 8);
+    return size_t(ADJ_NSCALAR) * (a > b ? a : b) * size_t(nz);
+}
 
-template <class R> void adj_centroid_parts(const Inst<R>* tab, const Inst<R>& h, const R* F, double* part, hipStream_t st);
+// Work-group partial sums (sum F, sum F z) of the centroid objective; returns
+// the number of work-groups (2 doubles each in part).
+template <class R> int adj_centroid_parts(const Inst<R>* tab, const Inst<R>& h, const R* F, double* part, hipStream_t st);
+// aF = gJ dJ/dF + seed folded through the ghost fill (seed may be null).
 template <class R> void adj_centroid(const Inst<R>* tab, const Inst<R>& h, double J, double sumF, double gJ,
                                      const double* seed, double* aF, hipStream_t st);
-template <class R> void adj_ghostF(const Inst<R>* tab, const Inst<R>& h, double* aF, hipStream_t st);
-template <class R> void adj_fct(const Inst<R>* tab, const Inst<R>& h, int dir, const R* F, const R* V, double* aF,
-                                double* aV, const FctTape<R>& tp, int fin, hipStream_t st);
+// One FCT sweep in reverse: aFo (interior) -> aFi (interior), aV += velocity adjoint.
+template <class R> void adj_fct(const Inst<R>* tab, const Inst<R>& h, int dir, const R* F, const R* V,
+                                const double* aFo, double* aFi, double* aV, int fin, hipStream_t st);
+// Correction + velocity BC in reverse: au* = (assigned), ap += , arho = , plane-0 copies -> au0 (assigned).
 template <class R> void adj_correct(const Inst<R>* tab, const Inst<R>& h, const R* F, const R* p, const double* au,
                                     const double* av, const double* aw, double* aus, double* avs, double* aws,
                                     double* ap, double* arho, double* au0, double* av0, double* aw0, hipStream_t st);
-template <class R> void adj_jacobi(const Inst<R>* tab, const Inst<R>& h, const double* gin, double* gout,
-                                   double* arhs, hipStream_t st);
+// ns (1 or 2) Jacobi sweeps in reverse, on h = D^-1 gbar: in_g: the input is
+// gbar, out_g: write gbar instead of h, first: assign arhs instead of adding.
+template <class R> void adj_jacobi(const Inst<R>* tab, const Inst<R>& h, const double* in, double* out,
+                                   double* arhs, int ns, int in_g, int out_g, int first, hipStream_t st);
 template <class R> void adj_rhs(const Inst<R>* tab, const Inst<R>& h, const R* F, const R* us, const R* vs,
                                 const R* ws, const double* arhs, double* aus, double* avs, double* aws, double* arho,
                                 hipStream_t st);
-// Adds dJ/dg (3 values) to acc[ADJ_GX..ADJ_GZ]; part needs 3 * adj_blocks(nx*ny*nz) doubles.
+// Predictor + rho/nu in reverse: au += (atomics), aF += ; adds dJ/d(rho_l, rho_g,
+// nu_l, nu_g, gx, gy, gz) to acc[0..7).  part: adj_part_size doubles.
 template <class R> void adj_predict(const Inst<R>* tab, const Inst<R>& h, const R* F, const R* u, const R* v,
-                                    const R* w, const double* aus, const double* avs, const double* aws, double* au,
-                                    double* av, double* aw, double* anu, double* part, double* acc, hipStream_t st);
-// Adds the F adjoint of rho and nu to aF and dJ/d(rho_l, rho_g, nu_l, nu_g) to
-// acc[ADJ_RHO_L..ADJ_NU_G]; part needs 4 * adj_blocks(ntot) doubles.
-template <class R> void adj_rhonu(const Inst<R>* tab, const Inst<R>& h, const R* F, const double* arho,
-                                  const double* anu, double* aF, double* part, double* acc, hipStream_t st);
-// dst += src over n doubles.
-void adj_axpy(const double* src, double* dst, long n, hipStream_t st);
+                                    const R* w, const double* aus, const double* avs, const double* aws,
+                                    const double* arho, double* au, double* av, double* aw, double* aF, double* part,
+                                    double* acc, hipStream_t st);
+// a = b = c = 0 over n doubles.
+void adj_zero3(double* a, double* b, double* c, long n, hipStream_t st);
+
+// Up to five copies of n elements in one launch.
+template <class R>
+struct FieldCopy {
+    const R* src[5];
+    R* dst[5];
+    int nf;
+};
+template <class R> void copy_fields(const FieldCopy<R>& fc, long n, hipStream_t st);
 
 }  // namespace vof

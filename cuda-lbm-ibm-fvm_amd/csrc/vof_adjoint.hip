@@ -1,14 +1,19 @@
 // vof_adjoint.hip — hand-written reverse-mode kernels of the VOF step.
 //
 // Each forward stage has a kernel that maps the adjoint of its outputs to the
-// adjoint of its inputs (a vector-Jacobian product) in FP64, evaluated at the
-// forward values of the step being reversed (FP32 or FP64 fields, recomputed
-// by the forward kernels themselves).  The switches of the scheme (upwind
-// side, limiter min/max, clips) are differentiated along the branch the
-// forward took; where a min/max has two equal operands the derivative is
-// split evenly between them, which is what a central difference sees at such
-// a kink (clamp01 at exactly 0 or 1: 1/2).  Scatter-type transposes use FP64
-// atomic adds; gathers are written per output cell.
+// adjoint of its inputs (a vector-Jacobian product).  Adjoints are FP64; the
+// stage is linearised in FP64 at its forward inputs (the fields of the step
+// being reversed, FP32 or FP64, recomputed by the forward kernels), i.e. the
+// kernels differentiate the FP64 form of each stage along the trajectory the
+// solver actually took.  In FP64 mode that is exactly the derivative of the
+// computed step.  The switches of the scheme (upwind side, limiter min/max,
+// clips) are differentiated along the branch taken; where a min/max has two
+// equal operands the derivative is split evenly between them, which is what a
+// central difference sees at such a kink (clamp01 at exactly 0 or 1: 1/2).
+//
+// Layout: every adjoint array is one field, (nx+2)(ny+2)(nz+2) doubles, x
+// fastest.  Column kernels use a 2-D grid: x covers the interior (x,y) plane
+// (or the stored plane), y the z index.
 #include "vof_adjoint.h"
 
 namespace vof {
@@ -17,11 +22,12 @@ namespace {
 
 __device__ __forceinline__ double tie_w(bool win, bool tie) { return win ? 1.0 : (tie ? 0.5 : 0.0); }
 
+__device__ __forceinline__ double clamp01d(double v) { return ((1.0 + v) - fmax(1.0, v)) - fmin(0.0, v); }
+
 // d clamp01(v)/dv: 1 inside (0,1), 1/2 at exactly 0 or 1, 0 outside.
-template <class R>
-__device__ __forceinline__ double dclamp(R v) {
-    if (v > R(0) && v < R(1)) return 1.0;
-    if (v == R(0) || v == R(1)) return 0.5;
+__device__ __forceinline__ double dclamp(double v) {
+    if (v > 0.0 && v < 1.0) return 1.0;
+    if (v == 0.0 || v == 1.0) return 0.5;
     return 0.0;
 }
 
@@ -29,30 +35,43 @@ __device__ __forceinline__ void aadd(double* p, double x) {
     if (x != 0.0) atomicAdd(p, x);
 }
 
-struct Cell {
-    int i, j, k, c;
-};
-
-// Thread t -> stored cell (with_ghosts) or interior cell (x fastest).
 template <class R>
-__device__ __forceinline__ bool cell_of(const Inst<R>& I, long t, bool with_ghosts, Cell& q) {
-    const int ex = with_ghosts ? 0 : 1;
-    const long nxr = I.nx + 2 - 2 * ex, nyr = I.ny + 2 - 2 * ex, nzr = I.nz + 2 - 2 * ex;
-    if (t >= nxr * nyr * nzr) return false;
-    q.i = int(t % nxr) + ex;
-    q.j = int((t / nxr) % nyr) + ex;
-    q.k = int(t / (nxr * nyr)) + ex;
-    q.c = q.k * I.sy + q.j * I.sx + q.i;
+__device__ __forceinline__ double rho_d(const Inst<R>& I, R F) {
+    const double f = clamp01d(double(F));
+    return double(I.rho_g) * (1.0 - f) + double(I.rho_l) * f;
+}
+template <class R>
+__device__ __forceinline__ double nu_d(const Inst<R>& I, R F) {
+    const double f = clamp01d(double(F));
+    return double(I.nu_l) * f + double(I.nu_g) * (1.0 - f);
+}
+
+// Interior cell of a 2-D launch (x: interior plane, y: k-1); false past the end.
+template <class R>
+__device__ __forceinline__ bool interior_cell(const Inst<R>& I, int& i, int& j, int& k, int& c) {
+    const int t = int(blockIdx.x) * 256 + int(threadIdx.x);
+    if (t >= I.nx * I.ny) return false;
+    j = 1 + t / I.nx;
+    i = 1 + (t - (j - 1) * I.nx);
+    k = 1 + int(blockIdx.y);
+    c = k * I.sy + j * I.sx + i;
     return true;
 }
 
+// Stored cell of a 2-D launch (x: stored plane, y: k).
 template <class R>
-__device__ __forceinline__ bool interior(const Inst<R>& I, int i, int j, int k) {
-    return i >= 1 && i <= I.nx && j >= 1 && j <= I.ny && k >= 1 && k <= I.nz;
+__device__ __forceinline__ bool stored_cell(const Inst<R>& I, int& i, int& j, int& k, int& c) {
+    const int t = int(blockIdx.x) * 256 + int(threadIdx.x);
+    if (t >= I.sy) return false;
+    j = t / I.sx;
+    i = t - j * I.sx;
+    k = int(blockIdx.y);
+    c = k * I.sy + t;
+    return true;
 }
 
-// Sum NQ per-lane values over the 256 lanes of a work-group into part[blk*NQ + q].
-// Every lane of the work-group must call it.
+// Sum NQ per-lane values over the 256 lanes of a work-group into
+// part[blk*NQ + q] (blk = linear work-group index).  Every lane must call it.
 template <int NQ>
 __device__ __forceinline__ void block_parts(const double (&v)[NQ], double* part) {
     __shared__ double sh[NQ][256];
@@ -63,8 +82,10 @@ __device__ __forceinline__ void block_parts(const double (&v)[NQ], double* part)
             for (int q = 0; q < NQ; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0)
-        for (int q = 0; q < NQ; ++q) part[size_t(blockIdx.x) * NQ + q] = sh[q][0];
+    if (threadIdx.x == 0) {
+        const size_t blk = size_t(blockIdx.y) * gridDim.x + blockIdx.x;
+        for (int q = 0; q < NQ; ++q) part[blk * NQ + q] = sh[q][0];
+    }
 }
 
 }  // namespace
@@ -96,221 +117,421 @@ template <class R>
 __global__ __launch_bounds__(256) void k_centroid_parts(const Inst<R>* __restrict__ tab, const R* __restrict__ F,
                                                         double* __restrict__ part) {
     const Inst<R>& I = tab[0];
-    Cell q;
+    int i, j, k, c;
     double v[2] = {0.0, 0.0};
-    if (cell_of(I, long(blockIdx.x) * 256 + threadIdx.x, false, q)) {
-        const double f = double(F[q.c]);
+    if (interior_cell(I, i, j, k, c)) {
+        const double f = double(F[c]);
         v[0] = f;
-        v[1] = f * ((double(q.k) - 0.5) * double(I.dz));
+        v[1] = f * ((double(k) - 0.5) * double(I.dz));
     }
     block_parts<2>(v, part);
 }
 
-// aF = gJ * dJ/dF (+ seed): gJ * (z - J) / sum(F) on interior cells, 0 on ghosts.
+// Seed of the final state's F: aF = gJ * dJ/dF + (seed folded through the
+// ghost fill).  The stored ghosts of F copy the nearest interior cell, so the
+// adjoint of a ghost entry is added to that cell; ghost entries of aF are 0.
 template <class R>
 __global__ __launch_bounds__(256) void k_centroid_adj(const Inst<R>* __restrict__ tab, double J, double sumF,
                                                       double gJ, const double* __restrict__ seed,
                                                       double* __restrict__ aF) {
     const Inst<R>& I = tab[0];
-    Cell q;
-    if (!cell_of(I, long(blockIdx.x) * 256 + threadIdx.x, true, q)) return;
-    double g = interior(I, q.i, q.j, q.k) ? gJ * (((double(q.k) - 0.5) * double(I.dz)) - J) / sumF : 0.0;
-    if (seed) g += seed[q.c];
-    aF[q.c] = g;
-}
-
-// Transpose of the F ghost fill (every ghost copies the nearest interior
-// cell): ghost adjoints are added to their source cell and cleared.
-template <class R>
-__global__ __launch_bounds__(256) void k_ghostF_adj(const Inst<R>* __restrict__ tab, double* __restrict__ aF) {
-    const Inst<R>& I = tab[0];
-    Cell q;
-    if (!cell_of(I, long(blockIdx.x) * 256 + threadIdx.x, true, q)) return;
-    const int ci = min(max(q.i, 1), I.nx), cj = min(max(q.j, 1), I.ny), ck = min(max(q.k, 1), I.nz);
-    if (ci == q.i && cj == q.j && ck == q.k) return;
-    const double g = aF[q.c];
-    aF[q.c] = 0.0;
-    aadd(&aF[ck * I.sy + cj * I.sx + ci], g);
+    int i, j, k, c;
+    if (!stored_cell(I, i, j, k, c)) return;
+    const bool in = i >= 1 && i <= I.nx && j >= 1 && j <= I.ny && k >= 1 && k <= I.nz;
+    if (!in) { aF[c] = 0.0; return; }
+    double g = gJ * (((double(k) - 0.5) * double(I.dz)) - J) / sumF;
+    if (seed) {
+        int ii[3], jj[3], kk[3], ni = 0, nj = 0, nk = 0;
+        ii[ni++] = i; if (i == 1) ii[ni++] = 0; if (i == I.nx) ii[ni++] = I.nx + 1;
+        jj[nj++] = j; if (j == 1) jj[nj++] = 0; if (j == I.ny) jj[nj++] = I.ny + 1;
+        kk[nk++] = k; if (k == 1) kk[nk++] = 0; if (k == I.nz) kk[nk++] = I.nz + 1;
+        for (int a = 0; a < ni; ++a)
+            for (int b = 0; b < nj; ++b)
+                for (int d = 0; d < nk; ++d) g += seed[kk[d] * I.sy + jj[b] * I.sx + ii[a]];
+    }
+    aF[c] = g;
 }
 
 // ---------------------------------------------------------------------------
-// FCT sweep reverse, one lane per line along the sweep direction.  The lane
-// re-runs the forward sweep of its line with the forward's association
-// (intermediates kept in the line's slots of the tape arrays), then walks the
-// four stages backwards, accumulating adjoints of its own line only (no races).
-// F: sweep input; V: face velocity along the line; aF: on entry the adjoint of
-// the sweep output, on exit that of the sweep input (in place: ghost entries
-// pass through, their only use is a flux through a wall face, whose velocity
-// is zero); aV += the face-velocity adjoints.  fin: this is the last sweep of
-// the step, whose output went through clamp01 a second time.
+// FCT sweep reverse as a march along the line.
+//
+// Forward quantities of a line (positions 0..n+1; face f lies between cells
+// f-1 and f; everything outside faces 1..n+1 / cells 1..n is zero, as in the
+// forward sweep):
+//   face f   L = vdt F(donor), H = vdt F(downwind), a = H - L     <- F(f-1), F(f), V(f)
+//   cell c   dv = vol - kdv (V(c+1) - V(c)),
+//            y1 = (F(c) + s1(L(c) - L(c+1))) vol / dv,  Ftd = clip(y1)
+//   cell c   rp, rm (Zalesak ratios)                            <- Ftd(c-1..c+1), a(c), a(c+1)
+//   face f   C = a(f) >= 0 ? min(rp(f), rm(f-1)) : min(rp(f-1), rm(f))   (f >= 2)
+//   cell c   yv = Ftd - ((aC)(c+1) - (aC)(c)) / s4 * vol / dv,  Fn = clamp01(yv)
+// For the y and z sweeps a lane walks a chunk of its line (lanes along x, so
+// every load is coalesced) with the front at position j.  It forms the
+// forward at j (face), j-1 (cell), j-2 (ratios, limiter) and j-3 (yv), and the
+// reverse follows: yv and the aC face adjoint at j-3, the ratios at j-4, Ftd,
+// dv and the face fluxes at j-5, F at j-6.  Every quantity lives in a register
+// window, X<d> = value at position j-d.  A chunk of cells m0..m1 starts the
+// walk at m0-6 so that every adjoint it writes is complete.  (The x sweep,
+// whose lines run along the contiguous axis, is k_fct_adj_x.)
+//
+// aFo (adjoint of the sweep output, interior cells) -> aFi (adjoint of the
+// sweep input, interior cells), aV += face-velocity adjoints.  Ghost entries
+// are neither read nor written: a ghost F only enters a flux through a wall
+// face, whose velocity is zero, so its adjoint is zero.  fin: the output went
+// through clamp01 a second time (last sweep of a step).
 template <class R, int DIR>
 __global__ __launch_bounds__(256) void k_fct_adj(const Inst<R>* __restrict__ tab, const R* __restrict__ F,
-                                                 const R* __restrict__ V, double* __restrict__ aF,
-                                                 double* __restrict__ aV, FctTape<R> tp, int fin) {
+                                                 const R* __restrict__ V, const double* __restrict__ aFo,
+                                                 double* __restrict__ aFi, double* __restrict__ aV, int ch,
+                                                 int fin) {
+    static_assert(DIR == 1 || DIR == 2, "marches serve the y and z sweeps");
     const Inst<R>& I = tab[0];
-    const int n = (DIR == 0) ? I.nx : (DIR == 1) ? I.ny : I.nz;
-    const int na = (DIR == 0) ? I.ny : I.nx;  // transverse axis along the lanes
+    const int n = (DIR == 1) ? I.ny : I.nz;
+    const int na = I.nx;  // lanes along x
     const int nb = (DIR == 2) ? I.ny : I.nz;
-    const long t = long(blockIdx.x) * 256 + threadIdx.x;
-    if (t >= long(na) * nb) return;
-    const int ta = 1 + int(t % na), tb = 1 + int(t / na);
-    int base, S;
-    if (DIR == 0) { base = tb * I.sy + ta * I.sx; S = 1; }     // x line at (y, z) = (ta, tb)
-    else if (DIR == 1) { base = tb * I.sy + ta; S = I.sx; }    // y line at (x, z) = (ta, tb)
-    else { base = tb * I.sx + ta; S = I.sy; }                  // z line at (x, y) = (ta, tb)
-    const R dt = I.dt, vol = I.vol;
-    const R kdv = (DIR == 0) ? I.dtdydz : (DIR == 1) ? I.dtdxdz : I.dtdxdy;
-    const R qf = (DIR == 2) ? I.dz : I.dx;
-    const R s4 = (DIR == 2) ? I.dz : I.dy;
-    auto s1 = [&](R d) -> R {
-        if (DIR == 0) return d * I.dy * I.dz / I.vol;
-        if (DIR == 1) return d * I.dy / I.dxdy;
-        return d * I.dy * I.dx / I.vol;
+    const int nlines = na * nb;
+    const int nbt = (nlines + 255) >> 8;
+    const int chn = int(blockIdx.x) / nbt;
+    const int t = (int(blockIdx.x) - chn * nbt) * 256 + int(threadIdx.x);
+    if (t >= nlines) return;
+    const int m0 = 1 + chn * ch;
+    if (m0 > n) return;
+    const int m1 = min(n, m0 + ch - 1);
+    const int tb = 1 + t / na, ta = 1 + (t - (tb - 1) * na);
+    // line of x = ta at z = tb (y sweep) or y = tb (z sweep)
+    const int base = (DIR == 1) ? tb * I.sy + ta : tb * I.sx + ta;
+    const int S = (DIR == 1) ? I.sx : I.sy;
+    const double dt = I.dt, vol = I.vol, dx = I.dx, dy = I.dy, dz = I.dz;
+    const double kdv = (DIR == 1) ? double(I.dtdxdz) : double(I.dtdxdy);
+    const double qf = (DIR == 2) ? dz : dx;
+    const double s4 = (DIR == 2) ? dz : dy;
+    const double dxdy = I.dxdy;
+    // stage-1 scaling with the forward's association, and its derivative
+    auto s1 = [&](double d) {
+        if (DIR == 1) return d * dy / dxdy;
+        return d * dy * dx / vol;
     };
-    const double ds1 = (DIR == 0) ? double(I.dy) * double(I.dz) / double(I.vol)
-                     : (DIR == 1) ? double(I.dy) / double(I.dxdy)
-                                  : double(I.dy) * double(I.dx) / double(I.vol);
-    auto at = [&](int m) { return base + m * S; };
-    // ---- forward re-run (face f between cells f-1 and f, stored at slot f)
-    for (int f = 1; f <= n + 1; ++f) {
-        const R vf = V[at(f)], vdt = vf * dt, flo = F[at(f - 1)], fhi = F[at(f)];
-        const R L = (vf >= R(0)) ? vdt * flo : vdt * fhi;
-        const R H = (vf <= R(0)) ? vdt * flo : vdt * fhi;
-        tp.L[at(f)] = L;
-        tp.a[at(f)] = H - L;
-    }
-    for (int c = 1; c <= n; ++c) {
-        const R dv = vol - kdv * (V[at(c + 1)] - V[at(c)]);
-        R o = (F[at(c)] + s1(tp.L[at(c)] - tp.L[at(c + 1)])) * vol / dv;
-        tp.y[at(c)] = o;
-        if (o > R(1) || o < R(0)) o = clamp01(o);
-        tp.ftd[at(c)] = o;
-        tp.dv[at(c)] = dv;
-    }
-    auto ftd = [&](int c) { return (c >= 1 && c <= n) ? tp.ftd[at(c)] : R(0); };
-    for (int c = 1; c <= n; ++c) {
-        const R fc = ftd(c), fl = ftd(c - 1), fr = ftd(c + 1);
-        const R fmx = vmax(fc, vmax(fl, fr)), fmn = vmin(fc, vmin(fl, fr));
-        const R alo = tp.a[at(c)], ahi = tp.a[at(c + 1)];
-        const R pp = vmax(R(0), alo) - vmin(R(0), ahi);
-        const R pm = vmax(R(0), ahi) - vmin(R(0), alo);
-        tp.rp[at(c)] = (pp > R(0)) ? vmin(R(1), (fmx - fc) * qf / pp) : R(0);
-        tp.rm[at(c)] = (pm > R(0)) ? vmin(R(1), (fc - fmn) * qf / pm) : R(0);
-    }
-    auto rp = [&](int c) { return (c >= 1 && c <= n) ? tp.rp[at(c)] : R(0); };
-    auto rm = [&](int c) { return (c >= 1 && c <= n) ? tp.rm[at(c)] : R(0); };
-    tp.C[at(1)] = R(0);
-    for (int f = 2; f <= n + 1; ++f)
-        tp.C[at(f)] = (tp.a[at(f)] >= R(0)) ? vmin(rp(f), rm(f - 1)) : vmin(rp(f - 1), rm(f));
-    for (int m = 0; m <= n + 1; ++m) {
-        tp.gftd[at(m)] = 0.0; tp.ga[at(m)] = 0.0; tp.gL[at(m)] = 0.0;
-        tp.grp[at(m)] = 0.0; tp.grm[at(m)] = 0.0; tp.gdv[at(m)] = 0.0;
-    }
-    // ---- stage 4: Fn = clamp01(yv), yv = ftd - ((aC(c+1) - aC(c)) / s4) * vol / dv, aC = a*C
-    for (int c = 1; c <= n; ++c) {
-        const R aC1 = tp.a[at(c + 1)] * tp.C[at(c + 1)], aC0 = tp.a[at(c)] * tp.C[at(c)];
-        const R dv = tp.dv[at(c)];
-        const R num = (aC1 - aC0) / s4;
-        const R yv = tp.ftd[at(c)] - num * vol / dv;
-        double g = aF[at(c)] * dclamp(yv);
-        if (fin) g *= dclamp(clamp01(yv));
-        aF[at(c)] = 0.0;  // the sweep overwrote the interior value
-        if (g == 0.0) continue;
-        const double k = double(vol) / (double(s4) * double(dv));
-        tp.gftd[at(c)] += g;
-        const double gaC1 = -g * k, gaC0 = g * k;
-        tp.ga[at(c + 1)] += gaC1 * double(tp.C[at(c + 1)]);
-        tp.ga[at(c)] += gaC0 * double(tp.C[at(c)]);
-        tp.gdv[at(c)] += g * double(num) * double(vol) / (double(dv) * double(dv));
-        for (int e = 0; e < 2; ++e) {  // limiter of faces c+1 and c
-            const int f = c + 1 - e;
-            const double gC = (e == 0 ? gaC1 : gaC0) * double(tp.a[at(f)]);
-            if (f < 2 || gC == 0.0) continue;
-            R x, y2;
-            int cx, cy;  // C = min(x, y2) with x = rp(cx), y2 = rm(cy)
-            if (tp.a[at(f)] >= R(0)) { x = rp(f); cx = f; y2 = rm(f - 1); cy = f - 1; }
-            else { x = rp(f - 1); cx = f - 1; y2 = rm(f); cy = f; }
-            if (cx >= 1 && cx <= n) tp.grp[at(cx)] += gC * tie_w(x < y2, x == y2);
-            if (cy >= 1 && cy <= n) tp.grm[at(cy)] += gC * tie_w(y2 < x, x == y2);
+    const double ds1 = (DIR == 1) ? dy / dxdy : dy * dx / vol;
+    auto ldF = [&](int m) { return (m >= 0 && m <= n + 1) ? double(F[base + m * S]) : 0.0; };
+    auto ldV = [&](int m) { return (m >= 0 && m <= n + 1) ? double(V[base + m * S]) : 0.0; };
+
+    const int j0 = m0 - 6, j1 = m1 + 6;
+    // forward windows
+    double F0 = ldF(j0 - 1), F1 = 0, F2 = 0, F3 = 0, F4 = 0, F5 = 0, F6 = 0;
+    double V0 = ldV(j0 - 1), V1 = 0, V2 = 0, V3 = 0, V4 = 0, V5 = 0;
+    double L0 = 0, L1 = 0;
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+    double dv1 = 1, dv2 = 1, dv3 = 1, dv4 = 1, dv5 = 1;
+    double Y1 = 0, Y2 = 0, Y3 = 0, Y4 = 0, Y5 = 0;
+    double T1 = 0, T2 = 0, T3 = 0, T4 = 0, T5 = 0;
+    double P2 = 0, P3 = 0, P4 = 0, M2 = 0, M3 = 0, M4 = 0;
+    double C2 = 0, C3 = 0;
+    // adjoint windows
+    double GA2 = 0, GA3 = 0;            // aC on faces
+    double GP3 = 0, GP4 = 0, GM3 = 0, GM4 = 0;  // rp, rm
+    double GT3 = 0, GT4 = 0, GT5 = 0;   // Ftd
+    double Ga3 = 0, Ga4 = 0, Ga5 = 0;   // a
+    double GD3 = 0, GD4 = 0, GD5 = 0;   // dv
+    double GL4 = 0, GL5 = 0;            // L (stage-1 part)
+    double GF5 = 0, GF6 = 0;            // F
+    double GV4 = 0, GV5 = 0;            // V
+    for (int j = j0; j <= j1; ++j) {
+        // ---- advance the windows by one position
+        F6 = F5; F5 = F4; F4 = F3; F3 = F2; F2 = F1; F1 = F0; F0 = ldF(j);
+        V5 = V4; V4 = V3; V3 = V2; V2 = V1; V1 = V0; V0 = ldV(j);
+        L1 = L0;
+        a4 = a3; a3 = a2; a2 = a1; a1 = a0;
+        dv5 = dv4; dv4 = dv3; dv3 = dv2; dv2 = dv1;
+        Y5 = Y4; Y4 = Y3; Y3 = Y2; Y2 = Y1;
+        T5 = T4; T4 = T3; T3 = T2; T2 = T1;
+        P4 = P3; P3 = P2; M4 = M3; M3 = M2;
+        C3 = C2;
+        GA3 = GA2; GA2 = 0;
+        GP4 = GP3; GP3 = 0; GM4 = GM3; GM3 = 0;
+        GT5 = GT4; GT4 = GT3; GT3 = 0;
+        Ga5 = Ga4; Ga4 = Ga3; Ga3 = 0;
+        GD5 = GD4; GD4 = GD3; GD3 = 0;
+        GL5 = GL4; GL4 = 0;
+        GF6 = GF5; GF5 = 0;
+        GV5 = GV4; GV4 = 0;
+        // ---- forward: face j
+        L0 = 0; a0 = 0;
+        if (j >= 1 && j <= n + 1) {
+            const double vdt = V0 * dt;
+            const double Lf = (V0 >= 0.0) ? vdt * F1 : vdt * F0;
+            const double Hf = (V0 <= 0.0) ? vdt * F1 : vdt * F0;
+            L0 = Lf;
+            a0 = Hf - Lf;
         }
-    }
-    // ---- stage 2: rp = min(1, Q+/P+), rm = min(1, Q-/P-)
-    for (int c = 1; c <= n; ++c) {
-        const R fc = ftd(c), fl = ftd(c - 1), fr = ftd(c + 1);
-        const R alo = tp.a[at(c)], ahi = tp.a[at(c + 1)];
-        for (int s = 0; s < 2; ++s) {
-            const double gr = (s == 0) ? tp.grp[at(c)] : tp.grm[at(c)];
-            if (gr == 0.0) continue;
-            const R x = (s == 0) ? alo : ahi, y = (s == 0) ? ahi : alo;  // P = max(0,x) - min(0,y)
-            const R P = vmax(R(0), x) - vmin(R(0), y);
-            if (!(P > R(0))) continue;
-            const R m1 = (s == 0) ? vmax(fl, fr) : vmin(fl, fr);
-            const R ext = (s == 0) ? vmax(fc, m1) : vmin(fc, m1);
-            const R Q = ((s == 0) ? (ext - fc) : (fc - ext)) * qf;
-            const R ratio = Q / P;
-            const double wr = tie_w(ratio < R(1), ratio == R(1));
-            if (wr == 0.0) continue;
-            const double gratio = gr * wr;
-            const double gQ = gratio / double(P);
-            const double gP = -gratio * double(Q) / (double(P) * double(P));
-            const double gext = ((s == 0) ? gQ : -gQ) * double(qf);
-            tp.gftd[at(c)] += ((s == 0) ? -gQ : gQ) * double(qf);
-            const double wc = tie_w((s == 0) ? (fc > m1) : (fc < m1), fc == m1);
-            tp.gftd[at(c)] += gext * wc;
-            const double wl = tie_w((s == 0) ? (fl > fr) : (fl < fr), fl == fr);
-            if (c - 1 >= 1) tp.gftd[at(c - 1)] += gext * (1.0 - wc) * wl;
-            if (c + 1 <= n) tp.gftd[at(c + 1)] += gext * (1.0 - wc) * (1.0 - wl);
-            const double dx = tie_w(x > R(0), x == R(0)), dy = -tie_w(y < R(0), y == R(0));
-            tp.ga[at(s == 0 ? c : c + 1)] += gP * dx;
-            tp.ga[at(s == 0 ? c + 1 : c)] += gP * dy;
+        // cell j-1: stage 1
+        dv1 = 1; Y1 = 0; T1 = 0;
+        if (j - 1 >= 1 && j - 1 <= n) {
+            dv1 = vol - kdv * (V0 - V1);
+            Y1 = (F1 + s1(L1 - L0)) * vol / dv1;
+            T1 = (Y1 > 1.0 || Y1 < 0.0) ? clamp01d(Y1) : Y1;
         }
-    }
-    // ---- a = H - L; stage 1 (with its clip) on cells
-    for (int f = 1; f <= n + 1; ++f) tp.gL[at(f)] -= tp.ga[at(f)];
-    for (int c = 1; c <= n; ++c) {
-        const double gf = tp.gftd[at(c)];
-        if (gf == 0.0) continue;
-        const R o = tp.y[at(c)];
-        const double go = gf * ((o > R(1) || o < R(0)) ? dclamp(o) : 1.0);
-        if (go == 0.0) continue;
-        const double dv = double(tp.dv[at(c)]);
-        const double k = double(vol) / dv;
-        aF[at(c)] += go * k;
-        tp.gL[at(c)] += go * k * ds1;
-        tp.gL[at(c + 1)] -= go * k * ds1;
-        tp.gdv[at(c)] -= go * double(o) / dv;
-    }
-    // ---- fluxes L (donor) and H (downwind): L = vdt * F(up), H = vdt * F(down)
-    for (int f = 1; f <= n + 1; ++f) {
-        const double gL = tp.gL[at(f)], gH = tp.ga[at(f)];
-        const R vf = V[at(f)], flo = F[at(f - 1)], fhi = F[at(f)];
-        const double vdt = double(vf * dt);
-        const bool lo_L = (vf >= R(0)), lo_H = (vf <= R(0));
-        double gv = 0.0;
-        if (gL != 0.0) {
-            aF[at(lo_L ? f - 1 : f)] += gL * vdt;
-            gv += gL * double(dt) * double(lo_L ? flo : fhi);
+        // cell j-2: Zalesak ratios
+        P2 = 0; M2 = 0;
+        if (j - 2 >= 1 && j - 2 <= n) {
+            const double fmx = fmax(T2, fmax(T3, T1)), fmn = fmin(T2, fmin(T3, T1));
+            const double pp = fmax(0.0, a2) - fmin(0.0, a1);
+            const double pm = fmax(0.0, a1) - fmin(0.0, a2);
+            if (pp > 0.0) P2 = fmin(1.0, (fmx - T2) * qf / pp);
+            if (pm > 0.0) M2 = fmin(1.0, (T2 - fmn) * qf / pm);
         }
-        if (gH != 0.0) {
-            aF[at(lo_H ? f - 1 : f)] += gH * vdt;
-            gv += gH * double(dt) * double(lo_H ? flo : fhi);
+        // face j-2: limiter
+        C2 = 0;
+        if (j - 2 >= 2 && j - 2 <= n + 1) C2 = (a2 >= 0.0) ? fmin(P2, M3) : fmin(P3, M2);
+        // ---- reverse, cell j-3: yv -> Ftd, aC(j-2), aC(j-3), dv
+        if (j - 3 >= 1 && j - 3 <= n) {
+            const double num = (a2 * C2 - a3 * C3) / s4;
+            const double yv = T3 - num * vol / dv3;
+            double g = aFo[base + (j - 3) * S] * dclamp(yv);
+            if (fin) g *= dclamp(clamp01d(yv));
+            if (g != 0.0) {
+                const double k = vol / (s4 * dv3);
+                GT3 += g;
+                GA2 -= g * k;
+                GA3 += g * k;
+                GD3 += g * num * vol / (dv3 * dv3);
+            }
         }
-        aV[at(f)] += gv;
+        // face j-3: aC = a C -> a, limiter operands
+        Ga3 += GA3 * C3;
+        if (j - 3 >= 2 && j - 3 <= n + 1 && GA3 != 0.0) {
+            const double gC = GA3 * a3;
+            if (a3 >= 0.0) {  // C = min(rp(f), rm(f-1))
+                GP3 += gC * tie_w(P3 < M4, P3 == M4);
+                GM4 += gC * tie_w(M4 < P3, P3 == M4);
+            } else {          // C = min(rp(f-1), rm(f))
+                GP4 += gC * tie_w(P4 < M3, P4 == M3);
+                GM3 += gC * tie_w(M3 < P4, P4 == M3);
+            }
+        }
+        // cell j-4: ratios -> Ftd(j-5..j-3), a(j-4), a(j-3)
+        if (j - 4 >= 1 && j - 4 <= n) {
+            const double fc = T4, fl = T5, fr = T3, alo = a4, ahi = a3;
+            for (int s = 0; s < 2; ++s) {
+                const double gr = s ? GM4 : GP4;
+                if (gr == 0.0) continue;
+                const double x = s ? ahi : alo, y = s ? alo : ahi;  // P = max(0,x) - min(0,y)
+                const double P = fmax(0.0, x) - fmin(0.0, y);
+                if (!(P > 0.0)) continue;
+                const double m1v = s ? fmin(fl, fr) : fmax(fl, fr);
+                const double ext = s ? fmin(fc, m1v) : fmax(fc, m1v);
+                const double Q = (s ? (fc - ext) : (ext - fc)) * qf;
+                const double ratio = Q / P;
+                const double wr = tie_w(ratio < 1.0, ratio == 1.0);
+                if (wr == 0.0) continue;
+                const double gratio = gr * wr;
+                const double gQ = gratio / P;
+                const double gP = -gratio * Q / (P * P);
+                const double gext = (s ? -gQ : gQ) * qf;
+                GT4 += (s ? gQ : -gQ) * qf;
+                const double wc = tie_w(s ? (fc < m1v) : (fc > m1v), fc == m1v);
+                const double wl = tie_w(s ? (fl < fr) : (fl > fr), fl == fr);
+                GT4 += gext * wc;
+                GT5 += gext * (1.0 - wc) * wl;
+                GT3 += gext * (1.0 - wc) * (1.0 - wl);
+                const double gx = gP * tie_w(x > 0.0, x == 0.0), gy = -gP * tie_w(y < 0.0, y == 0.0);
+                if (s) { Ga3 += gx; Ga4 += gy; } else { Ga4 += gx; Ga3 += gy; }
+            }
+        }
+        // cell j-5: stage 1 (with its clip) -> F, L(j-5), L(j-4), dv; dv -> V
+        if (j - 5 >= 1 && j - 5 <= n) {
+            const double o = Y5;
+            const double go = GT5 * ((o > 1.0 || o < 0.0) ? dclamp(o) : 1.0);
+            if (go != 0.0) {
+                const double k = vol / dv5;
+                GF5 += go * k;
+                GL5 += go * k * ds1;
+                GL4 -= go * k * ds1;
+                GD5 -= go * o / dv5;
+            }
+            GV4 -= GD5 * kdv;
+            GV5 += GD5 * kdv;
+        }
+        // face j-5: L, H -> F(j-6), F(j-5), V(j-5)
+        if (j - 5 >= 1 && j - 5 <= n + 1) {
+            const double gL = GL5 - Ga5, gH = Ga5;
+            const double vdt = V5 * dt;
+            const bool loL = V5 >= 0.0, loH = V5 <= 0.0;
+            if (loL) GF6 += gL * vdt; else GF5 += gL * vdt;
+            if (loH) GF6 += gH * vdt; else GF5 += gH * vdt;
+            GV5 += gL * dt * (loL ? F6 : F5) + gH * dt * (loH ? F6 : F5);
+            const int f = j - 5;
+            if ((f >= m0 && f <= m1) || (f == n + 1 && m1 == n)) aV[base + f * S] += GV5;
+        }
+        // cell j-6: complete
+        if (j - 6 >= m0 && j - 6 <= m1) aFi[base + (j - 6) * S] = GF6;
     }
-    // ---- dv = vol - kdv * (V(c+1) - V(c))
-    for (int c = 1; c <= n; ++c) {
-        const double g = tp.gdv[at(c)];
-        if (g == 0.0) continue;
-        aV[at(c + 1)] -= g * double(kdv);
-        aV[at(c)] += g * double(kdv);
+}
+
+// x sweep reverse with lanes along x (coalesced): overlapped waves over the
+// flattened (x,y) plane, every lane one position (cell p and face p), each
+// stage of the forward and of the reverse formed per lane with the
+// neighbours' values taken by whole-wave lane shifts.  Along any chain from an
+// input to an output there are at most six shifts each way, so a wave stores
+// its middle 64 - 2*6 lanes.  Quantities of invalid positions (outside cells
+// 1..nx, faces 1..nx+1, ghost rows) are zero, which also keeps consecutive rows
+// of the flattened plane apart.  Each wave walks kz planes.
+constexpr int kAdjXHalo = 6;
+
+template <class R>
+__global__ __launch_bounds__(256) void k_fct_adj_x(const Inst<R>* __restrict__ tab, const R* __restrict__ F,
+                                                   const R* __restrict__ V, const double* __restrict__ aFo,
+                                                   double* __restrict__ aFi, double* __restrict__ aV, int kz,
+                                                   int fin) {
+    const Inst<R>& I = tab[0];
+    constexpr int H = kAdjXHalo, S = 64 - 2 * H;
+    const int nx = I.nx, ny = I.ny, sx = I.sx, sy = I.sy;
+    const int P = sx * (ny + 2);
+    const int wpp = (P + S - 1) / S, bpp = (wpp + 3) >> 2;
+    const int zb = int(blockIdx.x) / bpp;
+    const int wv = (int(blockIdx.x) - zb * bpp) * 4 + (int(threadIdx.x) >> 6);
+    if (wv >= wpp) return;  // wave-uniform
+    const int lane = int(threadIdx.x) & 63;
+    const int t = wv * S - H + lane;
+    const int tc = min(max(t, 0), P - 1);
+    const int j = tc / sx, i = tc - j * sx;
+    const bool row = (t == tc) && j >= 1 && j <= ny;
+    const bool own = lane >= H && lane < 64 - H && t == tc;
+    const bool fc = row && i >= 1 && i <= nx + 1;   // face i
+    const bool cc = row && i >= 1 && i <= nx;       // cell i
+    const bool cf2 = row && i >= 2 && i <= nx + 1;  // face i with a limiter
+    const int k0 = 1 + zb * kz, k1 = min(I.nz, k0 + kz - 1);
+    const double dt = I.dt, vol = I.vol, kdv = I.dtdydz, qf = I.dx, s4 = I.dy;
+    const double dy = I.dy, dz = I.dz;
+    const double ds1 = dy * dz / vol;
+    auto up = [](double x) { return lane_up(x); };  // value of lane-1
+    auto dn = [](double x) { return lane_dn(x); };  // value of lane+1
+    for (int k = k0; k <= k1; ++k) {
+        const int o = k * sy + tc;
+        const double Fv = double(F[o]), Vv = double(V[o]);
+        const double aFov = cc ? aFo[o] : 0.0;
+        // ---- forward
+        const double F1 = up(Fv);
+        double L = 0.0, a = 0.0;
+        if (fc) {
+            const double vdt = Vv * dt;
+            const double Lf = (Vv >= 0.0) ? vdt * F1 : vdt * Fv;
+            const double Hf = (Vv <= 0.0) ? vdt * F1 : vdt * Fv;
+            L = Lf;
+            a = Hf - Lf;
+        }
+        const double Vn = dn(Vv), Ln = dn(L);
+        double dv = 1.0, y1 = 0.0, T = 0.0;
+        if (cc) {
+            dv = vol - kdv * (Vn - Vv);
+            y1 = (Fv + (L - Ln) * dy * dz / vol) * vol / dv;
+            T = (y1 > 1.0 || y1 < 0.0) ? clamp01d(y1) : y1;
+        }
+        const double Tl = up(T), Tr = dn(T), an = dn(a);
+        double rp = 0.0, rm = 0.0;
+        if (cc) {
+            const double fmx = fmax(T, fmax(Tl, Tr)), fmn = fmin(T, fmin(Tl, Tr));
+            const double pp = fmax(0.0, a) - fmin(0.0, an);
+            const double pm = fmax(0.0, an) - fmin(0.0, a);
+            if (pp > 0.0) rp = fmin(1.0, (fmx - T) * qf / pp);
+            if (pm > 0.0) rm = fmin(1.0, (T - fmn) * qf / pm);
+        }
+        const double rpl = up(rp), rml = up(rm);
+        double C = 0.0;
+        if (cf2) C = (a >= 0.0) ? fmin(rp, rml) : fmin(rpl, rm);
+        const double aC = a * C, aCn = dn(aC);
+        // ---- reverse: yv (cell) -> Ftd, aC faces, dv
+        double g = 0.0, gk = 0.0, gdv = 0.0;
+        if (cc) {
+            const double num = (aCn - aC) / s4;
+            const double yv = T - num * vol / dv;
+            g = aFov * dclamp(yv);
+            if (fin) g *= dclamp(clamp01d(yv));
+            gk = g * vol / (s4 * dv);
+            gdv = g * num * vol / (dv * dv);
+        }
+        double gT = g;
+        const double gaC = gk - up(gk);  // face i: +gk of cell i, -gk of cell i-1
+        // aC = a C (face) -> a, limiter operands
+        double ga = gaC * C;
+        double rp_own = 0.0, rm_own = 0.0, rp_prev = 0.0, rm_prev = 0.0;
+        if (cf2 && gaC != 0.0) {
+            const double gC = gaC * a;
+            if (a >= 0.0) {
+                rp_own = gC * tie_w(rp < rml, rp == rml);
+                rm_prev = gC * tie_w(rml < rp, rp == rml);
+            } else {
+                rp_prev = gC * tie_w(rpl < rm, rpl == rm);
+                rm_own = gC * tie_w(rm < rpl, rpl == rm);
+            }
+        }
+        const double grp = rp_own + dn(rp_prev), grm = rm_own + dn(rm_prev);
+        // ratios (cell) -> Ftd(i-1..i+1), a(i), a(i+1)
+        double tL = 0.0, tR = 0.0, gaLo = 0.0, gaHi = 0.0;
+        if (cc) {
+            for (int s = 0; s < 2; ++s) {
+                const double gr = s ? grm : grp;
+                if (gr == 0.0) continue;
+                const double x = s ? an : a, y = s ? a : an;  // P = max(0,x) - min(0,y)
+                const double Pq = fmax(0.0, x) - fmin(0.0, y);
+                if (!(Pq > 0.0)) continue;
+                const double m1v = s ? fmin(Tl, Tr) : fmax(Tl, Tr);
+                const double ext = s ? fmin(T, m1v) : fmax(T, m1v);
+                const double Q = (s ? (T - ext) : (ext - T)) * qf;
+                const double ratio = Q / Pq;
+                const double wr = tie_w(ratio < 1.0, ratio == 1.0);
+                if (wr == 0.0) continue;
+                const double gratio = gr * wr;
+                const double gQ = gratio / Pq;
+                const double gP = -gratio * Q / (Pq * Pq);
+                const double gext = (s ? -gQ : gQ) * qf;
+                gT += (s ? gQ : -gQ) * qf;
+                const double wc = tie_w(s ? (T < m1v) : (T > m1v), T == m1v);
+                const double wl = tie_w(s ? (Tl < Tr) : (Tl > Tr), Tl == Tr);
+                gT += gext * wc;
+                tL += gext * (1.0 - wc) * wl;
+                tR += gext * (1.0 - wc) * (1.0 - wl);
+                const double gx = gP * tie_w(x > 0.0, x == 0.0), gy = -gP * tie_w(y < 0.0, y == 0.0);
+                if (s) { gaHi += gx; gaLo += gy; } else { gaLo += gx; gaHi += gy; }
+            }
+        }
+        gT += dn(tL) + up(tR);
+        ga += gaLo + up(gaHi);
+        // stage 1 (cell) -> F, L(i), L(i+1), dv; dv -> V(i), V(i+1)
+        double gF = 0.0, gLc = 0.0, gLn = 0.0, gVo = 0.0, gVn = 0.0;
+        if (cc) {
+            const double go = gT * ((y1 > 1.0 || y1 < 0.0) ? dclamp(y1) : 1.0);
+            const double kk = vol / dv;
+            gF = go * kk;
+            gLc = go * kk * ds1;
+            gLn = -go * kk * ds1;
+            gdv -= go * y1 / dv;
+            gVo = gdv * kdv;
+            gVn = -gdv * kdv;
+        }
+        // fluxes (face) -> F(i-1), F(i), V(i)
+        const double gL = gLc + up(gLn) - ga, gH = ga;
+        double gV = gVo + up(gVn), toPrev = 0.0;
+        if (fc) {
+            const double vdt = Vv * dt;
+            const bool loL = Vv >= 0.0, loH = Vv <= 0.0;
+            if (loL) toPrev += gL * vdt; else gF += gL * vdt;
+            if (loH) toPrev += gH * vdt; else gF += gH * vdt;
+            gV += gL * dt * (loL ? F1 : Fv) + gH * dt * (loH ? F1 : Fv);
+        }
+        gF += dn(toPrev);
+        if (own && cc) aFi[o] = gF;
+        if (own && fc) aV[o] += gV;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Transpose of the velocity correction and its boundary closed form: stored
-// u(i,j,k) = u_in(i, cj, ck) for 2 <= i <= nx, 0 on the walls i = 1, nx+1, and
-// u_old(0, cj, ck) on plane 0 (likewise v, w).  u_in = u* - dt/r (p0 - p1) / h
-// with r = (rho0 + rho1)/2 feeds u*, p and rho; plane 0 feeds u_old.
+// Transpose of the velocity correction and its boundary closed form.  The
+// stored u(i,j,k) is u_in(i, cj, ck) for 2 <= i <= nx, zero on the walls
+// i = 1, nx+1 and u_old(0, cj, ck) on plane 0 (cj, ck: index clamped to the
+// interior; likewise v, w).  Per interior cell c, gathered (no atomics):
+//   G(face) = sum of the adjoints of the stored copies of that face,
+//   au*(c) = G(c) (faces i >= 2), ap(c) += the pressure-gradient terms of
+//   faces c and c+1, arho(c) = the density terms of faces c and c+1, and on
+//   the first interior plane of each axis the plane-0 copies -> au0.
 template <class R>
 __global__ __launch_bounds__(256) void k_correct_adj(const Inst<R>* __restrict__ tab, const R* __restrict__ F,
                                                      const R* __restrict__ p, const double* __restrict__ au,
@@ -320,60 +541,121 @@ __global__ __launch_bounds__(256) void k_correct_adj(const Inst<R>* __restrict__
                                                      double* __restrict__ arho, double* __restrict__ au0,
                                                      double* __restrict__ av0, double* __restrict__ aw0) {
     const Inst<R>& I = tab[0];
-    Cell q;
-    if (!cell_of(I, long(blockIdx.x) * 256 + threadIdx.x, true, q)) return;
+    int i, j, k, c;
+    if (!interior_cell(I, i, j, k, c)) return;
     const int nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
-    const int ci = min(max(q.i, 1), nx), cj = min(max(q.j, 1), ny), ck = min(max(q.k, 1), nz);
-    auto corr = [&](double g, int c, int off, R hi, double* ast) {
-        if (g == 0.0) return;
-        aadd(&ast[c], g);
-        const double r = 0.5 * (double(rho_of(I, F[c])) + double(rho_of(I, F[c - off])));
-        const double dp = double(p[c]) - double(p[c - off]);
-        const double kk = double(I.dt) / r * double(hi);
-        aadd(&ap[c], -g * kk);
-        aadd(&ap[c - off], g * kk);
-        const double gr = g * double(I.dt) * dp * double(hi) / (r * r);
-        aadd(&arho[c], 0.5 * gr);
-        aadd(&arho[c - off], 0.5 * gr);
+    // index sets of the stored copies along each axis
+    int I3[3], J3[3], K3[3], ni = 0, nj = 0, nk = 0;
+    I3[ni++] = i; if (i == 1) I3[ni++] = 0; if (i == nx) I3[ni++] = nx + 1;
+    J3[nj++] = j; if (j == 1) J3[nj++] = 0; if (j == ny) J3[nj++] = ny + 1;
+    K3[nk++] = k; if (k == 1) K3[nk++] = 0; if (k == nz) K3[nk++] = nz + 1;
+    auto Gu = [&](int ii) {  // u faces at x index ii, copies over (j, k)
+        double s = 0.0;
+        for (int b = 0; b < nj; ++b)
+            for (int d = 0; d < nk; ++d) s += au[K3[d] * sy + J3[b] * sx + ii];
+        return s;
     };
-    const double gu = au[q.c], gv = av[q.c], gw = aw[q.c];
-    if (q.i >= 2 && q.i <= nx) corr(gu, ck * sy + cj * sx + q.i, 1, I.dxi, aus);
-    else if (q.i == 0) aadd(&au0[ck * sy + cj * sx], gu);
-    if (q.j >= 2 && q.j <= ny) corr(gv, ck * sy + q.j * sx + ci, sx, I.dyi, avs);
-    else if (q.j == 0) aadd(&av0[ck * sy + ci], gv);
-    if (q.k >= 2 && q.k <= nz) corr(gw, q.k * sy + cj * sx + ci, sy, I.dzi, aws);
-    else if (q.k == 0) aadd(&aw0[cj * sx + ci], gw);
+    auto Gv = [&](int jj) {
+        double s = 0.0;
+        for (int a = 0; a < ni; ++a)
+            for (int d = 0; d < nk; ++d) s += av[K3[d] * sy + jj * sx + I3[a]];
+        return s;
+    };
+    auto Gw = [&](int kk) {
+        double s = 0.0;
+        for (int a = 0; a < ni; ++a)
+            for (int b = 0; b < nj; ++b) s += aw[kk * sy + J3[b] * sx + I3[a]];
+        return s;
+    };
+    const double dt = I.dt;
+    const double rc = rho_d(I, F[c]), pc = double(p[c]);
+    double gp = 0.0, gr = 0.0;
+    // face between cell c-off (m) and c (hi side); returns its G
+    auto face = [&](double G, int m, bool c_is_hi, double h) {
+        const double rm = rho_d(I, F[m]), pm = double(p[m]);
+        const double rr = 1.0 / (c_is_hi ? 0.5 * (rc + rm) : 0.5 * (rm + rc));
+        const double dp = c_is_hi ? (pc - pm) : (pm - pc);
+        const double kk = dt * rr * h;  // d u / d p, and d u / d r = kk dp / r
+        gp += c_is_hi ? -G * kk : G * kk;
+        gr += 0.5 * G * kk * dp * rr;
+    };
+    const double hx = I.dxi, hy = I.dyi, hz = I.dzi;
+    if (i >= 2) { const double G = Gu(i); aus[c] = G; face(G, c - 1, true, hx); }
+    if (i + 1 <= nx) face(Gu(i + 1), c + 1, false, hx);
+    if (j >= 2) { const double G = Gv(j); avs[c] = G; face(G, c - sx, true, hy); }
+    if (j + 1 <= ny) face(Gv(j + 1), c + sx, false, hy);
+    if (k >= 2) { const double G = Gw(k); aws[c] = G; face(G, c - sy, true, hz); }
+    if (k + 1 <= nz) face(Gw(k + 1), c + sy, false, hz);
+    ap[c] += gp;
+    arho[c] = gr;
+    if (i == 1) au0[k * sy + j * sx] = Gu(0);
+    if (j == 1) av0[k * sy + i] = Gv(0);
+    if (k == 1) aw0[j * sx + i] = Gw(0);
 }
 
-// Transpose of one Jacobi sweep p' = (rhs - N p) / ap on interior cells:
-// gout(c) = -sum over the interior neighbours m that read c of coef(m->c) gin(m) / ap(m),
-// arhs += gin / ap.  Ghost entries of gout come out zero (zero coefficients).
+// Transposed Jacobi sweeps.  A forward sweep is p' = D^-1 (rhs - N p) on
+// interior cells, with N the off-diagonal part (coefficient a_d towards every
+// interior neighbour, zero towards a ghost) and D = diag(ap), ap = -sum of the
+// coefficients (summed in the forward's order).  N is symmetric, so with
+// h = D^-1 gbar the reverse of one sweep is
+//   rhs_bar += h,   gbar_prev = -N h,   h_prev = D^-1 gbar_prev,
+// a zero-rhs Jacobi sweep on h.  A launch does one or two such sweeps; in_g:
+// the input is gbar (divided by ap on load), out_g: write gbar_prev instead of
+// h_prev (last launch), first: assign rhs_bar instead of adding.  Only interior
+// entries are read or written.
 template <class R>
-__global__ __launch_bounds__(256) void k_jacobi_adj(const Inst<R>* __restrict__ tab, const double* __restrict__ gin,
-                                                    double* __restrict__ gout, double* __restrict__ arhs) {
-    const Inst<R>& I = tab[0];
-    Cell q;
-    if (!cell_of(I, long(blockIdx.x) * 256 + threadIdx.x, true, q)) return;
-    const int nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
-    auto apc = [&](int i, int j, int k) {  // diagonal, summed in the forward's order
-        const R ae = (i != nx) ? I.axx : R(0), aw = (i != 1) ? I.axx : R(0);
-        const R an = (j != ny) ? I.ayy : R(0), as = (j != 1) ? I.ayy : R(0);
-        const R af = (k != nz) ? I.azz : R(0), ab = (k != 1) ? I.azz : R(0);
-        return double(-(ae + aw + an + as + af + ab));
-    };
-    const int i = q.i, j = q.j, k = q.k, c = q.c;
-    double s = 0.0;
-    if (interior(I, i + 1, j, k) && i + 1 != 1) s -= double(I.axx) * gin[c + 1] / apc(i + 1, j, k);
-    if (interior(I, i - 1, j, k) && i - 1 != nx) s -= double(I.axx) * gin[c - 1] / apc(i - 1, j, k);
-    if (interior(I, i, j + 1, k) && j + 1 != 1) s -= double(I.ayy) * gin[c + sx] / apc(i, j + 1, k);
-    if (interior(I, i, j - 1, k) && j - 1 != ny) s -= double(I.ayy) * gin[c - sx] / apc(i, j - 1, k);
-    if (interior(I, i, j, k + 1) && k + 1 != 1) s -= double(I.azz) * gin[c + sy] / apc(i, j, k + 1);
-    if (interior(I, i, j, k - 1) && k - 1 != nz) s -= double(I.azz) * gin[c - sy] / apc(i, j, k - 1);
-    gout[c] = s;
-    if (interior(I, i, j, k)) arhs[c] += gin[c] / apc(i, j, k);
+__device__ __forceinline__ double jac_ap(const Inst<R>& I, int i, int j, int k) {
+    const R ae = (i != I.nx) ? I.axx : R(0), aw = (i != 1) ? I.axx : R(0);
+    const R an = (j != I.ny) ? I.ayy : R(0), as = (j != 1) ? I.ayy : R(0);
+    const R af = (k != I.nz) ? I.azz : R(0), ab = (k != 1) ? I.azz : R(0);
+    return double(-(ae + aw + an + as + af + ab));
 }
 
-// Transpose of rhs = rho/dt * div(u*): each face gathers from its two cells.
+template <class R, int NS>
+__global__ __launch_bounds__(256) void k_jacobi_adj(const Inst<R>* __restrict__ tab, const double* __restrict__ in,
+                                                    double* __restrict__ out, double* __restrict__ arhs, int in_g,
+                                                    int out_g, int first) {
+    const Inst<R>& I = tab[0];
+    int i, j, k, c;
+    if (!interior_cell(I, i, j, k, c)) return;
+    const int nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
+    const double ax = I.axx, ay = I.ayy, az = I.azz;
+    auto H = [&](int ii, int jj, int kk) {  // input as h at an interior cell
+        const double x = in[kk * sy + jj * sx + ii];
+        return in_g ? x / jac_ap(I, ii, jj, kk) : x;
+    };
+    auto nh = [&](int ii, int jj, int kk) {  // -N h at an interior cell
+        double s = 0.0;
+        if (ii < nx) s -= ax * H(ii + 1, jj, kk);
+        if (ii > 1) s -= ax * H(ii - 1, jj, kk);
+        if (jj < ny) s -= ay * H(ii, jj + 1, kk);
+        if (jj > 1) s -= ay * H(ii, jj - 1, kk);
+        if (kk < nz) s -= az * H(ii, jj, kk + 1);
+        if (kk > 1) s -= az * H(ii, jj, kk - 1);
+        return s;
+    };
+    double acc = H(i, j, k);
+    double g;
+    if (NS == 1) {
+        g = nh(i, j, k);
+    } else {
+        // first sweep at the cell and its six neighbours, second at the cell
+        auto h1 = [&](int ii, int jj, int kk) { return nh(ii, jj, kk) / jac_ap(I, ii, jj, kk); };
+        acc += h1(i, j, k);
+        g = 0.0;
+        if (i < nx) g -= ax * h1(i + 1, j, k);
+        if (i > 1) g -= ax * h1(i - 1, j, k);
+        if (j < ny) g -= ay * h1(i, j + 1, k);
+        if (j > 1) g -= ay * h1(i, j - 1, k);
+        if (k < nz) g -= az * h1(i, j, k + 1);
+        if (k > 1) g -= az * h1(i, j, k - 1);
+    }
+    out[c] = out_g ? g : g / jac_ap(I, i, j, k);
+    arhs[c] = first ? acc : arhs[c] + acc;
+}
+
+// Transpose of rhs = rho/dt * div(u*) on interior cells: each interior face
+// i >= 2 (j >= 2, k >= 2) gathers from its two cells; arho += arhs div / dt.
 template <class R>
 __global__ __launch_bounds__(256) void k_rhs_adj(const Inst<R>* __restrict__ tab, const R* __restrict__ F,
                                                  const R* __restrict__ us, const R* __restrict__ vs,
@@ -381,28 +663,18 @@ __global__ __launch_bounds__(256) void k_rhs_adj(const Inst<R>* __restrict__ tab
                                                  double* __restrict__ aus, double* __restrict__ avs,
                                                  double* __restrict__ aws, double* __restrict__ arho) {
     const Inst<R>& I = tab[0];
-    Cell q;
-    if (!cell_of(I, long(blockIdx.x) * 256 + threadIdx.x, true, q)) return;
-    const int sx = I.sx, sy = I.sy, c = q.c;
+    int i, j, k, c;
+    if (!interior_cell(I, i, j, k, c)) return;
+    const int sx = I.sx, sy = I.sy;
     const double rdt = 1.0 / double(I.dt);
-    auto rf = [&](int cc) { return double(rho_of(I, F[cc])) * rdt; };
-    double gx = 0.0, gy = 0.0, gz = 0.0;
-    if (interior(I, q.i, q.j, q.k)) {
-        const double g = arhs[c] * rf(c);
-        gx -= g * double(I.dxi);
-        gy -= g * double(I.dyi);
-        gz -= g * double(I.dzi);
-        const double div = (double(us[c + 1]) - double(us[c])) * double(I.dxi) +
-                           (double(vs[c + sx]) - double(vs[c])) * double(I.dyi) +
-                           (double(ws[c + sy]) - double(ws[c])) * double(I.dzi);
-        arho[c] += arhs[c] * div * rdt;
-    }
-    if (interior(I, q.i - 1, q.j, q.k)) gx += arhs[c - 1] * rf(c - 1) * double(I.dxi);
-    if (interior(I, q.i, q.j - 1, q.k)) gy += arhs[c - sx] * rf(c - sx) * double(I.dyi);
-    if (interior(I, q.i, q.j, q.k - 1)) gz += arhs[c - sy] * rf(c - sy) * double(I.dzi);
-    aus[c] += gx;
-    avs[c] += gy;
-    aws[c] += gz;
+    const double hx = I.dxi, hy = I.dyi, hz = I.dzi;
+    const double gc = arhs[c] * rho_d(I, F[c]) * rdt;
+    if (i >= 2) aus[c] += (arhs[c - 1] * rho_d(I, F[c - 1]) * rdt - gc) * hx;
+    if (j >= 2) avs[c] += (arhs[c - sx] * rho_d(I, F[c - sx]) * rdt - gc) * hy;
+    if (k >= 2) aws[c] += (arhs[c - sy] * rho_d(I, F[c - sy]) * rdt - gc) * hz;
+    const double div = (double(us[c + 1]) - double(us[c])) * hx + (double(vs[c + sx]) - double(vs[c])) * hy +
+                       (double(ws[c + sy]) - double(ws[c])) * hz;
+    arho[c] += arhs[c] * div * rdt;
 }
 
 // Partial derivatives of one predictor face value
@@ -414,205 +686,279 @@ struct MomAdj {
     double c = 0, xm = 0, xp = 0, ym = 0, yp = 0, zm = 0, zp = 0, ax = 0, ay = 0, az = 0;
 };
 
-template <class R>
-__device__ __forceinline__ void mom_adj(const Inst<R>& I, double g, R c, R xm, R xp, R ym, R yp, R zm, R zp, R ax,
-                                        R ay, R az, R n0, int own, MomAdj& d, double& dnu) {
-    const double hx = double(I.dxi), hy = double(I.dyi), hz = double(I.dzi);
-    const double gd = g * double(I.dt), n = double(n0);
-    dnu += gd * ((double(xm) - 2.0 * double(c) + double(xp)) * hx * hx +
-                 (double(ym) - 2.0 * double(c) + double(yp)) * hy * hy +
-                 (double(zm) - 2.0 * double(c) + double(zp)) * hz * hz);
+__device__ __forceinline__ void mom_adj(double hx, double hy, double hz, double dt, double g, double c, double xm,
+                                        double xp, double ym, double yp, double zm, double zp, double ax, double ay,
+                                        double az, double n, int own, MomAdj& d, double& dnu) {
+    const double gd = g * dt;
+    dnu += gd * ((xm - 2.0 * c + xp) * hx * hx + (ym - 2.0 * c + yp) * hy * hy + (zm - 2.0 * c + zp) * hz * hz);
     d.c += g - 2.0 * gd * n * (hx * hx + hy * hy + hz * hz);
     d.xm += gd * n * hx * hx; d.xp += gd * n * hx * hx;
     d.ym += gd * n * hy * hy; d.yp += gd * n * hy * hy;
     d.zm += gd * n * hz * hz; d.zp += gd * n * hz * hz;
-    const double axd = double(ax), ayd = double(ay), azd = double(az);
     double ddx, ddy, ddz;
-    if (ax > R(0)) { ddx = (double(c) - double(xm)) * hx; d.c -= gd * axd * hx; d.xm += gd * axd * hx; }
-    else { ddx = (double(xp) - double(c)) * hx; d.c += gd * axd * hx; d.xp -= gd * axd * hx; }
-    if (ay > R(0)) { ddy = (double(c) - double(ym)) * hy; d.c -= gd * ayd * hy; d.ym += gd * ayd * hy; }
-    else { ddy = (double(yp) - double(c)) * hy; d.c += gd * ayd * hy; d.yp -= gd * ayd * hy; }
-    if (az > R(0)) { ddz = (double(c) - double(zm)) * hz; d.c -= gd * azd * hz; d.zm += gd * azd * hz; }
-    else { ddz = (double(zp) - double(c)) * hz; d.c += gd * azd * hz; d.zp -= gd * azd * hz; }
+    if (ax > 0.0) { ddx = (c - xm) * hx; d.c -= gd * ax * hx; d.xm += gd * ax * hx; }
+    else { ddx = (xp - c) * hx; d.c += gd * ax * hx; d.xp -= gd * ax * hx; }
+    if (ay > 0.0) { ddy = (c - ym) * hy; d.c -= gd * ay * hy; d.ym += gd * ay * hy; }
+    else { ddy = (yp - c) * hy; d.c += gd * ay * hy; d.yp -= gd * ay * hy; }
+    if (az > 0.0) { ddz = (c - zm) * hz; d.c -= gd * az * hz; d.zm += gd * az * hz; }
+    else { ddz = (zp - c) * hz; d.c += gd * az * hz; d.zp -= gd * az * hz; }
     if (own == 0) d.c -= gd * ddx; else d.ax -= gd * ddx;
     if (own == 1) d.c -= gd * ddy; else d.ay -= gd * ddy;
     if (own == 2) d.c -= gd * ddz; else d.az -= gd * ddz;
 }
 
-// Transpose of the momentum predictor: the lane of interior cell c scatters the
-// adjoints of u* (face i >= 2), v* (face j >= 2) and w* (face k >= 2) into the
-// velocity adjoints (atomics) and nu(c); it also sums dJ/dg = dt * sum a(u*).
+// Transpose of the momentum predictor, fused with that of rho/nu.
+//
+// A work-group owns a column of PTX x PTY interior cells and walks a chunk of
+// kz planes.  Per plane every lane forms the partial derivatives of the three
+// faces of its cell (u* on face i >= 2, v* on j >= 2, w* on k >= 2) and adds
+// them, times the face adjoints, into an LDS window of (PTX+2) x (PTY+2) cells
+// x 3 planes x 3 velocity components.  Once a plane can receive nothing more
+// from this work-group it is written out: a cell whose whole stencil lies in
+// the column and the chunk gets a plain store, the others (shared with a
+// neighbouring column or chunk) a global atomic add onto arrays the host
+// zeroed.  The lane then adds to aF(c) the F adjoint of rho(c) and nu(c) (nu(c)
+// enters only the three faces of cell c).  Work-group sums for
+// dJ/d(rho_l, rho_g, nu_l, nu_g, gx, gy, gz).
+constexpr int PTX = 32, PTY = 8;
+constexpr int PWX = PTX + 2, PWY = PTY + 2, PWC = PWX * PWY;
+
+__device__ __forceinline__ int pslot(int p) { return p % 3; }
+
 template <class R>
 __global__ __launch_bounds__(256) void k_predict_adj(const Inst<R>* __restrict__ tab, const R* __restrict__ F,
                                                      const R* __restrict__ u, const R* __restrict__ v,
                                                      const R* __restrict__ w, const double* __restrict__ aus,
                                                      const double* __restrict__ avs, const double* __restrict__ aws,
-                                                     double* __restrict__ au, double* __restrict__ av,
-                                                     double* __restrict__ aw, double* __restrict__ anu,
-                                                     double* __restrict__ part) {
+                                                     const double* __restrict__ arho, double* __restrict__ au,
+                                                     double* __restrict__ av, double* __restrict__ aw,
+                                                     double* __restrict__ aF, double* __restrict__ part, int kz) {
     const Inst<R>& I = tab[0];
-    Cell q;
-    double gsum[3] = {0.0, 0.0, 0.0};
-    if (cell_of(I, long(blockIdx.x) * 256 + threadIdx.x, false, q)) {
-        const int sx = I.sx, sy = I.sy, c = q.c;
-        const R n0 = nu_of(I, F[c]);
-        double gn = 0.0;
-        auto scatter7 = [&](double* a, const MomAdj& d) {
-            aadd(&a[c], d.c); aadd(&a[c - 1], d.xm); aadd(&a[c + 1], d.xp); aadd(&a[c - sx], d.ym);
-            aadd(&a[c + sx], d.yp); aadd(&a[c - sy], d.zm); aadd(&a[c + sy], d.zp);
-        };
-        auto scatter4 = [&](double* a, double g, int o0, int o1, int o2, int o3) {
-            const double h = 0.25 * g;
-            aadd(&a[c + o0], h); aadd(&a[c + o1], h); aadd(&a[c + o2], h); aadd(&a[c + o3], h);
-        };
-        if (q.i >= 2 && aus[c] != 0.0) {  // u*: advecting (u, vh, wh)
-            const R vh = R(0.25) * (v[c - 1] + v[c - 1 + sx] + v[c] + v[c + sx]);
-            const R wh = R(0.25) * (w[c - 1] + w[c - 1 + sy] + w[c] + w[c + sy]);
-            MomAdj d;
-            mom_adj(I, aus[c], u[c], u[c - 1], u[c + 1], u[c - sx], u[c + sx], u[c - sy], u[c + sy], u[c], vh, wh, n0,
-                    0, d, gn);
-            scatter7(au, d);
-            scatter4(av, d.ay, -1, -1 + sx, 0, sx);
-            scatter4(aw, d.az, -1, -1 + sy, 0, sy);
-            gsum[0] = aus[c] * double(I.dt);
+    __shared__ double win[3][3][PWC];  // [plane slot][component][window cell]
+    const int nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
+    const int nbx = (nx + PTX - 1) / PTX, nby = (ny + PTY - 1) / PTY;
+    const int b = int(blockIdx.x);
+    const int bz = b / (nbx * nby), bxy = b - bz * nbx * nby;
+    const int by = bxy / nbx, bx = bxy - by * nbx;
+    const int x0 = 1 + bx * PTX, y0 = 1 + by * PTY, k0 = 1 + bz * kz, k1 = min(nz, k0 + kz - 1);
+    const int tid = int(threadIdx.x), tx = tid % PTX, ty = tid / PTX;
+    const int i = x0 + tx, j = y0 + ty;
+    const bool valid = i <= nx && j <= ny;
+    for (int q = tid; q < 9 * PWC; q += 256) (&win[0][0][0])[q] = 0.0;
+    __syncthreads();
+    const double hx = I.dxi, hy = I.dyi, hz = I.dzi, dt = I.dt;
+    double sums[7] = {0, 0, 0, 0, 0, 0, 0};
+    double* const outp[3] = {au, av, aw};
+    // write plane p of the window (inner: cells with a complete stencil may be stored)
+    auto flush = [&](int p, bool inner_plane) {
+        if (p < 0 || p > nz + 1) return;
+        const int s = pslot(p);
+        for (int q = tid; q < 3 * PWC; q += 256) {
+            const int comp = q / PWC, wc = q - comp * PWC;
+            const int ly = wc / PWX, lx = wc - ly * PWX;
+            const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+            if (gx > nx + 1 || gy > ny + 1) continue;
+            const double val = win[s][comp][wc];
+            double* dst = outp[comp] + (size_t(p) * sy + size_t(gy) * sx + gx);
+            const bool inner = inner_plane && lx >= 2 && lx <= PTX - 1 && ly >= 2 && ly <= PTY - 1;
+            if (inner) *dst = val;
+            else if (val != 0.0) atomicAdd(dst, val);
         }
-        if (q.j >= 2 && avs[c] != 0.0) {  // v*: advecting (uh, v, wh)
-            const R uh = R(0.25) * (u[c - sx] + u[c] + u[c + 1 - sx] + u[c + 1]);
-            const R wh = R(0.25) * (w[c - sx + sy] + w[c - sx] + w[c] + w[c + sy]);
-            MomAdj d;
-            mom_adj(I, avs[c], v[c], v[c - 1], v[c + 1], v[c - sx], v[c + sx], v[c - sy], v[c + sy], uh, v[c], wh, n0,
-                    1, d, gn);
-            scatter7(av, d);
-            scatter4(au, d.ax, -sx, 0, 1 - sx, 1);
-            scatter4(aw, d.az, -sx + sy, -sx, 0, sy);
-            gsum[1] = avs[c] * double(I.dt);
-        }
-        if (q.k >= 2 && aws[c] != 0.0) {  // w*: advecting (uh, vh, w)
-            const R uh = R(0.25) * (u[c + 1 - sy] + u[c - sy] + u[c + 1] + u[c]);
-            const R vh = R(0.25) * (v[c + sx - sy] + v[c - sy] + v[c] + v[c + sx]);
-            MomAdj d;
-            mom_adj(I, aws[c], w[c], w[c - 1], w[c + 1], w[c - sx], w[c + sx], w[c - sy], w[c + sy], uh, vh, w[c], n0,
-                    2, d, gn);
-            scatter7(aw, d);
-            scatter4(au, d.ax, 1 - sy, -sy, 1, 0);
-            scatter4(av, d.ay, sx - sy, -sy, 0, sx);
-            gsum[2] = aws[c] * double(I.dt);
-        }
-        anu[c] += gn;
-    }
-    block_parts<3>(gsum, part);
-}
-
-// rho = rho_g (1 - f) + rho_l f, nu = nu_l f + nu_g (1 - f), f = clamp01(F):
-// aF += (arho (rho_l - rho_g) + anu (nu_l - nu_g)) dclamp(F); work-group sums of
-// arho f, arho (1-f), anu f, anu (1-f) for the parameter gradients.
-template <class R>
-__global__ __launch_bounds__(256) void k_rhonu_adj(const Inst<R>* __restrict__ tab, const R* __restrict__ F,
-                                                   const double* __restrict__ arho, const double* __restrict__ anu,
-                                                   double* __restrict__ aF, double* __restrict__ part) {
-    const Inst<R>& I = tab[0];
-    Cell q;
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
-    if (cell_of(I, long(blockIdx.x) * 256 + threadIdx.x, true, q)) {
-        const R f = clamp01(F[q.c]);
-        const double gr = arho[q.c], gn = anu[q.c];
-        if (gr != 0.0 || gn != 0.0) {
+    };
+    for (int k = k0; k <= k1; ++k) {
+        if (valid) {
+            const int c = k * sy + j * sx + i;
+            const double n0 = nu_d(I, F[c]);
+            double gn = 0.0;
+            auto U = [&](int q) { return double(u[q]); };
+            auto Vv = [&](int q) { return double(v[q]); };
+            auto W = [&](int q) { return double(w[q]); };
+            // add g to component comp of the cell at offset (di, dj, dk)
+            auto put = [&](int comp, int di, int dj, int dk, double g) {
+                if (g != 0.0) atomicAdd(&win[pslot(k + dk)][comp][(ty + 1 + dj) * PWX + (tx + 1 + di)], g);
+            };
+            auto put7 = [&](int comp, const MomAdj& d) {
+                put(comp, 0, 0, 0, d.c); put(comp, -1, 0, 0, d.xm); put(comp, 1, 0, 0, d.xp);
+                put(comp, 0, -1, 0, d.ym); put(comp, 0, 1, 0, d.yp); put(comp, 0, 0, -1, d.zm);
+                put(comp, 0, 0, 1, d.zp);
+            };
+            if (i >= 2 && aus[c] != 0.0) {  // u*: advecting (u, vh, wh)
+                const double vh = 0.25 * (Vv(c - 1) + Vv(c - 1 + sx) + Vv(c) + Vv(c + sx));
+                const double wh = 0.25 * (W(c - 1) + W(c - 1 + sy) + W(c) + W(c + sy));
+                MomAdj d;
+                mom_adj(hx, hy, hz, dt, aus[c], U(c), U(c - 1), U(c + 1), U(c - sx), U(c + sx), U(c - sy),
+                        U(c + sy), U(c), vh, wh, n0, 0, d, gn);
+                put7(0, d);
+                const double qy = 0.25 * d.ay, qz = 0.25 * d.az;  // v at (i-1|i, j|j+1), w at (i-1|i, k|k+1)
+                put(1, -1, 0, 0, qy); put(1, -1, 1, 0, qy); put(1, 0, 0, 0, qy); put(1, 0, 1, 0, qy);
+                put(2, -1, 0, 0, qz); put(2, -1, 0, 1, qz); put(2, 0, 0, 0, qz); put(2, 0, 0, 1, qz);
+                sums[4] += aus[c] * dt;
+            }
+            if (j >= 2 && avs[c] != 0.0) {  // v*: advecting (uh, v, wh)
+                const double uh = 0.25 * (U(c - sx) + U(c) + U(c + 1 - sx) + U(c + 1));
+                const double wh = 0.25 * (W(c - sx + sy) + W(c - sx) + W(c) + W(c + sy));
+                MomAdj d;
+                mom_adj(hx, hy, hz, dt, avs[c], Vv(c), Vv(c - 1), Vv(c + 1), Vv(c - sx), Vv(c + sx), Vv(c - sy),
+                        Vv(c + sy), uh, Vv(c), wh, n0, 1, d, gn);
+                put7(1, d);
+                const double qx = 0.25 * d.ax, qz = 0.25 * d.az;  // u at (i|i+1, j-1|j), w at (j-1|j, k|k+1)
+                put(0, 0, -1, 0, qx); put(0, 0, 0, 0, qx); put(0, 1, -1, 0, qx); put(0, 1, 0, 0, qx);
+                put(2, 0, -1, 1, qz); put(2, 0, -1, 0, qz); put(2, 0, 0, 0, qz); put(2, 0, 0, 1, qz);
+                sums[5] += avs[c] * dt;
+            }
+            if (k >= 2 && aws[c] != 0.0) {  // w*: advecting (uh, vh, w)
+                const double uh = 0.25 * (U(c + 1 - sy) + U(c - sy) + U(c + 1) + U(c));
+                const double vh = 0.25 * (Vv(c + sx - sy) + Vv(c - sy) + Vv(c) + Vv(c + sx));
+                MomAdj d;
+                mom_adj(hx, hy, hz, dt, aws[c], W(c), W(c - 1), W(c + 1), W(c - sx), W(c + sx), W(c - sy),
+                        W(c + sy), uh, vh, W(c), n0, 2, d, gn);
+                put7(2, d);
+                const double qx = 0.25 * d.ax, qy = 0.25 * d.ay;  // u at (i|i+1, k-1|k), v at (j|j+1, k-1|k)
+                put(0, 1, 0, -1, qx); put(0, 0, 0, -1, qx); put(0, 1, 0, 0, qx); put(0, 0, 0, 0, qx);
+                put(1, 0, 1, -1, qy); put(1, 0, 0, -1, qy); put(1, 0, 0, 0, qy); put(1, 0, 1, 0, qy);
+                sums[6] += aws[c] * dt;
+            }
+            // rho(c), nu(c) <- f = clamp01(F(c))
+            const double gr = arho[c];
+            const double f = clamp01d(double(F[c]));
             const double gf = gr * (double(I.rho_l) - double(I.rho_g)) + gn * (double(I.nu_l) - double(I.nu_g));
-            aF[q.c] += gf * dclamp(F[q.c]);
-            s[0] = gr * double(f);
-            s[1] = gr * double(R(1) - f);
-            s[2] = gn * double(f);
-            s[3] = gn * double(R(1) - f);
+            aF[c] += gf * dclamp(double(F[c]));
+            sums[0] += gr * f;
+            sums[1] += gr * (1.0 - f);
+            sums[2] += gn * f;
+            sums[3] += gn * (1.0 - f);
         }
+        __syncthreads();
+        // plane k-1 is complete for this work-group (faces of planes k-2..k done)
+        flush(k - 1, k - 1 >= k0 + 1);
+        __syncthreads();
+        const int s = pslot(k - 1);  // reused for plane k+2
+        for (int q = tid; q < 3 * PWC; q += 256) win[s][q / PWC][q % PWC] = 0.0;
+        __syncthreads();
     }
-    block_parts<4>(s, part);
+    flush(k1, false);
+    flush(k1 + 1, false);
+    block_parts<7>(sums, part);
 }
 
-__global__ __launch_bounds__(256) void k_axpy(const double* __restrict__ src, double* __restrict__ dst, long n) {
+__global__ __launch_bounds__(256) void k_zero3(double* __restrict__ a, double* __restrict__ b,
+                                               double* __restrict__ c, long n) {
     const long t = long(blockIdx.x) * 256 + threadIdx.x;
-    if (t < n) dst[t] += src[t];
+    if (t < n) { a[t] = 0.0; b[t] = 0.0; c[t] = 0.0; }
+}
+
+// Up to five whole-field copies in one launch.
+template <class R>
+__global__ __launch_bounds__(256) void k_copy_fields(FieldCopy<R> fc, long n) {
+    const long t = long(blockIdx.x) * 256 + threadIdx.x;
+    if (t >= n) return;
+    for (int q = 0; q < fc.nf; ++q) fc.dst[q][t] = fc.src[q][t];
 }
 
 // ---------------------------------------------------------------------------
 // Host launchers.
+namespace {
+template <class R> dim3 grid_interior(const Inst<R>& h) { return dim3(adj_blocks(long(h.nx) * h.ny), h.nz); }
+template <class R> dim3 grid_stored(const Inst<R>& h) { return dim3(adj_blocks(long(h.sy)), h.nz + 2); }
+}  // namespace
+
 template <class R>
-void adj_centroid_parts(const Inst<R>* tab, const Inst<R>& h, const R* F, double* part, hipStream_t st) {
-    hipLaunchKernelGGL(k_centroid_parts<R>, dim3(adj_blocks(long(h.nx) * h.ny * h.nz)), dim3(256), 0, st, tab, F,
-                       part);
+int adj_centroid_parts(const Inst<R>* tab, const Inst<R>& h, const R* F, double* part, hipStream_t st) {
+    const dim3 g = grid_interior(h);
+    hipLaunchKernelGGL(k_centroid_parts<R>, g, dim3(256), 0, st, tab, F, part);
+    return int(g.x * g.y);
 }
 template <class R>
 void adj_centroid(const Inst<R>* tab, const Inst<R>& h, double J, double sumF, double gJ, const double* seed,
                   double* aF, hipStream_t st) {
-    hipLaunchKernelGGL(k_centroid_adj<R>, dim3(adj_blocks(h.ntot)), dim3(256), 0, st, tab, J, sumF, gJ, seed, aF);
+    hipLaunchKernelGGL(k_centroid_adj<R>, grid_stored(h), dim3(256), 0, st, tab, J, sumF, gJ, seed, aF);
 }
 template <class R>
-void adj_ghostF(const Inst<R>* tab, const Inst<R>& h, double* aF, hipStream_t st) {
-    hipLaunchKernelGGL(k_ghostF_adj<R>, dim3(adj_blocks(h.ntot)), dim3(256), 0, st, tab, aF);
-}
-template <class R>
-void adj_fct(const Inst<R>* tab, const Inst<R>& h, int dir, const R* F, const R* V, double* aF, double* aV,
-             const FctTape<R>& tp, int fin, hipStream_t st) {
-    const long lines = (dir == 0) ? long(h.ny) * h.nz : (dir == 1) ? long(h.nx) * h.nz : long(h.nx) * h.ny;
-    const dim3 grid(adj_blocks(lines));
-    if (dir == 0) hipLaunchKernelGGL((k_fct_adj<R, 0>), grid, dim3(256), 0, st, tab, F, V, aF, aV, tp, fin);
-    else if (dir == 1) hipLaunchKernelGGL((k_fct_adj<R, 1>), grid, dim3(256), 0, st, tab, F, V, aF, aV, tp, fin);
-    else hipLaunchKernelGGL((k_fct_adj<R, 2>), grid, dim3(256), 0, st, tab, F, V, aF, aV, tp, fin);
+void adj_fct(const Inst<R>* tab, const Inst<R>& h, int dir, const R* F, const R* V, const double* aFo,
+             double* aFi, double* aV, int fin, hipStream_t st) {
+    if (dir == 0) {  // lanes along x; z chunk of 8 planes, fewer while under 1024 work-groups
+        constexpr int S = 64 - 2 * kAdjXHalo;
+        const int P = h.sx * (h.ny + 2);
+        const int bpp = ((P + S - 1) / S + 3) / 4;
+        int kz = 8;
+        while (kz > 1 && bpp * ((h.nz + kz - 1) / kz) < 1024) kz /= 2;
+        hipLaunchKernelGGL(k_fct_adj_x<R>, dim3(bpp * ((h.nz + kz - 1) / kz)), dim3(256), 0, st, tab, F, V, aFo, aFi,
+                           aV, kz, fin);
+        return;
+    }
+    // y and z: marches with lanes along x; chunk length 16 cells, 8 when that
+    // leaves too few work-groups to fill the chip
+    const int n = (dir == 1) ? h.ny : h.nz;
+    const long lines = (dir == 1) ? long(h.nx) * h.nz : long(h.nx) * h.ny;
+    const int nbt = adj_blocks(lines);
+    int ch = 16;
+    if (long(nbt) * ((n + ch - 1) / ch) < 1024) ch = 8;
+    const dim3 grid(nbt * ((n + ch - 1) / ch));
+    if (dir == 1) hipLaunchKernelGGL((k_fct_adj<R, 1>), grid, dim3(256), 0, st, tab, F, V, aFo, aFi, aV, ch, fin);
+    else hipLaunchKernelGGL((k_fct_adj<R, 2>), grid, dim3(256), 0, st, tab, F, V, aFo, aFi, aV, ch, fin);
 }
 template <class R>
 void adj_correct(const Inst<R>* tab, const Inst<R>& h, const R* F, const R* p, const double* au, const double* av,
                  const double* aw, double* aus, double* avs, double* aws, double* ap, double* arho, double* au0,
                  double* av0, double* aw0, hipStream_t st) {
-    hipLaunchKernelGGL(k_correct_adj<R>, dim3(adj_blocks(h.ntot)), dim3(256), 0, st, tab, F, p, au, av, aw, aus, avs,
-                       aws, ap, arho, au0, av0, aw0);
+    hipLaunchKernelGGL(k_correct_adj<R>, grid_interior(h), dim3(256), 0, st, tab, F, p, au, av, aw, aus, avs, aws,
+                       ap, arho, au0, av0, aw0);
 }
 template <class R>
-void adj_jacobi(const Inst<R>* tab, const Inst<R>& h, const double* gin, double* gout, double* arhs, hipStream_t st) {
-    hipLaunchKernelGGL(k_jacobi_adj<R>, dim3(adj_blocks(h.ntot)), dim3(256), 0, st, tab, gin, gout, arhs);
+void adj_jacobi(const Inst<R>* tab, const Inst<R>& h, const double* in, double* out, double* arhs, int ns, int in_g,
+                int out_g, int first, hipStream_t st) {
+    if (ns == 2)
+        hipLaunchKernelGGL((k_jacobi_adj<R, 2>), grid_interior(h), dim3(256), 0, st, tab, in, out, arhs, in_g, out_g,
+                           first);
+    else
+        hipLaunchKernelGGL((k_jacobi_adj<R, 1>), grid_interior(h), dim3(256), 0, st, tab, in, out, arhs, in_g, out_g,
+                           first);
 }
 template <class R>
 void adj_rhs(const Inst<R>* tab, const Inst<R>& h, const R* F, const R* us, const R* vs, const R* ws,
              const double* arhs, double* aus, double* avs, double* aws, double* arho, hipStream_t st) {
-    hipLaunchKernelGGL(k_rhs_adj<R>, dim3(adj_blocks(h.ntot)), dim3(256), 0, st, tab, F, us, vs, ws, arhs, aus, avs,
-                       aws, arho);
+    hipLaunchKernelGGL(k_rhs_adj<R>, grid_interior(h), dim3(256), 0, st, tab, F, us, vs, ws, arhs, aus, avs, aws,
+                       arho);
 }
 template <class R>
 void adj_predict(const Inst<R>* tab, const Inst<R>& h, const R* F, const R* u, const R* v, const R* w,
-                 const double* aus, const double* avs, const double* aws, double* au, double* av, double* aw,
-                 double* anu, double* part, double* acc, hipStream_t st) {
-    const int nb = adj_blocks(long(h.nx) * h.ny * h.nz);
-    hipLaunchKernelGGL(k_predict_adj<R>, dim3(nb), dim3(256), 0, st, tab, F, u, v, w, aus, avs, aws, au, av, aw, anu,
-                       part);
-    hipLaunchKernelGGL(k_sum_parts<3>, dim3(1), dim3(256), 0, st, part, nb, acc + ADJ_GX);
+                 const double* aus, const double* avs, const double* aws, const double* arho, double* au, double* av,
+                 double* aw, double* aF, double* part, double* acc, hipStream_t st) {
+    // z chunk: 16 planes, fewer while that leaves under 1024 work-groups
+    const int nxy = ((h.nx + PTX - 1) / PTX) * ((h.ny + PTY - 1) / PTY);
+    int kz = 16;
+    while (kz > 2 && nxy * ((h.nz + kz - 1) / kz) < 1024) kz /= 2;
+    const int nb = nxy * ((h.nz + kz - 1) / kz);
+    hipLaunchKernelGGL(k_predict_adj<R>, dim3(nb), dim3(256), 0, st, tab, F, u, v, w, aus, avs, aws, arho, au, av,
+                       aw, aF, part, kz);
+    hipLaunchKernelGGL(k_sum_parts<7>, dim3(1), dim3(256), 0, st, part, nb, acc);
+}
+void adj_zero3(double* a, double* b, double* c, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero3, dim3(adj_blocks(n)), dim3(256), 0, st, a, b, c, n);
 }
 template <class R>
-void adj_rhonu(const Inst<R>* tab, const Inst<R>& h, const R* F, const double* arho, const double* anu, double* aF,
-               double* part, double* acc, hipStream_t st) {
-    const int nb = adj_blocks(h.ntot);
-    hipLaunchKernelGGL(k_rhonu_adj<R>, dim3(nb), dim3(256), 0, st, tab, F, arho, anu, aF, part);
-    hipLaunchKernelGGL(k_sum_parts<4>, dim3(1), dim3(256), 0, st, part, nb, acc + ADJ_RHO_L);
+void copy_fields(const FieldCopy<R>& fc, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_copy_fields<R>, dim3(adj_blocks(n)), dim3(256), 0, st, fc, n);
 }
-void adj_axpy(const double* src, double* dst, long n, hipStream_t st) {
-    hipLaunchKernelGGL(k_axpy, dim3(adj_blocks(n)), dim3(256), 0, st, src, dst, n);
-}
+template void copy_fields<float>(const FieldCopy<float>&, long, hipStream_t);
+template void copy_fields<double>(const FieldCopy<double>&, long, hipStream_t);
 
 #define VOF_ADJ_INSTANTIATE(R)                                                                                     \
-    template void adj_centroid_parts<R>(const Inst<R>*, const Inst<R>&, const R*, double*, hipStream_t);          \
+    template int adj_centroid_parts<R>(const Inst<R>*, const Inst<R>&, const R*, double*, hipStream_t);           \
     template void adj_centroid<R>(const Inst<R>*, const Inst<R>&, double, double, double, const double*, double*, \
                                   hipStream_t);                                                                    \
-    template void adj_ghostF<R>(const Inst<R>*, const Inst<R>&, double*, hipStream_t);                            \
-    template void adj_fct<R>(const Inst<R>*, const Inst<R>&, int, const R*, const R*, double*, double*,           \
-                             const FctTape<R>&, int, hipStream_t);                                                 \
+    template void adj_fct<R>(const Inst<R>*, const Inst<R>&, int, const R*, const R*, const double*, double*,     \
+                             double*, int, hipStream_t);                                                           \
     template void adj_correct<R>(const Inst<R>*, const Inst<R>&, const R*, const R*, const double*, const double*, \
                                  const double*, double*, double*, double*, double*, double*, double*, double*,     \
                                  double*, hipStream_t);                                                            \
-    template void adj_jacobi<R>(const Inst<R>*, const Inst<R>&, const double*, double*, double*, hipStream_t);    \
+    template void adj_jacobi<R>(const Inst<R>*, const Inst<R>&, const double*, double*, double*, int, int, int,   \
+                                int, hipStream_t);                                                                 \
     template void adj_rhs<R>(const Inst<R>*, const Inst<R>&, const R*, const R*, const R*, const R*,              \
                              const double*, double*, double*, double*, double*, hipStream_t);                      \
     template void adj_predict<R>(const Inst<R>*, const Inst<R>&, const R*, const R*, const R*, const R*,          \
-                                 const double*, const double*, const double*, double*, double*, double*, double*,  \
-                                 double*, double*, hipStream_t);                                                   \
-    template void adj_rhonu<R>(const Inst<R>*, const Inst<R>&, const R*, const double*, const double*, double*,    \
-                               double*, double*, hipStream_t);
+                                 const double*, const double*, const double*, const double*, double*, double*,     \
+                                 double*, double*, double*, double*, hipStream_t);
 
 VOF_ADJ_INSTANTIATE(float)
 VOF_ADJ_INSTANTIATE(double)

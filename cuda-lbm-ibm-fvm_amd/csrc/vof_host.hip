@@ -206,23 +206,20 @@ struct Ensemble : EnsBase {
         int f_fin = 0, p_fin = 0;
         double J = 0.0, sumF = 0.0;
         double* d = nullptr;  // FP64 arrays, ntot each, see adj_alloc
-        size_t d_arrays = 0;
-        double *aF = nullptr, *au[2] = {}, *av[2] = {}, *aw[2] = {}, *ap[2] = {}, *gt = nullptr;
-        double *aus = nullptr, *avs = nullptr, *aws = nullptr, *arhs = nullptr, *arho = nullptr, *anu = nullptr;
+        double *aF[2] = {}, *au[2] = {}, *av[2] = {}, *aw[2] = {}, *ap[2] = {}, *gt = nullptr;
+        double *aus = nullptr, *avs = nullptr, *aws = nullptr, *arhs = nullptr, *arho = nullptr;
         double* seed[5] = {};  // F, u, v, w, p seeds of the final state (nullptr: zero)
         double* seed_mem = nullptr;
-        R* tapeR = nullptr;
-        vof::FctTape<R> tp{};
         double* part = nullptr;
         double* acc = nullptr;
-        int cur = 0;
+        int cur = 0, fcur = 0;  // which of the pairs holds the live adjoint state
         bool fwd_done = false, bwd_done = false;
         double fwd_ms = 0.0, bwd_ms = 0.0;
     } adj;
 
     void adj_release() override {
         Adj& a = adj;
-        for (void* p : {(void*)a.store, (void*)a.d, (void*)a.seed_mem, (void*)a.tapeR, (void*)a.part, (void*)a.acc})
+        for (void* p : {(void*)a.store, (void*)a.d, (void*)a.seed_mem, (void*)a.part, (void*)a.acc})
             if (p) (void)hipFree(p);
         a = Adj();
     }
@@ -774,26 +771,54 @@ struct Ensemble : EnsBase {
     }
 
     // ---------------------------------------------------------------- reverse mode
+    // Store layout, in units of 5 fields: checkpoints [0, nck), the K+1 states
+    // of the interval being reversed [nck, nck+K+1), the final state nck+K+1,
+    // and the K step tapes (u*, v*, w*, F after the first and the second sweep)
+    // [nck+K+2, nck+2K+2).
     R* adj_state(size_t idx) const { return adj.store + idx * 5 * adj.nt; }
+    size_t adj_final() const { return size_t(adj.nck) + size_t(adj.K) + 1; }
+    R* adj_tape(size_t m) const { return adj_state(adj_final() + 1 + m); }
 
     vof_status save_state(R* dst) {
-        const size_t b = adj.nt * sizeof(R);
         const auto& I = ih[0];
-        const R* src[5] = {I.F[fsel], I.u, I.v, I.w, I.p[psel]};
-        for (int q = 0; q < 5; ++q)
-            VOF_HIP(hipMemcpyAsync(dst + q * adj.nt, src[q], b, hipMemcpyDeviceToDevice, st));
+        const size_t nt = adj.nt;
+        vof::FieldCopy<R> fc{{I.F[fsel], I.u, I.v, I.w, I.p[psel]}, {dst, dst + nt, dst + 2 * nt, dst + 3 * nt, dst + 4 * nt}, 5};
+        vof::copy_fields(fc, long(nt), st);
         return VOF_OK;
     }
     // Load a stored state into F[fs], u, v, w, p[ps] and select those buffers.
     vof_status load_state(const R* s, int fs, int ps) {
-        const size_t b = adj.nt * sizeof(R);
         const auto& I = ih[0];
-        R* dst[5] = {I.F[fs], I.u, I.v, I.w, I.p[ps]};
-        for (int q = 0; q < 5; ++q)
-            VOF_HIP(hipMemcpyAsync(dst[q], s + q * adj.nt, b, hipMemcpyDeviceToDevice, st));
+        const size_t nt = adj.nt;
+        vof::FieldCopy<R> fc{{s, s + nt, s + 2 * nt, s + 3 * nt, s + 4 * nt}, {I.F[fs], I.u, I.v, I.w, I.p[ps]}, 5};
+        vof::copy_fields(fc, long(nt), st);
         fsel = fs;
         psel = ps;
         return VOF_OK;
+    }
+
+    // One step (as enqueue_step) that also tapes u*, v*, w* and F after the
+    // first and the second FCT sweep into tape (5 fields).
+    void enqueue_step_taped(int64_t istep, R* tape) {
+        const auto& I = ih[0];
+        const size_t nt = adj.nt;
+        vof::launch_predict(e, g_pred, fsel, st);
+        vof::launch_rhs(e, g_rhs, fsel, st);
+        enqueue_jacobi();
+        vof::launch_correct(e, g_corr, fsel, psel, st);
+        static const int order[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
+        const int* od = order[int(istep % 3)];
+        vof::Geo* gd[3] = {&g_fx, &g_fy, &g_fz};
+        for (int d = 0; d < 3; ++d) {
+            if (d == 2) {  // F[fsel^1] = after the first sweep, F[fsel] = after the second
+                vof::FieldCopy<R> fc{{I.F[fsel ^ 1], I.F[fsel]}, {tape + 3 * nt, tape + 4 * nt}, 2};
+                vof::copy_fields(fc, long(nt), st);
+            }
+            vof::launch_fct(e, *gd[od[d]], od[d], fsel, d == 2, st);
+            fsel ^= 1;
+        }
+        vof::FieldCopy<R> fc{{I.us, I.vs, I.ws}, {tape, tape + nt, tape + 2 * nt}, 3};
+        vof::copy_fields(fc, long(nt), st);
     }
 
     vof_status adj_forward(int64_t first, int n, int every, double* J) override {
@@ -806,7 +831,7 @@ struct Ensemble : EnsBase {
         const int K = std::min(every, n);
         const int nck = (n + K - 1) / K;
         const size_t nt = size_t(ih[0].ntot);
-        const size_t states = size_t(nck) + size_t(K) + 1;
+        const size_t states = size_t(nck) + 2 * size_t(K) + 2;
         if (adj.store && (adj.nt != nt || adj.store_states < states)) {
             (void)hipFree(adj.store);
             adj.store = nullptr;
@@ -822,7 +847,7 @@ struct Ensemble : EnsBase {
             adj.store_states = states;
         }
         if (!adj.part) {
-            const size_t np = 4 * size_t(vof::adj_blocks(long(nt)));
+            const size_t np = vof::adj_part_size(ih[0].nx, ih[0].ny, ih[0].nz);
             VOF_HIP(hipMalloc(&adj.part, np * sizeof(double)));
             VOF_HIP(hipMalloc(&adj.acc, vof::ADJ_NSCALAR * sizeof(double)));
         }
@@ -840,14 +865,13 @@ struct Ensemble : EnsBase {
             if (s % K == 0) VOF_TRY(save_state(adj_state(size_t(s / K))));
             VOF_TRY(step(first + s, 1, true));
         }
-        VOF_TRY(save_state(adj_state(size_t(nck) + size_t(K))));
+        VOF_TRY(save_state(adj_state(adj_final())));
         adj.f_fin = fsel;
         adj.p_fin = psel;
         const auto& I = ih[0];
-        vof::adj_centroid_parts(e.tab, I, I.F[fsel], adj.part, st);
+        const int nb = vof::adj_centroid_parts(e.tab, I, I.F[fsel], adj.part, st);
         VOF_HIP(hipGetLastError());
         VOF_HIP(hipEventRecord(e1, st));
-        const int nb = vof::adj_blocks(long(I.nx) * I.ny * I.nz);
         std::vector<double> h(2 * size_t(nb));
         VOF_HIP(hipMemcpyAsync(h.data(), adj.part, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
         VOF_HIP(hipStreamSynchronize(st));
@@ -891,76 +915,65 @@ struct Ensemble : EnsBase {
         return VOF_OK;
     }
 
+    // FP64 adjoint arrays, one field each, allocated zeroed once: entries a
+    // kernel never writes (the F ghosts, the unused u* wall faces) stay zero.
     vof_status adj_alloc() {
         const size_t nt = adj.nt;
-        constexpr size_t kArrays = 16 + 6;
+        constexpr size_t kArrays = 16;
         if (!adj.d) {
             VOF_HIP(hipMalloc(&adj.d, kArrays * nt * sizeof(double)));
-            VOF_HIP(hipMalloc(&adj.tapeR, 8 * nt * sizeof(R)));
-            adj.d_arrays = kArrays;
+            VOF_HIP(hipMemsetAsync(adj.d, 0, kArrays * nt * sizeof(double), st));
         }
         double* x = adj.d;
         auto next = [&]() { double* r = x; x += nt; return r; };
-        adj.aF = next();
-        for (int s = 0; s < 2; ++s) { adj.au[s] = next(); adj.av[s] = next(); adj.aw[s] = next(); adj.ap[s] = next(); }
+        for (int s = 0; s < 2; ++s) {
+            adj.aF[s] = next(); adj.au[s] = next(); adj.av[s] = next(); adj.aw[s] = next(); adj.ap[s] = next();
+        }
         adj.gt = next(); adj.aus = next(); adj.avs = next(); adj.aws = next();
-        adj.arhs = next(); adj.arho = next(); adj.anu = next();
-        adj.tp.gftd = next(); adj.tp.ga = next(); adj.tp.gL = next();
-        adj.tp.grp = next(); adj.tp.grm = next(); adj.tp.gdv = next();
-        R* y = adj.tapeR;
-        R** rs[8] = {&adj.tp.ftd, &adj.tp.a, &adj.tp.L, &adj.tp.rp, &adj.tp.rm, &adj.tp.C, &adj.tp.dv, &adj.tp.y};
-        for (R** r : rs) { *r = y; y += nt; }
+        adj.arhs = next(); adj.arho = next();
         return VOF_OK;
     }
 
-    vof_status zero(double* p) {
-        VOF_HIP(hipMemsetAsync(p, 0, adj.nt * sizeof(double), st));
-        return VOF_OK;
-    }
-
-    // Reverse of step `istep` whose input state is S (stored): recompute the
-    // step's intermediates with the forward kernels, then apply the stage
-    // adjoints in reverse order.  On entry adj.{aF, au/av/aw/ap[cur]} hold the
-    // adjoint of the step's output; on exit those of its input (cur flipped).
-    vof_status reverse_step(int64_t istep, const R* S) {
+    // Reverse of step `istep`: S is its input state, S1 its output state and
+    // T its tape (u*, v*, w*, F after the first and the second sweep), all
+    // recorded by the forward kernels.  Applies the stage adjoints in reverse
+    // order.  On entry aF[fcur] and au/av/aw/ap[cur] hold the adjoint of the
+    // step's output; on exit those of its input.
+    vof_status reverse_step(int64_t istep, const R* S, const R* S1, const R* T) {
         auto& I = ih[0];
         const size_t nt = adj.nt;
         const R *SF = S, *Su = S + nt, *Sv = S + 2 * nt, *Sw = S + 3 * nt;
-        VOF_TRY(load_state(S, 0, 0));
-        vof::launch_predict(e, g_pred, 0, st);
-        vof::launch_rhs(e, g_rhs, 0, st);
-        enqueue_jacobi();
-        vof::launch_correct(e, g_corr, 0, psel, st);
+        const R *us = T, *vs = T + nt, *ws = T + 2 * nt, *p1 = S1 + 4 * nt;
         static const int order[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
         const int* od = order[int(istep % 3)];
-        vof::Geo* gd[3] = {&g_fx, &g_fy, &g_fz};
-        vof::launch_fct(e, *gd[od[0]], od[0], 0, 0, st);  // F' -> F[1]
-        vof::launch_fct(e, *gd[od[1]], od[1], 1, 0, st);  // F'' -> F[0]
-        const R* Fin[3] = {SF, I.F[1], I.F[0]};
-        const R* vel[3] = {I.u, I.v, I.w};
+        const R* Fin[3] = {SF, T + 3 * nt, T + 4 * nt};
+        const R* vel[3] = {S1 + nt, S1 + 2 * nt, S1 + 3 * nt};
         const int c = adj.cur, x = c ^ 1;
         double* avel[3] = {adj.au[c], adj.av[c], adj.aw[c]};
-        vof::adj_ghostF(e.tab, I, adj.aF, st);
-        for (int d = 2; d >= 0; --d)
-            vof::adj_fct(e.tab, I, od[d], Fin[d], vel[od[d]], adj.aF, avel[od[d]], adj.tp, d == 2, st);
-        for (double* z : {adj.aus, adj.avs, adj.aws, adj.arhs, adj.arho, adj.anu, adj.au[x], adj.av[x], adj.aw[x]})
-            VOF_TRY(zero(z));
-        vof::adj_correct(e.tab, I, SF, I.p[psel], adj.au[c], adj.av[c], adj.aw[c], adj.aus, adj.avs, adj.aws,
+        int f = adj.fcur;
+        for (int d = 2; d >= 0; --d, f ^= 1)
+            vof::adj_fct(e.tab, I, od[d], Fin[d], vel[od[d]], adj.aF[f], adj.aF[f ^ 1], avel[od[d]], d == 2, st);
+        adj.fcur = f;
+        vof::adj_zero3(adj.au[x], adj.av[x], adj.aw[x], long(nt), st);
+        vof::adj_correct(e.tab, I, SF, p1, adj.au[c], adj.av[c], adj.aw[c], adj.aus, adj.avs, adj.aws,
                          adj.ap[c], adj.arho, adj.au[x], adj.av[x], adj.aw[x], st);
         if (sweeps == 0) {
             VOF_HIP(hipMemcpyAsync(adj.ap[x], adj.ap[c], nt * sizeof(double), hipMemcpyDeviceToDevice, st));
         } else {
-            const double* gin = adj.ap[c];
-            for (int s = 0; s < sweeps; ++s) {
-                double* out = ((sweeps - 1 - s) % 2 == 0) ? adj.ap[x] : adj.gt;
-                vof::adj_jacobi(e.tab, I, gin, out, adj.arhs, st);
-                gin = out;
+            // one sweep per launch (two per launch recompute the intermediate
+            // at six neighbours, which costs more than it saves); the outputs
+            // alternate so that the last lands in ap[x]
+            const int nl = sweeps;
+            const double* in = adj.ap[c];
+            for (int l = 0; l < nl; ++l) {
+                double* out = ((nl - 1 - l) % 2 == 0) ? adj.ap[x] : adj.gt;
+                vof::adj_jacobi(e.tab, I, in, out, adj.arhs, 1, l == 0, l == nl - 1, l == 0, st);
+                in = out;
             }
+            vof::adj_rhs(e.tab, I, SF, us, vs, ws, adj.arhs, adj.aus, adj.avs, adj.aws, adj.arho, st);
         }
-        vof::adj_rhs(e.tab, I, SF, I.us, I.vs, I.ws, adj.arhs, adj.aus, adj.avs, adj.aws, adj.arho, st);
-        vof::adj_predict(e.tab, I, SF, Su, Sv, Sw, adj.aus, adj.avs, adj.aws, adj.au[x], adj.av[x], adj.aw[x],
-                         adj.anu, adj.part, adj.acc, st);
-        vof::adj_rhonu(e.tab, I, SF, adj.arho, adj.anu, adj.aF, adj.part, adj.acc, st);
+        vof::adj_predict(e.tab, I, SF, Su, Sv, Sw, adj.aus, adj.avs, adj.aws, adj.arho, adj.au[x], adj.av[x],
+                         adj.aw[x], adj.aF[f], adj.part, adj.acc, st);
         adj.cur = x;
         VOF_HIP(hipGetLastError());
         return VOF_OK;
@@ -977,26 +990,30 @@ struct Ensemble : EnsBase {
         VOF_HIP(hipEventRecord(e0, st));
         VOF_HIP(hipMemsetAsync(adj.acc, 0, vof::ADJ_NSCALAR * sizeof(double), st));
         adj.cur = 0;
-        vof::adj_centroid(e.tab, I, adj.J, adj.sumF, gJ, adj.seed[0], adj.aF, st);
+        adj.fcur = 0;
+        vof::adj_centroid(e.tab, I, adj.J, adj.sumF, gJ, adj.seed[0], adj.aF[0], st);
         double* init[4] = {adj.au[0], adj.av[0], adj.aw[0], adj.ap[0]};
         for (int q = 0; q < 4; ++q) {
             if (adj.seed[q + 1])
                 VOF_HIP(hipMemcpyAsync(init[q], adj.seed[q + 1], nt * sizeof(double), hipMemcpyDeviceToDevice, st));
             else
-                VOF_TRY(zero(init[q]));
+                VOF_HIP(hipMemsetAsync(init[q], 0, nt * sizeof(double), st));
         }
-        const R* seg = adj_state(size_t(adj.nck));
+        // per interval, last first: one taped forward pass from its checkpoint
+        // (states and tapes of every step), then its steps in reverse
         for (int k = adj.nck - 1; k >= 0; --k) {
             const int s0 = k * adj.K, len = std::min(adj.n, s0 + adj.K) - s0;
             VOF_TRY(load_state(adj_state(size_t(k)), 0, 0));
             for (int m = 0; m < len; ++m) {
-                VOF_TRY(save_state(const_cast<R*>(seg) + size_t(m) * 5 * nt));
-                if (m + 1 < len) VOF_TRY(replay(adj.first + s0 + m));
+                VOF_TRY(save_state(adj_state(size_t(adj.nck) + size_t(m))));
+                enqueue_step_taped(adj.first + s0 + m, adj_tape(size_t(m)));
             }
+            VOF_TRY(save_state(adj_state(size_t(adj.nck) + size_t(len))));
             for (int m = len - 1; m >= 0; --m)
-                VOF_TRY(reverse_step(adj.first + s0 + m, seg + size_t(m) * 5 * nt));
+                VOF_TRY(reverse_step(adj.first + s0 + m, adj_state(size_t(adj.nck) + size_t(m)),
+                                     adj_state(size_t(adj.nck) + size_t(m) + 1), adj_tape(size_t(m))));
         }
-        VOF_TRY(load_state(adj_state(size_t(adj.nck) + size_t(adj.K)), adj.f_fin, adj.p_fin));
+        VOF_TRY(load_state(adj_state(adj_final()), adj.f_fin, adj.p_fin));
         VOF_HIP(hipEventRecord(e1, st));
         double acc[vof::ADJ_NSCALAR];
         VOF_HIP(hipMemcpyAsync(acc, adj.acc, sizeof(acc), hipMemcpyDeviceToHost, st));
@@ -1025,9 +1042,20 @@ struct Ensemble : EnsBase {
         const int q = field_slot(f);
         if (q < 0 || !host || count != adj.nt) { set_err("adj_download: field must be F/U/V/W/P with %zu values", adj.nt); return VOF_ERR_INVALID; }
         const int c = adj.cur;
-        const double* src[5] = {adj.aF, adj.au[c], adj.av[c], adj.aw[c], adj.ap[c]};
+        const double* src[5] = {adj.aF[adj.fcur], adj.au[c], adj.av[c], adj.aw[c], adj.ap[c]};
         VOF_HIP(hipMemcpyAsync(host, src[q], count * sizeof(double), hipMemcpyDeviceToHost, st));
         VOF_HIP(hipStreamSynchronize(st));
+        if (f == VOF_FIELD_P) {
+            // ghost pressures are multiplied by zero coefficients and never
+            // read otherwise: they carry no sensitivity
+            const auto& I = ih[0];
+            double* h = host;
+            for (int k = 0; k < I.nz + 2; ++k)
+                for (int j = 0; j < I.ny + 2; ++j)
+                    for (int i = 0; i < I.nx + 2; ++i)
+                        if (i == 0 || j == 0 || k == 0 || i == I.nx + 1 || j == I.ny + 1 || k == I.nz + 1)
+                            h[size_t(k) * I.sy + size_t(j) * I.sx + i] = 0.0;
+        }
         return VOF_OK;
     }
 
