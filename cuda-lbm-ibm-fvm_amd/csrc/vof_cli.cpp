@@ -3,8 +3,8 @@
 //
 // Same flags, same stdout lines and exit codes, same output files
 // (output/step-NNNNN.vtk, byte-identical writer).  Extension flags select the
-// grid, precision, gravity, IC box and step mode; without them the run is the
-// reference's default case.  Deliberate deviation: --nstep 0, which is an
+// grid, precision, gravity, IC box, step mode and surface tension (--csf,
+// --sigma); without them the run is the reference's default case.  Deliberate deviation: --nstep 0, which is an
 // integer modulo by zero in the reference (:1403), is rejected with an error.
 #include <sys/stat.h>
 
@@ -89,6 +89,10 @@ int main(int argc, char** argv) {
             box[2] = std::atof(argv[++a]);
         } else if (arg == "--remove-p-mean") {
             prm.flags |= VOF_FLAG_REMOVE_P_MEAN;
+        } else if (arg == "--csf") {  // surface tension with the parameter block's sigma
+            prm.flags |= VOF_FLAG_CSF;
+        } else if (arg == "--sigma" && a + 1 < argc) {
+            prm.sigma = std::atof(argv[++a]);
         } else if (arg == "--graph") {
             use_graph = true;
         } else if (arg == "--diag") {

@@ -35,6 +35,8 @@ struct Inst {
     R* ws;    // predicted velocities u*, v*, w*
     R* p[2];  // pressure, ping-pong across Jacobi launches
     R* rhs;   // rho/dt * div(u*), formed once per step
+    R* fsm;   // smoothed F for the interface normals (CSF extension only; null otherwise)
+    R* kappa; // interface curvature (CSF extension only; null otherwise)
 };
 
 // Pick one of a ping-pong pair with constant indices only, so that a copied
@@ -140,6 +142,87 @@ template <class R>
 __device__ __forceinline__ R nu_of(const Inst<R>& I, R F) {  // :101
     const R f = clamp01(F);
     return I.nu_l * f + I.nu_g * (R(1) - f);
+}
+
+// ---------------------------------------------------------------------------
+// CSF surface tension (extension; voftrial1.cu stores sigma, :35, :1279, and
+// never uses it).  Continuum surface force a = sigma kappa grad F / rho:
+//  * the interface normals come from Fs, F smoothed once by the separable
+//    (1/4, 1/2, 1/4) filter on interior cells; Fs outside the interior is the
+//    nearest interior value (index clamping);
+//  * the normal n = grad Fs at a vertex comes from the 2x2x2 cells around it
+//    (the four on the high side of each axis minus the four on the low side,
+//    times 1/(4h)), the unit normal is m = n / (|n| + eps), eps = 1e-8 / dx;
+//  * the curvature of a cell is kappa = -div m from its 8 vertices (the four
+//    on the high side of each axis minus the four on the low side, times 1/(4h));
+//  * on a face between cells c-1 and c the acceleration is
+//    sigma (kappa(c-1)+kappa(c))/2 (F(c)-F(c-1))/h / ((rho(c-1)+rho(c))/2).
+// With the smoothing, the face curvature of a resolved sphere is 2/R to 1%.
+// The association written here is the contract the CPU oracle repeats.
+template <class R>
+__device__ __forceinline__ R csf_smooth(const Inst<R>& I, const R* __restrict__ F, int c) {
+    const int sx = I.sx, sy = I.sy;
+    R ry[3];
+    for (int dz = -1; dz <= 1; ++dz) {
+        R rx[3];
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int q = c + dz * sy + dy * sx;
+            rx[dy + 1] = (R(0.25) * F[q - 1] + R(0.5) * F[q]) + R(0.25) * F[q + 1];
+        }
+        ry[dz + 1] = (R(0.25) * rx[0] + R(0.5) * rx[1]) + R(0.25) * rx[2];
+    }
+    return (R(0.25) * ry[0] + R(0.5) * ry[1]) + R(0.25) * ry[2];
+}
+
+template <class R>
+struct VNormal {
+    R x, y, z;
+};
+
+// Unit normal at the vertex whose low corner is cell (i,j,k), 0 <= i <= nx etc.
+template <class R>
+__device__ __forceinline__ VNormal<R> csf_vertex_normal(const Inst<R>& I, const R* __restrict__ Fs, int i, int j,
+                                                        int k) {
+    const int sx = I.sx, sy = I.sy;
+    const int i0 = max(i, 1), i1 = min(i + 1, I.nx), j0 = max(j, 1), j1 = min(j + 1, I.ny);
+    const int k0 = max(k, 1), k1 = min(k + 1, I.nz);
+    const R f000 = Fs[k0 * sy + j0 * sx + i0], f100 = Fs[k0 * sy + j0 * sx + i1];
+    const R f010 = Fs[k0 * sy + j1 * sx + i0], f110 = Fs[k0 * sy + j1 * sx + i1];
+    const R f001 = Fs[k1 * sy + j0 * sx + i0], f101 = Fs[k1 * sy + j0 * sx + i1];
+    const R f011 = Fs[k1 * sy + j1 * sx + i0], f111 = Fs[k1 * sy + j1 * sx + i1];
+    VNormal<R> n;
+    n.x = (((f100 + f110) + (f101 + f111)) - ((f000 + f010) + (f001 + f011))) * (R(0.25) * I.dxi);
+    n.y = (((f010 + f110) + (f011 + f111)) - ((f000 + f100) + (f001 + f101))) * (R(0.25) * I.dyi);
+    n.z = (((f001 + f101) + (f011 + f111)) - ((f000 + f100) + (f010 + f110))) * (R(0.25) * I.dzi);
+    const R mag = sqrt((n.x * n.x + n.y * n.y) + n.z * n.z);
+    const R inv = R(1) / (mag + R(1e-8) * I.dxi);
+    n.x = n.x * inv;
+    n.y = n.y * inv;
+    n.z = n.z * inv;
+    return n;
+}
+
+// Curvature of interior cell (i,j,k) from the smoothed field Fs.
+template <class R>
+__device__ __forceinline__ R csf_curvature(const Inst<R>& I, const R* __restrict__ Fs, int i, int j, int k) {
+    const VNormal<R> v000 = csf_vertex_normal(I, Fs, i - 1, j - 1, k - 1), v100 = csf_vertex_normal(I, Fs, i, j - 1, k - 1);
+    const VNormal<R> v010 = csf_vertex_normal(I, Fs, i - 1, j, k - 1), v110 = csf_vertex_normal(I, Fs, i, j, k - 1);
+    const VNormal<R> v001 = csf_vertex_normal(I, Fs, i - 1, j - 1, k), v101 = csf_vertex_normal(I, Fs, i, j - 1, k);
+    const VNormal<R> v011 = csf_vertex_normal(I, Fs, i - 1, j, k), v111 = csf_vertex_normal(I, Fs, i, j, k);
+    const R kx = (((v100.x + v110.x) + (v101.x + v111.x)) - ((v000.x + v010.x) + (v001.x + v011.x))) * (R(0.25) * I.dxi);
+    const R ky = (((v010.y + v110.y) + (v011.y + v111.y)) - ((v000.y + v100.y) + (v001.y + v101.y))) * (R(0.25) * I.dyi);
+    const R kz = (((v001.z + v101.z) + (v011.z + v111.z)) - ((v000.z + v100.z) + (v010.z + v110.z))) * (R(0.25) * I.dzi);
+    return -((kx + ky) + kz);
+}
+
+// CSF acceleration on the face between cells c-off (low) and c (high), h = 1/spacing.
+template <class R>
+__device__ __forceinline__ R csf_accel(const Inst<R>& I, const R* __restrict__ F, const R* __restrict__ kap, int c,
+                                       int off, R h) {
+    const R kf = R(0.5) * (kap[c - off] + kap[c]);
+    const R gF = (F[c] - F[c - off]) * h;
+    const R rf = R(0.5) * (rho_of(I, F[c - off]) + rho_of(I, F[c]));
+    return I.sigma * kf * gF / rf;
 }
 
 }  // namespace vof

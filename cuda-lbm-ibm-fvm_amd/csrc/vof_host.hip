@@ -320,10 +320,12 @@ struct Ensemble : EnsBase {
         flags = p[0].flags;
         if (p[0].jacobi_tblock > 0) tun.jacobi_tblock = p[0].jacobi_tblock;
         ih.resize(size_t(n));
+        const bool csf = (flags & VOF_FLAG_CSF) != 0;
+        const size_t nfield = csf ? 13 : 11;  // + smoothed F and curvature with CSF
         size_t total = 0;
         for (int i = 0; i < n; ++i) {
             derive(p[i], ih[size_t(i)]);
-            total += size_t(ih[size_t(i)].ntot) * 11;
+            total += size_t(ih[size_t(i)].ntot) * nfield;
         }
         VOF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         const size_t bytes = total * sizeof(R);
@@ -338,9 +340,12 @@ struct Ensemble : EnsBase {
         VOF_HIP(hipMemset(arena, 0, bytes));
         R* cur = static_cast<R*>(arena);
         for (auto& I : ih) {
-            R** slots[11] = {&I.F[0], &I.F[1], &I.u, &I.v, &I.w, &I.us, &I.vs, &I.ws, &I.p[0], &I.p[1], &I.rhs};
-            for (R** s : slots) {
-                *s = cur;
+            R** slots[13] = {&I.F[0], &I.F[1], &I.u, &I.v, &I.w, &I.us, &I.vs, &I.ws, &I.p[0], &I.p[1], &I.rhs,
+                             &I.fsm, &I.kappa};
+            I.fsm = nullptr;
+            I.kappa = nullptr;
+            for (size_t q = 0; q < nfield; ++q) {
+                *slots[q] = cur;
                 cur += I.ntot;
             }
         }
@@ -426,8 +431,13 @@ struct Ensemble : EnsBase {
     }
 
     // ---- one step, the reference loop body (voftrial1.cu:1362-1401).
-    void enqueue_step(int64_t istep) {
+    void enqueue_predict() {
         vof::launch_predict(e, g_pred, fsel, st);             // :1364-1368
+        if (flags & VOF_FLAG_CSF) vof::launch_csf(e, g_sub, fsel, st);
+    }
+
+    void enqueue_step(int64_t istep) {
+        enqueue_predict();
         vof::launch_rhs(e, g_rhs, fsel, st);                  // rhs of :418-422
         enqueue_jacobi();                                     // :1371-1375
         if (flags & VOF_FLAG_REMOVE_P_MEAN) {
@@ -509,6 +519,10 @@ struct Ensemble : EnsBase {
             case 5: vof::launch_fct(e, g_fy, 1, fsel, 0, st); fsel ^= 1; break;
             case 6: vof::launch_fct(e, g_fz, 2, fsel, 0, st); fsel ^= 1; break;
             case 7: vof::launch_clamp_F(e, g_sub, fsel, st); vof::launch_ghost_F(e, g_ghost, fsel, st); break;
+            case 9:
+                if (!(flags & VOF_FLAG_CSF)) { set_err("stage 9 (CSF) needs VOF_FLAG_CSF"); return VOF_ERR_INVALID; }
+                vof::launch_csf(e, g_sub, fsel, st);
+                break;
             default: set_err("unknown stage %d", stage); return VOF_ERR_INVALID;
         }
         VOF_HIP(hipGetLastError());
@@ -528,6 +542,7 @@ struct Ensemble : EnsBase {
             case VOF_FIELD_VSTAR: return I.vs;
             case VOF_FIELD_WSTAR: return I.ws;
             case VOF_FIELD_RHS: return I.rhs;
+            case VOF_FIELD_KAPPA: return I.kappa;
             default: return nullptr;
         }
     }
@@ -567,7 +582,10 @@ struct Ensemble : EnsBase {
             set_err("upload: count %zu != %d stored cells", count, I.ntot);
             return VOF_ERR_INVALID;
         }
-        if (f == VOF_FIELD_RHO || f == VOF_FIELD_NU) { set_err("rho and nu are derived from F"); return VOF_ERR_INVALID; }
+        if (f == VOF_FIELD_RHO || f == VOF_FIELD_NU || f == VOF_FIELD_KAPPA) {
+            set_err("rho, nu and kappa are derived from F");
+            return VOF_ERR_INVALID;
+        }
         R* dst = field_ptr(inst, f);
         if (!dst) { set_err("upload: unknown field %d", int(f)); return VOF_ERR_INVALID; }
         VOF_HIP(hipMemcpyAsync(dst, host, count * sizeof(R), hipMemcpyHostToDevice, st));
@@ -712,7 +730,7 @@ struct Ensemble : EnsBase {
             for (auto& x : ev) VOF_HIP(hipEventCreate(&x));
             for (int s = 0; s < steps; ++s, ++is) {
                 VOF_HIP(hipEventRecord(ev[0], st));
-                vof::launch_predict(e, g_pred, fsel, st);
+                enqueue_predict();
                 VOF_HIP(hipEventRecord(ev[1], st));
                 vof::launch_rhs(e, g_rhs, fsel, st);
                 VOF_HIP(hipEventRecord(ev[2], st));
@@ -802,7 +820,7 @@ struct Ensemble : EnsBase {
     void enqueue_step_taped(int64_t istep, R* tape) {
         const auto& I = ih[0];
         const size_t nt = adj.nt;
-        vof::launch_predict(e, g_pred, fsel, st);
+        enqueue_predict();
         vof::launch_rhs(e, g_rhs, fsel, st);
         enqueue_jacobi();
         vof::launch_correct(e, g_corr, fsel, psel, st);
@@ -1072,8 +1090,8 @@ vof_status validate(const vof_params& p) {
     if (p.precision != 4 && p.precision != 8) { set_err("precision must be 4 or 8, got %d", p.precision); return VOF_ERR_INVALID; }
     if (p.jacobi_sweeps < 0) { set_err("jacobi_sweeps must be >= 0"); return VOF_ERR_INVALID; }
     if (!(p.dt > 0.0) || !(p.Lx > 0.0) || !(p.Ly > 0.0) || !(p.Lz > 0.0)) { set_err("dt and L must be positive"); return VOF_ERR_INVALID; }
-    if (p.flags & (VOF_FLAG_CSF | VOF_FLAG_RHO_PREDICTOR | VOF_FLAG_NO_SLIP)) {
-        set_err("flags 0x%x are not available in this build", p.flags & (VOF_FLAG_CSF | VOF_FLAG_RHO_PREDICTOR | VOF_FLAG_NO_SLIP));
+    if (p.flags & (VOF_FLAG_RHO_PREDICTOR | VOF_FLAG_NO_SLIP)) {
+        set_err("flags 0x%x are not available in this build", p.flags & (VOF_FLAG_RHO_PREDICTOR | VOF_FLAG_NO_SLIP));
         return VOF_ERR_UNSUPPORTED;
     }
     return VOF_OK;

@@ -31,6 +31,8 @@ template <class R> void launch_fct(const Ens<R>& e, const Geo& g, int dir, int s
 template <class R> void launch_clamp_F(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
 template <class R> void launch_ghost_F(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
 template <class R> void launch_sub_mean(const Ens<R>& e, const Geo& g, int psel, const double* means, hipStream_t st);
+// CSF extension: curvature, then the surface-tension acceleration on u*, v*, w*.
+template <class R> void launch_csf(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
 template <class R>
 void launch_reduce(const Ens<R>& e, const Geo& g, int fsel, int psel, int mode, double* part,
                    const long long* ncell, double* out, double* means, hipStream_t st);

@@ -688,6 +688,53 @@ __global__ __launch_bounds__(256) void k_clamp_F(const Inst<R>* __restrict__ tab
 }
 
 // ---------------------------------------------------------------------------
+// CSF surface tension (extension, VOF_FLAG_CSF): F smoothed on interior cells,
+// the curvature of every interior cell from it, then the surface-tension
+// acceleration added to the predicted face velocities (dt a on faces i >= 2,
+// j >= 2, k >= 2, as the predictor writes them).  They run after the
+// predictor and before the rhs.
+template <class R>
+__global__ __launch_bounds__(256) void k_csf_smooth(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
+                                                    int ninst, int fsel, int kz) {
+    VOF_PROLOGUE(R)
+    WMap m;
+    if (!wmap<R, 1>(I, lb, kz, 1, I.nz, m)) return;
+    if (!(m.own && m.i >= 1 && m.i <= I.nx && m.j >= 1 && m.j <= I.ny)) return;
+    const R* __restrict__ F = sel2(I.F, fsel);
+    for (int k = m.k0; k <= m.k1; ++k) {
+        const int c = k * I.sy + m.tc;
+        I.fsm[c] = csf_smooth(I, F, c);
+    }
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_curvature(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
+                                                   int ninst, int kz) {
+    VOF_PROLOGUE(R)
+    WMap m;
+    if (!wmap<R, 1>(I, lb, kz, 1, I.nz, m)) return;
+    if (!(m.own && m.i >= 1 && m.i <= I.nx && m.j >= 1 && m.j <= I.ny)) return;
+    for (int k = m.k0; k <= m.k1; ++k) I.kappa[k * I.sy + m.tc] = csf_curvature(I, I.fsm, m.i, m.j, k);
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_csf(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
+                                             int ninst, int fsel, int kz) {
+    VOF_PROLOGUE(R)
+    WMap m;
+    if (!wmap<R, 1>(I, lb, kz, 1, I.nz, m)) return;
+    const int i = m.i, j = m.j;
+    if (!(m.own && i >= 1 && i <= I.nx && j >= 1 && j <= I.ny)) return;
+    const R* __restrict__ F = sel2(I.F, fsel);
+    for (int k = m.k0; k <= m.k1; ++k) {
+        const int c = k * I.sy + m.tc;
+        if (i >= 2) I.us[c] = I.us[c] + I.dt * csf_accel(I, F, I.kappa, c, 1, I.dxi);
+        if (j >= 2) I.vs[c] = I.vs[c] + I.dt * csf_accel(I, F, I.kappa, c, I.sx, I.dyi);
+        if (k >= 2) I.ws[c] = I.ws[c] + I.dt * csf_accel(I, F, I.kappa, c, I.sy, I.dzi);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Mean-pressure removal (extension; no reference counterpart): subtract the
 // FP64 interior mean, stored by the reduction, from interior p.
 template <class R>
@@ -871,6 +918,12 @@ void launch_ghost_F(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st) {
     hipLaunchKernelGGL(k_ghost_F<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, fsel);
 }
 template <class R>
+void launch_csf(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st) {
+    hipLaunchKernelGGL(k_csf_smooth<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, fsel, g.kz);
+    hipLaunchKernelGGL(k_curvature<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, g.kz);
+    hipLaunchKernelGGL(k_csf<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, fsel, g.kz);
+}
+template <class R>
 void launch_sub_mean(const Ens<R>& e, const Geo& g, int psel, const double* means, hipStream_t st) {
     hipLaunchKernelGGL(k_sub_mean<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, psel, means, g.kz);
 }
@@ -892,6 +945,7 @@ void launch_reduce(const Ens<R>& e, const Geo& g, int fsel, int psel, int mode, 
     template void launch_clamp_F<R>(const Ens<R>&, const Geo&, int, hipStream_t);                        \
     template void launch_ghost_F<R>(const Ens<R>&, const Geo&, int, hipStream_t);                        \
     template void launch_sub_mean<R>(const Ens<R>&, const Geo&, int, const double*, hipStream_t);        \
+    template void launch_csf<R>(const Ens<R>&, const Geo&, int, hipStream_t);                            \
     template void launch_reduce<R>(const Ens<R>&, const Geo&, int, int, int, double*, const long long*, \
                                    double*, double*, hipStream_t);
 

@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libvof_hip.so")
 CLI_PATH = os.path.join(PKG, "lib", "3dvof")
 
 FIELDS = {"F": 0, "U": 1, "V": 2, "W": 3, "P": 4, "USTAR": 5, "VSTAR": 6, "WSTAR": 7, "RHS": 8, "RHO": 9,
-          "NU": 10}
+          "NU": 10, "KAPPA": 11}
 FLAG_REMOVE_P_MEAN = 0x1
 FLAG_CSF = 0x2
 FLAG_RHO_PREDICTOR = 0x4
@@ -35,7 +35,7 @@ STAGE_TIME_NAMES = ("predictor", "rhs", "jacobi", "p_mean", "correction", "fct_x
 
 # Stages for Solver.debug_stage (test hook).
 (STAGE_PREDICT, STAGE_RHS, STAGE_JACOBI, STAGE_CORRECT, STAGE_FCT_X, STAGE_FCT_Y, STAGE_FCT_Z, STAGE_GHOST_F,
- STAGE_JACOBI2) = range(9)
+ STAGE_JACOBI2, STAGE_CSF) = range(10)
 
 
 class VofParams(ctypes.Structure):

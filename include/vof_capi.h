@@ -51,12 +51,16 @@ typedef enum vof_field {
     VOF_FIELD_WSTAR = 7,
     VOF_FIELD_RHS = 8,
     VOF_FIELD_RHO = 9,
-    VOF_FIELD_NU = 10
+    VOF_FIELD_NU = 10,
+    VOF_FIELD_KAPPA = 11 /* interface curvature of the last step (VOF_FLAG_CSF only; read-only) */
 } vof_field;
 
 /* Extension flags (no counterpart in the reference; all default off). */
 #define VOF_FLAG_REMOVE_P_MEAN 0x1u /* subtract the FP64 interior mean of p after the Poisson sweeps */
-#define VOF_FLAG_CSF           0x2u /* continuum-surface-force surface tension in the predictor */
+/* Continuum-surface-force surface tension: after the predictor, every interior
+ * face gains dt * sigma * kappa_f * dF/dn / rho_f, kappa = -div(grad F/|grad F|)
+ * from vertex normals.  sigma = 0 gives the parity step. */
+#define VOF_FLAG_CSF           0x2u
 #define VOF_FLAG_RHO_PREDICTOR 0x4u /* gravity/CSF scaled by face density in the predictor */
 #define VOF_FLAG_NO_SLIP       0x8u /* tangential wall ghosts mirror to give zero wall velocity */
 #define VOF_FLAG_WATCHDOG      0x10u/* check F in [0,1] and finite u,v,w,p after every step */
@@ -178,8 +182,9 @@ vof_status vof_autotune(vof_solver* s, int64_t first_istep, int steps_per_trial,
 
 /* Test hook: run one stage on the current state.  stage: 0 predictor,
  * 1 rhs, 2 one Jacobi sweep, 3 correction + velocity BC, 4/5/6 FCT sweep
- * x/y/z, 7 clamp + F ghost fill, 8 two Jacobi sweeps in one launch.  Not
- * part of the reference surface. */
+ * x/y/z, 7 clamp + F ghost fill, 8 two Jacobi sweeps in one launch, 9 CSF
+ * smoothing + curvature + face force (VOF_FLAG_CSF only).  Not part of the
+ * reference surface. */
 vof_status vof_debug_stage(vof_solver* s, int stage);
 
 /* Number of interior cells n_x n_y n_z and of stored cells (n+2)^3. */

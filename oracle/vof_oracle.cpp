@@ -45,7 +45,7 @@ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v
 
 enum OrcField {
     O_F = 0, O_FTD, O_AX, O_AY, O_AZ, O_CX, O_CY, O_CZ, O_RP, O_RM,
-    O_U, O_V, O_W, O_US, O_VS, O_WS, O_P, O_PT, O_RHO, O_NU, O_NFIELDS
+    O_U, O_V, O_W, O_US, O_VS, O_WS, O_P, O_PT, O_RHO, O_NU, O_FSM, O_KAPPA, O_NFIELDS
 };
 
 struct OrcBase {
@@ -581,10 +581,89 @@ struct Orc : OrcBase {
                 }
     }
 
+    // ---- CSF surface tension (extension, VOF_FLAG_CSF; sigma is stored but never
+    //      used by the reference, :35, :1279).  F smoothed once by the separable
+    //      (1/4, 1/2, 1/4) filter on interior cells.
+    void smooth() {
+        const R* F = fld(O_F);
+        R* fs = fld(O_FSM);
+#pragma omp parallel for collapse(2)
+        for (int k = kmin; k <= kmax; ++k)
+            for (int j = jmin; j <= jmax; ++j)
+                for (int i = imin; i <= imax; ++i) {
+                    R ry[3];
+                    for (int dz = -1; dz <= 1; ++dz) {
+                        R rx[3];
+                        for (int dy = -1; dy <= 1; ++dy)
+                            rx[dy + 1] = (R(0.25) * F[id(i - 1, j + dy, k + dz)] + R(0.5) * F[id(i, j + dy, k + dz)]) +
+                                         R(0.25) * F[id(i + 1, j + dy, k + dz)];
+                        ry[dz + 1] = (R(0.25) * rx[0] + R(0.5) * rx[1]) + R(0.25) * rx[2];
+                    }
+                    fs[id(i, j, k)] = (R(0.25) * ry[0] + R(0.5) * ry[1]) + R(0.25) * ry[2];
+                }
+    }
+    // Unit normal at the vertex whose low corner is cell (i,j,k), from the
+    // smoothed field of the 2x2x2 cells around it (indices clamped to the interior).
+    struct VN { R x, y, z; };
+    VN vertex_normal(int i, int j, int k) const {
+        const R* F = a[O_FSM].data();
+        const int i0 = std::max(i, 1), i1 = std::min(i + 1, nx), j0 = std::max(j, 1), j1 = std::min(j + 1, ny);
+        const int k0 = std::max(k, 1), k1 = std::min(k + 1, nz);
+        const R f000 = F[id(i0, j0, k0)], f100 = F[id(i1, j0, k0)], f010 = F[id(i0, j1, k0)],
+                f110 = F[id(i1, j1, k0)], f001 = F[id(i0, j0, k1)], f101 = F[id(i1, j0, k1)],
+                f011 = F[id(i0, j1, k1)], f111 = F[id(i1, j1, k1)];
+        VN n;
+        n.x = (((f100 + f110) + (f101 + f111)) - ((f000 + f010) + (f001 + f011))) * (R(0.25) * dxi);
+        n.y = (((f010 + f110) + (f011 + f111)) - ((f000 + f100) + (f001 + f101))) * (R(0.25) * dyi);
+        n.z = (((f001 + f101) + (f011 + f111)) - ((f000 + f100) + (f010 + f110))) * (R(0.25) * dzi);
+        const R mag = std::sqrt((n.x * n.x + n.y * n.y) + n.z * n.z);
+        const R inv = R(1) / (mag + R(1e-8) * dxi);
+        n.x = n.x * inv; n.y = n.y * inv; n.z = n.z * inv;
+        return n;
+    }
+    // kappa = -div m over the cell's 8 vertices, interior cells.
+    void curvature() {
+        R* kap = fld(O_KAPPA);
+#pragma omp parallel for collapse(2)
+        for (int k = kmin; k <= kmax; ++k)
+            for (int j = jmin; j <= jmax; ++j)
+                for (int i = imin; i <= imax; ++i) {
+                    const VN v000 = vertex_normal(i - 1, j - 1, k - 1), v100 = vertex_normal(i, j - 1, k - 1);
+                    const VN v010 = vertex_normal(i - 1, j, k - 1), v110 = vertex_normal(i, j, k - 1);
+                    const VN v001 = vertex_normal(i - 1, j - 1, k), v101 = vertex_normal(i, j - 1, k);
+                    const VN v011 = vertex_normal(i - 1, j, k), v111 = vertex_normal(i, j, k);
+                    const R kx = (((v100.x + v110.x) + (v101.x + v111.x)) - ((v000.x + v010.x) + (v001.x + v011.x))) * (R(0.25) * dxi);
+                    const R ky = (((v010.y + v110.y) + (v011.y + v111.y)) - ((v000.y + v100.y) + (v001.y + v101.y))) * (R(0.25) * dyi);
+                    const R kz = (((v001.z + v101.z) + (v011.z + v111.z)) - ((v000.z + v100.z) + (v010.z + v110.z))) * (R(0.25) * dzi);
+                    kap[id(i, j, k)] = -((kx + ky) + kz);
+                }
+    }
+    // u* += dt sigma kappa_f dF/dn / rho_f on the faces the predictor writes.
+    void csf() {
+        const R *F = fld(O_F), *kap = fld(O_KAPPA), *rho = fld(O_RHO);
+        R *us = fld(O_US), *vs = fld(O_VS), *ws = fld(O_WS);
+        auto acc = [&](size_t c, size_t m, R h) {
+            const R kf = R(0.5) * (kap[m] + kap[c]);
+            const R gF = (F[c] - F[m]) * h;
+            const R rf = R(0.5) * (rho[m] + rho[c]);
+            return sigma * kf * gF / rf;
+        };
+#pragma omp parallel for collapse(2)
+        for (int k = kmin; k <= kmax; ++k)
+            for (int j = jmin; j <= jmax; ++j)
+                for (int i = imin; i <= imax; ++i) {
+                    const size_t c = id(i, j, k);
+                    if (i > imin) us[c] = us[c] + dt * acc(c, id(i - 1, j, k), dxi);
+                    if (j > jmin) vs[c] = vs[c] + dt * acc(c, id(i, j - 1, k), dyi);
+                    if (k > kmin) ws[c] = ws[c] + dt * acc(c, id(i, j, k - 1), dzi);
+                }
+    }
+
     // One time step, loop body :1362-1401 (istep is the already-incremented index).
     void step(long long istep) {
         nu_rho();                       // :1364
         advect_u(); advect_v(); advect_w();  // :1366-1368
+        if (flags & VOF_FLAG_CSF) { smooth(); curvature(); csf(); }
         jacobi();                       // :1371-1375
         if (flags & VOF_FLAG_REMOVE_P_MEAN) remove_p_mean();
         update_uvw();                   // :1377-1379
@@ -655,6 +734,7 @@ int field_index(int f) {
         case VOF_FIELD_WSTAR: return O_WS;
         case VOF_FIELD_RHO: return O_RHO;
         case VOF_FIELD_NU: return O_NU;
+        case VOF_FIELD_KAPPA: return O_KAPPA;
         default: return -1;
     }
 }
@@ -726,7 +806,8 @@ int orc_step(void* h, long long first_istep, int nsteps) {
 
 // Individual stages, for stage-level tests.  id: 0 nu_rho, 1 advect_u, 2 advect_v,
 // 3 advect_w, 4 one Jacobi sweep (ping-pong), 5 update_uvw, 6 bc_vel, 7 fct_x,
-// 8 fct_y, 9 fct_z, 10 clamp_F, 11 bc_F, 12 remove_p_mean.
+// 8 fct_y, 9 fct_z, 10 clamp_F, 11 bc_F, 12 remove_p_mean, 13 CSF smoothing and
+// curvature, 14 CSF face acceleration.
 int orc_stage(void* h, int stage) {
     int rc = 0;
     ORC_DISPATCH(h, {
@@ -749,6 +830,8 @@ int orc_stage(void* h, int stage) {
             case 10: o->clamp_F(); break;
             case 11: o->bc_F(); break;
             case 12: o->remove_p_mean(); break;
+            case 13: o->smooth(); o->curvature(); break;
+            case 14: o->csf(); break;
             default: rc = 1;
         }
     });
