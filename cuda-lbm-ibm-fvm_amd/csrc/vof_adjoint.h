@@ -8,7 +8,7 @@
 namespace vof {
 
 // Scalar gradients accumulated on the device, in this order.
-enum AdjScalar { ADJ_RHO_L = 0, ADJ_RHO_G, ADJ_NU_L, ADJ_NU_G, ADJ_GX, ADJ_GY, ADJ_GZ, ADJ_NSCALAR };
+enum AdjScalar { ADJ_RHO_L = 0, ADJ_RHO_G, ADJ_NU_L, ADJ_NU_G, ADJ_GX, ADJ_GY, ADJ_GZ, ADJ_SIGMA, ADJ_NSCALAR };
 
 inline int adj_blocks(long n) { return int((n + 255) / 256); }
 // doubles needed by the `part` scratch of the launchers below
@@ -43,6 +43,13 @@ template <class R> void adj_predict(const Inst<R>* tab, const Inst<R>& h, const 
                                     const R* w, const double* aus, const double* avs, const double* aws,
                                     const double* arho, double* au, double* av, double* aw, double* aF, double* part,
                                     double* acc, hipStream_t st);
+// CSF in reverse (VOF_FLAG_CSF): recompute Fs and kappa of the step into fsm,
+// kap; then akap = (scratch), aF += and arho += the face-force VJP, afs =
+// (scratch) the curvature VJP, aF += its smoothing transpose (ghosts folded to
+// the nearest interior cell); adds dJ/dsigma to acc[ADJ_SIGMA].
+template <class R> void adj_csf(const Inst<R>* tab, const Inst<R>& h, const R* F, R* fsm, R* kap, const double* aus,
+                                const double* avs, const double* aws, double* akap, double* afs, double* aF,
+                                double* arho, double* part, double* acc, hipStream_t st);
 // a = b = c = 0 over n doubles.
 void adj_zero3(double* a, double* b, double* c, long n, hipStream_t st);
 

@@ -841,6 +841,143 @@ __global__ __launch_bounds__(256) void k_predict_adj(const Inst<R>* __restrict__
     block_parts<7>(sums, part);
 }
 
+// ---------------------------------------------------------------------------
+// CSF surface tension in reverse (VOF_FLAG_CSF).  Forward, per face between
+// cells m (low) and q (high): u* += dt a, a = sigma kf gF / rf with
+// kf = (kappa(m)+kappa(q))/2, gF = (F(q)-F(m)) h, rf = (rho(m)+rho(q))/2;
+// kappa from the unit vertex normals of Fs, Fs the smoothed F.  The step's Fs
+// and kappa are first recomputed from its F with the forward's functions.
+template <class R>
+__global__ __launch_bounds__(256) void k_adj_smooth(const Inst<R>* __restrict__ tab, const R* __restrict__ F,
+                                                    R* __restrict__ fsm) {
+    const Inst<R>& I = tab[0];
+    int i, j, k, c;
+    if (interior_cell(I, i, j, k, c)) fsm[c] = csf_smooth(I, F, c);
+}
+template <class R>
+__global__ __launch_bounds__(256) void k_adj_curv(const Inst<R>* __restrict__ tab, const R* __restrict__ fsm,
+                                                  R* __restrict__ kap) {
+    const Inst<R>& I = tab[0];
+    int i, j, k, c;
+    if (interior_cell(I, i, j, k, c)) kap[c] = csf_curvature(I, fsm, i, j, k);
+}
+
+// Face-force VJP, gathered per interior cell c over the (up to) six faces it
+// touches: akap(c) = (assigned), aF(c) += (the F difference), arho(c) += (the
+// face density); work-group sums of dJ/dsigma (each face counted once, at its
+// high cell).
+template <class R>
+__global__ __launch_bounds__(256) void k_csf_adj(const Inst<R>* __restrict__ tab, const R* __restrict__ F,
+                                                 const R* __restrict__ kap, const double* __restrict__ aus,
+                                                 const double* __restrict__ avs, const double* __restrict__ aws,
+                                                 double* __restrict__ akap, double* __restrict__ aF,
+                                                 double* __restrict__ arho, double* __restrict__ part) {
+    const Inst<R>& I = tab[0];
+    int i, j, k, c;
+    double sg[1] = {0.0};
+    if (interior_cell(I, i, j, k, c)) {
+        const double sigma = I.sigma, dt = I.dt;
+        double gk = 0.0, gf = 0.0, gr = 0.0;
+        // face with high cell q, low cell q - off; c_hi: c is the high cell
+        auto face = [&](const double* ag, int q, int off, double h, bool c_hi) {
+            const double g = ag[q] * dt;
+            if (g == 0.0) return;
+            const int m = q - off;
+            const double kf = 0.5 * (double(kap[m]) + double(kap[q]));
+            const double gF = (double(F[q]) - double(F[m])) * h;
+            const double rr = 1.0 / (0.5 * (rho_d(I, F[m]) + rho_d(I, F[q])));
+            const double a = sigma * kf * gF * rr;
+            if (c_hi) sg[0] += g * kf * gF * rr;
+            gk += 0.5 * g * sigma * gF * rr;
+            gf += (c_hi ? g : -g) * sigma * kf * h * rr;
+            gr -= 0.5 * g * a * rr;
+        };
+        const int sx = I.sx, sy = I.sy;
+        const double hx = I.dxi, hy = I.dyi, hz = I.dzi;
+        if (i >= 2) face(aus, c, 1, hx, true);
+        if (i + 1 <= I.nx) face(aus, c + 1, 1, hx, false);
+        if (j >= 2) face(avs, c, sx, hy, true);
+        if (j + 1 <= I.ny) face(avs, c + sx, sx, hy, false);
+        if (k >= 2) face(aws, c, sy, hz, true);
+        if (k + 1 <= I.nz) face(aws, c + sy, sy, hz, false);
+        akap[c] = gk;
+        aF[c] += gf;
+        arho[c] += gr;
+    }
+    block_parts<1>(sg, part);
+}
+
+// Curvature VJP per vertex (low corner (i,j,k), 0 <= i <= nx ...): gather the
+// unit-normal adjoint from the curvature adjoints of the (interior) cells
+// around it, through the normalisation to grad Fs, and scatter onto the 8
+// (index-clamped) cells that formed it.  2-D launch over (nx+1)(ny+1) x (nz+1).
+template <class R>
+__global__ __launch_bounds__(256) void k_curv_adj(const Inst<R>* __restrict__ tab, const R* __restrict__ fsm,
+                                                  const double* __restrict__ akap, double* __restrict__ afs) {
+    const Inst<R>& I = tab[0];
+    const int nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
+    const int t = int(blockIdx.x) * 256 + int(threadIdx.x);
+    if (t >= (nx + 1) * (ny + 1)) return;
+    const int j = t / (nx + 1), i = t - j * (nx + 1), k = int(blockIdx.y);
+    const double qx = 0.25 * double(I.dxi), qy = 0.25 * double(I.dyi), qz = 0.25 * double(I.dzi);
+    // kappa(cell) = -(kx+ky+kz): the vertex is on the high side of cell i (di=0), low side of i+1
+    double amx = 0.0, amy = 0.0, amz = 0.0;
+    for (int dk = 0; dk < 2; ++dk)
+        for (int dj = 0; dj < 2; ++dj)
+            for (int di = 0; di < 2; ++di) {
+                const int ci = i + di, cj = j + dj, ck = k + dk;
+                if (ci < 1 || ci > nx || cj < 1 || cj > ny || ck < 1 || ck > nz) continue;
+                const double g = akap[ck * sy + cj * sx + ci];
+                amx -= (di ? -qx : qx) * g;
+                amy -= (dj ? -qy : qy) * g;
+                amz -= (dk ? -qz : qz) * g;
+            }
+    if (amx == 0.0 && amy == 0.0 && amz == 0.0) return;
+    const int i0 = max(i, 1), i1 = min(i + 1, nx), j0 = max(j, 1), j1 = min(j + 1, ny);
+    const int k0 = max(k, 1), k1 = min(k + 1, nz);
+    const int cell[8] = {k0 * sy + j0 * sx + i0, k0 * sy + j0 * sx + i1, k0 * sy + j1 * sx + i0,
+                         k0 * sy + j1 * sx + i1, k1 * sy + j0 * sx + i0, k1 * sy + j0 * sx + i1,
+                         k1 * sy + j1 * sx + i0, k1 * sy + j1 * sx + i1};  // (di,dj,dk) = bits 0,1,2
+    double f[8];
+    for (int b = 0; b < 8; ++b) f[b] = fsm[cell[b]];
+    const double nxv = (((f[1] + f[3]) + (f[5] + f[7])) - ((f[0] + f[2]) + (f[4] + f[6]))) * qx;
+    const double nyv = (((f[2] + f[3]) + (f[6] + f[7])) - ((f[0] + f[1]) + (f[4] + f[5]))) * qy;
+    const double nzv = (((f[4] + f[5]) + (f[6] + f[7])) - ((f[0] + f[1]) + (f[2] + f[3]))) * qz;
+    const double mag = sqrt((nxv * nxv + nyv * nyv) + nzv * nzv);
+    const double inv = 1.0 / (mag + 1e-8 * double(I.dxi));
+    double anx = amx * inv, any = amy * inv, anz = amz * inv;
+    if (mag > 0.0) {
+        const double s = (nxv * amx + nyv * amy + nzv * amz) * inv * inv / mag;
+        anx -= nxv * s;
+        any -= nyv * s;
+        anz -= nzv * s;
+    }
+    for (int b = 0; b < 8; ++b) {
+        const double g = ((b & 1) ? anx : -anx) * qx + ((b & 2) ? any : -any) * qy + ((b & 4) ? anz : -anz) * qz;
+        aadd(&afs[cell[b]], g);
+    }
+}
+
+// Smoothing transpose: every interior cell spreads its Fs adjoint back over
+// the 27 cells it averaged, a ghost cell standing for its nearest interior cell.
+template <class R>
+__global__ __launch_bounds__(256) void k_smooth_adj(const Inst<R>* __restrict__ tab, const double* __restrict__ afs,
+                                                    double* __restrict__ aF) {
+    const Inst<R>& I = tab[0];
+    int i, j, k, c;
+    if (!interior_cell(I, i, j, k, c)) return;
+    const double g = afs[c];
+    if (g == 0.0) return;
+    const double w[3] = {0.25, 0.5, 0.25};
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int ti = min(max(i + dx, 1), I.nx), tj = min(max(j + dy, 1), I.ny);
+                const int tk = min(max(k + dz, 1), I.nz);
+                aadd(&aF[tk * I.sy + tj * I.sx + ti], w[dx + 1] * w[dy + 1] * w[dz + 1] * g);
+            }
+}
+
 __global__ __launch_bounds__(256) void k_zero3(double* __restrict__ a, double* __restrict__ b,
                                                double* __restrict__ c, long n) {
     const long t = long(blockIdx.x) * 256 + threadIdx.x;
@@ -933,6 +1070,27 @@ void adj_predict(const Inst<R>* tab, const Inst<R>& h, const R* F, const R* u, c
                        aw, aF, part, kz);
     hipLaunchKernelGGL(k_sum_parts<7>, dim3(1), dim3(256), 0, st, part, nb, acc);
 }
+template <class R>
+void adj_csf(const Inst<R>* tab, const Inst<R>& h, const R* F, R* fsm, R* kap, const double* aus, const double* avs,
+             const double* aws, double* akap, double* afs, double* aF, double* arho, double* part, double* acc,
+             hipStream_t st) {
+    const dim3 g = grid_interior(h);
+    hipLaunchKernelGGL(k_adj_smooth<R>, g, dim3(256), 0, st, tab, F, fsm);
+    hipLaunchKernelGGL(k_adj_curv<R>, g, dim3(256), 0, st, tab, fsm, kap);
+    hipLaunchKernelGGL(k_csf_adj<R>, g, dim3(256), 0, st, tab, F, kap, aus, avs, aws, akap, aF, arho, part);
+    hipLaunchKernelGGL(k_sum_parts<1>, dim3(1), dim3(256), 0, st, part, int(g.x * g.y), acc + ADJ_SIGMA);
+    (void)hipMemsetAsync(afs, 0, size_t(h.ntot) * sizeof(double), st);
+    hipLaunchKernelGGL(k_curv_adj<R>, dim3(adj_blocks(long(h.nx + 1) * (h.ny + 1)), h.nz + 1), dim3(256), 0, st, tab,
+                       fsm, akap, afs);
+    hipLaunchKernelGGL(k_smooth_adj<R>, g, dim3(256), 0, st, tab, afs, aF);
+}
+template void adj_csf<float>(const Inst<float>*, const Inst<float>&, const float*, float*, float*, const double*,
+                             const double*, const double*, double*, double*, double*, double*, double*, double*,
+                             hipStream_t);
+template void adj_csf<double>(const Inst<double>*, const Inst<double>&, const double*, double*, double*,
+                              const double*, const double*, const double*, double*, double*, double*, double*,
+                              double*, double*, hipStream_t);
+
 void adj_zero3(double* a, double* b, double* c, long n, hipStream_t st) {
     hipLaunchKernelGGL(k_zero3, dim3(adj_blocks(n)), dim3(256), 0, st, a, b, c, n);
 }

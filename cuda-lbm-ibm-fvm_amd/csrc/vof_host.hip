@@ -208,6 +208,7 @@ struct Ensemble : EnsBase {
         double* d = nullptr;  // FP64 arrays, ntot each, see adj_alloc
         double *aF[2] = {}, *au[2] = {}, *av[2] = {}, *aw[2] = {}, *ap[2] = {}, *gt = nullptr;
         double *aus = nullptr, *avs = nullptr, *aws = nullptr, *arhs = nullptr, *arho = nullptr;
+        double *akap = nullptr, *afs = nullptr;  // CSF only
         double* seed[5] = {};  // F, u, v, w, p seeds of the final state (nullptr: zero)
         double* seed_mem = nullptr;
         double* part = nullptr;
@@ -841,8 +842,8 @@ struct Ensemble : EnsBase {
 
     vof_status adj_forward(int64_t first, int n, int every, double* J) override {
         if (ninst() != 1) { set_err("reverse mode runs on a single instance"); return VOF_ERR_UNSUPPORTED; }
-        if (flags & ~uint32_t(VOF_FLAG_WATCHDOG)) {
-            set_err("reverse mode supports flags 0 and VOF_FLAG_WATCHDOG only (got 0x%x)", flags);
+        if (flags & ~uint32_t(VOF_FLAG_WATCHDOG | VOF_FLAG_CSF)) {
+            set_err("reverse mode supports flags VOF_FLAG_CSF and VOF_FLAG_WATCHDOG only (got 0x%x)", flags);
             return VOF_ERR_UNSUPPORTED;
         }
         if (n < 1 || every < 1 || !J) { set_err("adj_forward: need nsteps >= 1, checkpoint_every >= 1, J"); return VOF_ERR_INVALID; }
@@ -937,7 +938,7 @@ struct Ensemble : EnsBase {
     // kernel never writes (the F ghosts, the unused u* wall faces) stay zero.
     vof_status adj_alloc() {
         const size_t nt = adj.nt;
-        constexpr size_t kArrays = 16;
+        const size_t kArrays = (flags & VOF_FLAG_CSF) ? 18 : 16;
         if (!adj.d) {
             VOF_HIP(hipMalloc(&adj.d, kArrays * nt * sizeof(double)));
             VOF_HIP(hipMemsetAsync(adj.d, 0, kArrays * nt * sizeof(double), st));
@@ -949,6 +950,7 @@ struct Ensemble : EnsBase {
         }
         adj.gt = next(); adj.aus = next(); adj.avs = next(); adj.aws = next();
         adj.arhs = next(); adj.arho = next();
+        if (flags & VOF_FLAG_CSF) { adj.akap = next(); adj.afs = next(); }
         return VOF_OK;
     }
 
@@ -990,6 +992,9 @@ struct Ensemble : EnsBase {
             }
             vof::adj_rhs(e.tab, I, SF, us, vs, ws, adj.arhs, adj.aus, adj.avs, adj.aws, adj.arho, st);
         }
+        if (flags & VOF_FLAG_CSF)
+            vof::adj_csf(e.tab, I, SF, I.fsm, I.kappa, adj.aus, adj.avs, adj.aws, adj.akap, adj.afs, adj.aF[f],
+                         adj.arho, adj.part, adj.acc, st);
         vof::adj_predict(e.tab, I, SF, Su, Sv, Sw, adj.aus, adj.avs, adj.aws, adj.arho, adj.au[x], adj.av[x],
                          adj.aw[x], adj.aF[f], adj.part, adj.acc, st);
         adj.cur = x;
@@ -1050,7 +1055,7 @@ struct Ensemble : EnsBase {
             out->d_gx = acc[vof::ADJ_GX];
             out->d_gy = acc[vof::ADJ_GY];
             out->d_gz = acc[vof::ADJ_GZ];
-            out->d_sigma = 0.0;
+            out->d_sigma = acc[vof::ADJ_SIGMA];
         }
         return VOF_OK;
     }

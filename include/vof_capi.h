@@ -228,17 +228,20 @@ int vof_batch_size(const vof_batch* b);
  * vof_adj_forward advances nsteps steps from the solver's current state
  * (exactly as vof_step does) keeping a copy of the state every
  * checkpoint_every steps, and returns J.  vof_adj_backward then runs the
- * steps in reverse, recomputing each checkpoint interval forward once and
- * each step's intermediates from its stored state, and leaves the solver at
+ * steps in reverse, recomputing each checkpoint interval forward once (taping
+ * each step's u*, v*, w* and the F between sweeps), and leaves the solver at
  * the state vof_adj_forward ended on.  The seed of the final state is
  * gJ * dJ/d(state) plus any field set with vof_adj_seed between the two calls
  * (vof_adj_forward clears them), so gJ = 0 with seeds gives a general
- * vector-Jacobian product.
- * Supported: flags 0 or VOF_FLAG_WATCHDOG. */
+ * vector-Jacobian product.  The F ghost layer is treated as what the solver
+ * keeps it, a copy of the nearest interior cell: the adjoint of F is reported
+ * on interior cells (ghost entries 0), that of p likewise (ghost pressures
+ * only meet zero coefficients).
+ * Supported: flags 0, VOF_FLAG_CSF, VOF_FLAG_WATCHDOG. */
 typedef struct vof_adj_scalars {
     double d_rho_l, d_rho_g, d_nu_l, d_nu_g; /* dJ/d parameter */
     double d_gx, d_gy, d_gz;
-    double d_sigma; /* 0: sigma does not enter the reference physics */
+    double d_sigma; /* 0 unless VOF_FLAG_CSF: sigma does not enter the reference physics */
 } vof_adj_scalars;
 vof_status vof_adj_forward(vof_solver* s, int64_t first_istep, int nsteps, int checkpoint_every, double* J);
 /* Seed of a final-state field (F, U, V, W or P), count = stored cells, FP64. */

@@ -16,7 +16,8 @@ T = torch.float64
 
 class Prm:
     def __init__(self, nx, ny, nz, L=(0.1, 0.1, 0.1), rho_l=1000.0, rho_g=50.0, nu_l=1e-6, nu_g=1.5e-5,
-                 g=(0.0, 0.0, -9.81), dt=4e-6, sweeps=10):
+                 g=(0.0, 0.0, -9.81), dt=4e-6, sweeps=10, csf=False, sigma=0.0):
+        self.csf, self.sigma = csf, sigma  # surface tension (extension); sigma may be a 0-d tensor
         self.nx, self.ny, self.nz = nx, ny, nz
         self.dx, self.dy, self.dz = L[0] / nx, L[1] / ny, L[2] / nz
         self.dxi, self.dyi, self.dzi = 1.0 / self.dx, 1.0 / self.dy, 1.0 / self.dz
@@ -32,6 +33,30 @@ def clamp01(v):
     one = torch.ones((), dtype=v.dtype)
     zero = torch.zeros((), dtype=v.dtype)
     return ((one + v) - torch.maximum(one, v)) - torch.minimum(zero, v)
+
+
+class _Sqrt(torch.autograd.Function):
+    """Correctly rounded FP64 square root (numpy's), derivative 1/(2 sqrt x).
+
+    CPU torch.sqrt in FP64 is not correctly rounded (it differs from IEEE sqrt
+    by one ulp on some inputs), while the oracle's std::sqrt and the device
+    sqrt are; this keeps the restatement bitwise with both."""
+
+    @staticmethod
+    def forward(ctx, x):
+        import numpy as np
+        out = torch.from_numpy(np.sqrt(x.detach().cpu().numpy())).to(x.dtype)
+        ctx.save_for_backward(out)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        (out,) = ctx.saved_tensors
+        return g * 0.5 / out
+
+
+def sqrt_rn(x):
+    return _Sqrt.apply(x)
 
 
 def interior(a):
@@ -230,6 +255,52 @@ def ghost_F(P, F_int):
     return F_int[ck][:, cj][:, :, ci]
 
 
+def csf(P, F, rho, us, vs, ws):
+    """CSF surface tension (extension, VOF_FLAG_CSF): F smoothed by the separable
+    (1/4,1/2,1/4) filter on interior cells, unit normals at vertices from the
+    smoothed field (indices clamped to the interior), kappa = -div of the unit
+    normal per cell, face acceleration sigma kappa_f dF/dn / rho_f added to u*."""
+    rx = (0.25 * F[:, :, :-2] + 0.5 * F[:, :, 1:-1]) + 0.25 * F[:, :, 2:]
+    ry = (0.25 * rx[:, :-2, :] + 0.5 * rx[:, 1:-1, :]) + 0.25 * rx[:, 2:, :]
+    fs = (0.25 * ry[:-2] + 0.5 * ry[1:-1]) + 0.25 * ry[2:]
+    Fp = torch.nn.functional.pad(fs[None, None], (1, 1, 1, 1, 1, 1), mode="replicate")[0, 0]
+    f000, f100, f010, f110 = Fp[:-1, :-1, :-1], Fp[:-1, :-1, 1:], Fp[:-1, 1:, :-1], Fp[:-1, 1:, 1:]
+    f001, f101, f011, f111 = Fp[1:, :-1, :-1], Fp[1:, :-1, 1:], Fp[1:, 1:, :-1], Fp[1:, 1:, 1:]
+    nx_ = (((f100 + f110) + (f101 + f111)) - ((f000 + f010) + (f001 + f011))) * (0.25 * P.dxi)
+    ny_ = (((f010 + f110) + (f011 + f111)) - ((f000 + f100) + (f001 + f101))) * (0.25 * P.dyi)
+    nz_ = (((f001 + f101) + (f011 + f111)) - ((f000 + f100) + (f010 + f110))) * (0.25 * P.dzi)
+    s = (nx_ * nx_ + ny_ * ny_) + nz_ * nz_
+    mag = torch.where(s > 0, sqrt_rn(torch.where(s > 0, s, torch.ones((), dtype=T))), torch.zeros((), dtype=T))
+    inv = torch.ones((), dtype=T) / (mag + 1e-8 * P.dxi)
+    mx, my, mz = nx_ * inv, ny_ * inv, nz_ * inv
+    n3 = (P.nz, P.ny, P.nx)
+
+    def v(m, di, dj, dk):
+        return m[dk:dk + n3[0], dj:dj + n3[1], di:di + n3[2]]
+
+    kx = (((v(mx, 1, 0, 0) + v(mx, 1, 1, 0)) + (v(mx, 1, 0, 1) + v(mx, 1, 1, 1))) -
+          ((v(mx, 0, 0, 0) + v(mx, 0, 1, 0)) + (v(mx, 0, 0, 1) + v(mx, 0, 1, 1)))) * (0.25 * P.dxi)
+    ky = (((v(my, 0, 1, 0) + v(my, 1, 1, 0)) + (v(my, 0, 1, 1) + v(my, 1, 1, 1))) -
+          ((v(my, 0, 0, 0) + v(my, 1, 0, 0)) + (v(my, 0, 0, 1) + v(my, 1, 0, 1)))) * (0.25 * P.dyi)
+    kz = (((v(mz, 0, 0, 1) + v(mz, 1, 0, 1)) + (v(mz, 0, 1, 1) + v(mz, 1, 1, 1))) -
+          ((v(mz, 0, 0, 0) + v(mz, 1, 0, 0)) + (v(mz, 0, 1, 0) + v(mz, 1, 1, 0)))) * (0.25 * P.dzi)
+    kap = torch.nn.functional.pad(-((kx + ky) + kz), (1, 1, 1, 1, 1, 1))
+
+    K = J = I = slice(1, -1)
+
+    def acc(c, m, h):
+        kf = 0.5 * (kap[m] + kap[c])
+        gF = (F[c] - F[m]) * h
+        rf = 0.5 * (rho[m] + rho[c])
+        return P.sigma * kf * gF / rf
+
+    I2 = slice(2, -1)
+    us = put(us, (K, J, I2), us[K, J, I2] + P.dt * acc((K, J, I2), (K, J, slice(1, -2)), P.dxi))
+    vs = put(vs, (K, I2, I), vs[K, I2, I] + P.dt * acc((K, I2, I), (K, slice(1, -2), I), P.dyi))
+    ws = put(ws, (I2, J, I), ws[I2, J, I] + P.dt * acc((I2, J, I), (slice(1, -2), J, I), P.dzi))
+    return us, vs, ws
+
+
 def step(P, st, istep, coeffs=None):
     F, u, v, w, p = st
     coeffs = coeffs if coeffs is not None else jacobi_coeffs(P)
@@ -237,6 +308,8 @@ def step(P, st, istep, coeffs=None):
     rho = P.rho_g * (1.0 - f) + P.rho_l * f
     nu = P.nu_l * f + P.nu_g * (1.0 - f)
     us, vs, ws = predictor(P, u, v, w, nu)
+    if P.csf:
+        us, vs, ws = csf(P, F, rho, us, vs, ws)
     b = rhs_of(P, us, vs, ws, rho)
     for _ in range(P.sweeps):
         p = jacobi(P, p, b, coeffs)

@@ -204,3 +204,67 @@ def test_adjoint_rejects_unsupported_flags(vofmod):
     with pytest.raises(vofmod.VofError):
         s.adj_forward(1, 2, 1)
     s.close()
+
+
+def fold_ghosts(g):
+    """Add each ghost entry to the interior cell it copies (the F ghost fill),
+    leaving zero ghosts."""
+    out = np.zeros_like(g)
+    nz, ny, nx = (s - 2 for s in g.shape)
+    ck = np.clip(np.arange(nz + 2), 1, nz)
+    cj = np.clip(np.arange(ny + 2), 1, ny)
+    ci = np.clip(np.arange(nx + 2), 1, nx)
+    np.add.at(out, (ck[:, None, None], cj[None, :, None], ci[None, None, :]), g)
+    return out
+
+
+@pytest.mark.parametrize("first,nsteps,every", [(1, 1, 1), (2, 3, 2)])
+def test_csf_vjp_matches_taped_reference_fp64(vofmod, first, nsteps, every):
+    from test_csf import sphere_F
+
+    n, sigma, dt = 12, 0.07, 1e-4
+    prm = vofmod.default_params(n=n, precision=8, dt=dt, g=(0.0, 0.0, -9.81), flags=vofmod.FLAG_CSF,
+                                sigma=sigma, rho_l=1.0, rho_g=1.0)
+    rng = np.random.default_rng(50 + first)
+    F = sphere_F(n, (0.045, 0.055, 0.05), 0.027)
+    st = {"F": F, "P": np.zeros(F.shape)}
+    for k in "UVW":
+        st[k] = rng.normal(0, 0.05, F.shape)
+    seeds = {k: rng.normal(0, 1, F.shape) for k in "FUVWP"}
+    gJ = 0.7
+    J, sc, gg = gpu_vjp(vofmod, prm, st, first, nsteps, every, gJ, seeds)
+
+    leaf = lambda x: torch.tensor(float(x), dtype=T, requires_grad=True)
+    P = ar.Prm(n, n, n, dt=dt, g=(0.0, 0.0, -9.81), rho_l=leaf(1.0), rho_g=leaf(1.0), csf=True, sigma=leaf(sigma))
+    x0 = [torch.tensor(st[k], dtype=T, requires_grad=True) for k in "FUVWP"]
+    s = tuple(x0)
+    coeffs = ar.jacobi_coeffs(P)
+    for m in range(nsteps):
+        s = ar.step(P, s, first + m, coeffs)
+    L = gJ * ar.centroid_height(P, s[0]) + sum((torch.tensor(seeds[k]) * t).sum() for k, t in zip("FUVWP", s))
+    L.backward()
+    ref = {k: t.grad.numpy() for k, t in zip("FUVWP", x0)}
+    ref["F"] = fold_ghosts(ref["F"])
+    for k in "FUVW":
+        e = rel_err(gg[k], ref[k])
+        assert e < 1e-9, f"d/d{k}0: relative error {e:.2e}"
+    pin = (slice(1, -1),) * 3
+    assert rel_err(gg["P"][pin], ref["P"][pin]) < 1e-9
+    for name, t in (("sigma", P.sigma), ("rho_l", P.rho_l), ("rho_g", P.rho_g)):
+        assert sc[name] == pytest.approx(t.grad.item(), rel=1e-9, abs=1e-30), name
+
+
+def test_csf_sigma_gradient_matches_central_differences(vofmod):
+    from test_csf import sphere_F
+
+    n, nsteps, sigma = 24, 30, 0.07
+    kw = dict(n=n, precision=8, dt=1e-4, g=(0.0, 0.0, -9.81), flags=vofmod.FLAG_CSF, rho_l=1.0, rho_g=1.0)
+    F = sphere_F(n, (0.045, 0.055, 0.05), 0.027)
+    st = {"F": F, "U": np.zeros(F.shape), "V": np.zeros(F.shape), "W": np.zeros(F.shape), "P": np.zeros(F.shape)}
+    J, sc, _ = gpu_vjp(vofmod, vofmod.default_params(sigma=sigma, **kw), st, 1, nsteps, 10, 1.0, {})
+    h = 1e-4
+    fd = (centroid(vofmod, vofmod.default_params(sigma=sigma * (1 + h), **kw), st, 1, nsteps) -
+          centroid(vofmod, vofmod.default_params(sigma=sigma * (1 - h), **kw), st, 1, nsteps)) / (2 * h * sigma)
+    print(f"dJ/dsigma: adjoint {sc['sigma']:.10e} fd {fd:.10e}", flush=True)
+    assert sc["sigma"] != 0.0
+    assert sc["sigma"] == pytest.approx(fd, rel=1e-4)
