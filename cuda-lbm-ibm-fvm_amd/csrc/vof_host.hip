@@ -353,6 +353,7 @@ struct Ensemble : EnsBase {
         VOF_HIP(hipMalloc(&e.tab, ih.size() * sizeof(vof::Inst<R>)));
         VOF_HIP(hipMemcpy(e.tab, ih.data(), ih.size() * sizeof(vof::Inst<R>), hipMemcpyHostToDevice));
         e.ninst = n;
+        e.one = ih[0];
         VOF_HIP(hipMalloc(&red_out, size_t(n) * vof::kNumReduce * sizeof(double)));
         VOF_HIP(hipMalloc(&means, size_t(n) * sizeof(double)));
         VOF_HIP(hipMalloc(&ncell, size_t(n) * sizeof(long long)));

@@ -17,7 +17,25 @@ template <class R>
 struct Ens {
     Inst<R>* tab = nullptr;  // device copy of the instance table
     int ninst = 0;
+    Inst<R> one{};           // host copy of instance 0 (the whole table when ninst == 1)
 };
+
+// Kernel argument block of every step kernel: the instance table and the
+// launch shape's work-group prefix table; for a single instance also the
+// record itself, by value, so that a work-group has it with its kernel
+// arguments instead of after two dependent scalar loads (prefix, record).
+template <class R>
+struct EnsArg {
+    const Inst<R>* tab;
+    const int* boff;
+    int ninst;
+    Inst<R> one;  // valid when ninst == 1
+};
+
+template <class R>
+inline EnsArg<R> ens_arg(const Ens<R>& e, const Geo& g) {
+    return EnsArg<R>{e.tab, g.boff, e.ninst, e.one};
+}
 
 constexpr int kNumReduce = 7;
 

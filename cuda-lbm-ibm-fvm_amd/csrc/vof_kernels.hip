@@ -28,9 +28,13 @@ namespace vof {
 // opaque to alias analysis, and a reference would be re-read after each one.
 #define VOF_PROLOGUE(R)                                                   \
     const int b_ = xcd_remap(blockIdx.x, gridDim.x);                      \
-    const int inst_ = (ninst == 1) ? 0 : find_inst(boff, ninst, b_);      \
-    const int lb = b_ - boff[inst_];                                      \
-    const Inst<R> I = tab[inst_];
+    int lb = b_, inst_ = 0;                                               \
+    Inst<R> I = ea.one;                                                   \
+    if (ea.ninst != 1) {                                                  \
+        inst_ = find_inst(ea.boff, ea.ninst, b_);                         \
+        lb = b_ - ea.boff[inst_];                                         \
+        I = ea.tab[inst_];                                                \
+    }
 
 template <class R> __device__ __forceinline__ R sh_up(R x) { return lane_up(x); }  // value of lane-1
 template <class R> __device__ __forceinline__ R sh_dn(R x) { return lane_dn(x); }  // value of lane+1
@@ -88,8 +92,8 @@ __device__ __forceinline__ R mom(const Inst<R>& I, R c, R xm, R xp, R ym, R yp, 
 // written, as in the reference).  Per plane: 10 coalesced loads (u,v,w at k+1
 // and at rows j-1, j+1 of plane k+1, F at k), 8 lane shuffles.
 template <class R>
-__global__ __launch_bounds__(256) void k_predict(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
-                                                 int ninst, int fsel, int kz) {
+__global__ __launch_bounds__(256) void k_predict(const EnsArg<R> ea,
+                                                 int fsel, int kz) {
     VOF_PROLOGUE(R)
     WMap m;
     if (!wmap<R, 1>(I, lb, kz, 1, I.nz, m)) return;
@@ -166,8 +170,8 @@ __global__ __launch_bounds__(256) void k_predict(const Inst<R>* __restrict__ tab
 // solve_p_jacobi_kernel, :418-422), formed once per step instead of in every
 // sweep; it depends only on u* and rho, so the sweeps see the same bits.
 template <class R>
-__global__ __launch_bounds__(256) void k_rhs(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
-                                             int ninst, int fsel, int kz) {
+__global__ __launch_bounds__(256) void k_rhs(const EnsArg<R> ea,
+                                             int fsel, int kz) {
     VOF_PROLOGUE(R)
     WMap m;
     if (!wmap<R, 1>(I, lb, kz, 1, I.nz, m)) return;
@@ -200,8 +204,8 @@ __global__ __launch_bounds__(256) void k_rhs(const Inst<R>* __restrict__ tab, co
 // Boundary neighbours carry a zero coefficient times the (zero) ghost p, kept
 // in the expression so the rounding sequence is the reference's.
 template <class R>
-__global__ __launch_bounds__(256) void k_jacobi(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
-                                                int ninst, int psrc, int kz) {
+__global__ __launch_bounds__(256) void k_jacobi(const EnsArg<R> ea,
+                                                int psrc, int kz) {
     VOF_PROLOGUE(R)
     WMap m;
     if (!wmap<R, 1>(I, lb, kz, 1, I.nz, m)) return;
@@ -245,6 +249,56 @@ __global__ __launch_bounds__(256) void k_jacobi(const Inst<R>* __restrict__ tab,
     }
 }
 
+// The same sweep for z-chunks of KZ planes, with every operand of the chunk
+// (p on KZ+2 planes of the column, p on the two neighbouring rows and rhs on
+// KZ planes) loaded before the first point is formed: a wave has all its
+// loads in flight at once instead of one plane's worth per memory latency.
+template <class R, int KZ>
+__global__ __launch_bounds__(256) void k_jacobi_blk(const EnsArg<R> ea, int psrc) {
+    VOF_PROLOGUE(R)
+    WMap m;
+    if (!wmap<R, 1>(I, lb, KZ, 1, I.nz, m)) return;
+    const int i = m.i, j = m.j, sx = I.sx, sy = I.sy;
+    const int P = sx * (I.ny + 2);
+    const bool out = m.own && i >= 1 && i <= I.nx && j >= 1 && j <= I.ny;
+    const int tc = m.tc, tjm = max(tc - sx, 0), tjp = min(tc + sx, P - 1);
+    const Buf<R> p(sel2(I.p, psrc), I.ntot), pt(sel2(I.p, psrc ^ 1), I.ntot), rhs(I.rhs, I.ntot);
+    const R ae = (i != I.nx) ? I.axx : R(0);
+    const R aw = (i != 1) ? I.axx : R(0);
+    const R an = (j != I.ny) ? I.ayy : R(0);
+    const R as = (j != 1) ? I.ayy : R(0);
+    const R azz = I.azz;
+    const R ap_in = -(ae + aw + an + as + azz + azz);
+    const R ap_lo = -(ae + aw + an + as + ((I.nz != 1) ? azz : R(0)) + R(0));
+    const R ap_hi = -(ae + aw + an + as + R(0) + ((I.nz != 1) ? azz : R(0)));
+    const R ap_one = -(ae + aw + an + as + R(0) + R(0));
+    const int k0 = m.k0, k1 = m.k1, nz = I.nz;
+    R pc[KZ + 2], pjm[KZ], pjp[KZ], b[KZ];  // pc[e]: plane k0-1+e; the others: plane k0+e
+#pragma unroll
+    for (int e = 0; e < KZ + 2; ++e) pc[e] = p.ld(tc, min(k0 - 1 + e, nz + 1) * sy);
+#pragma unroll
+    for (int e = 0; e < KZ; ++e) {
+        const int o = min(k0 + e, nz) * sy;
+        pjm[e] = p.ld(tjm, o);
+        pjp[e] = p.ld(tjp, o);
+        b[e] = rhs.ld(tc, o);
+    }
+#pragma unroll
+    for (int e = 0; e < KZ; ++e) {
+        const int k = k0 + e;
+        if (k > k1) break;
+        const R p_k = pc[e + 1];
+        const R pim = sh_up(p_k), pip = sh_dn(p_k);
+        if (out) {
+            const R af = (k != nz) ? azz : R(0);
+            const R ab = (k != 1) ? azz : R(0);
+            const R ap = (k != 1 && k != nz) ? ap_in : (nz == 1 ? ap_one : (k == 1 ? ap_lo : ap_hi));
+            pt.st((b[e] - ae * pip - aw * pim - an * pjp[e] - as * pjm[e] - af * pc[e + 2] - ab * pc[e]) / ap, tc,
+                  k * sy);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Two Jacobi sweeps per launch (temporal blocking; the reference runs one
 // sweep per launch, :1371-1375).  A work-group of 16 waves covers a tile of
@@ -266,8 +320,8 @@ __device__ __forceinline__ R jac_point(R b, R ae, R aw, R an, R as, R af, R ab, 
 }
 
 template <class R>
-__global__ __launch_bounds__(1024) void k_jacobi2(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
-                                                  int ninst, int psrc, int kz) {
+__global__ __launch_bounds__(1024) void k_jacobi2(const EnsArg<R> ea,
+                                                  int psrc, int kz) {
     VOF_PROLOGUE(R)
     __shared__ R ring[3][kJ2Rows][64];
     const int nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
@@ -345,8 +399,8 @@ __device__ __forceinline__ R corr(const Inst<R>& I, R st, R r0, R r1, R p0, R p1
 }
 
 template <class R>
-__global__ __launch_bounds__(256) void k_correct(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
-                                                 int ninst, int fsel, int psel, int kz) {
+__global__ __launch_bounds__(256) void k_correct(const EnsArg<R> ea,
+                                                 int fsel, int psel, int kz) {
     VOF_PROLOGUE(R)
     WMap m;
     if (!wmap<R, 1>(I, lb, kz, 1, I.nz, m)) return;
@@ -446,12 +500,12 @@ __device__ __forceinline__ R fct_stage1_scale(const Inst<R>& I, R d) {
     return d * I.dy * I.dx / I.vol;
 }
 
+// Ghost copies of interior cell (i,j,k): every ghost whose nearest interior
+// cell it is.  The whole offset is per lane (i, j, k may all differ across
+// the wave), so the buffer instruction's scalar offset stays 0.
 template <class R>
-__device__ __forceinline__ void store_F(const Inst<R>& I, const Buf<R>& Fo, int i, int j, int k, R val,
-                                        bool ghosts) {
+__device__ __forceinline__ void store_F_ghosts(const Inst<R>& I, const Buf<R>& Fo, int i, int j, int k, R val) {
     const int sx = I.sx, sy = I.sy;
-    Fo.st(val, j * sx + i, k * sy);
-    if (!ghosts) return;
     if (!(i == 1 || i == I.nx || j == 1 || j == I.ny || k == 1 || k == I.nz)) return;
     int ii[3], jj[3], kk[3], ni = 0, nj = 0, nk = 0;
     ii[ni++] = i; if (i == 1) ii[ni++] = 0; if (i == I.nx) ii[ni++] = I.nx + 1;
@@ -460,7 +514,15 @@ __device__ __forceinline__ void store_F(const Inst<R>& I, const Buf<R>& Fo, int 
     for (int a = 0; a < ni; ++a)
         for (int b = 0; b < nj; ++b)
             for (int c = 0; c < nk; ++c)
-                if (a | b | c) Fo.st(val, jj[b] * sx + ii[a], kk[c] * sy);
+                if (a | b | c) Fo.st(val, kk[c] * sy + jj[b] * sx + ii[a], 0);
+}
+
+// Store of cell (i,j,k), k wave-uniform; plus its ghost copies on the last sweep.
+template <class R>
+__device__ __forceinline__ void store_F(const Inst<R>& I, const Buf<R>& Fo, int i, int j, int k, R val,
+                                        bool ghosts) {
+    Fo.st(val, j * I.sx + i, k * I.sy);
+    if (ghosts) store_F_ghosts(I, Fo, i, j, k, val);
 }
 
 // Gas cells stay exactly zero: if F vanishes on the whole 7-cell stencil of a
@@ -472,8 +534,8 @@ __device__ __forceinline__ void store_F(const Inst<R>& I, const Buf<R>& Fo, int 
 // the wave); each wave walks kz planes.  Row ends need no care: ghost lanes
 // carry the reference's zero conventions, which decouple consecutive rows.
 template <class R>
-__global__ __launch_bounds__(256) void k_fct_x(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
-                                               int ninst, int src, int kz, int fin) {
+__global__ __launch_bounds__(256) void k_fct_x(const EnsArg<R> ea,
+                                               int src, int kz, int fin) {
     VOF_PROLOGUE(R)
     WMap m;
     if (!wmap<R, 3>(I, lb, kz, 1, I.nz, m)) return;
@@ -533,108 +595,200 @@ __global__ __launch_bounds__(256) void k_fct_x(const Inst<R>* __restrict__ tab, 
 // sweep line through a register window, producing one Fnew per step two cells
 // behind the Ftd front.  Lines are cut into chunks of ch cells with a 5-cell
 // warm-up, so that there are enough waves to cover memory latency.
+//
+// The window of the march, positioned on cell c of the line: F and V of c,
+// the low-face flux L and antidiffusive flux a of faces c, c-1, c-2, Ftd and
+// dv of cells c-1, c-2, the Zalesak ratios of c-1, c-2 and the limiter of
+// face c-2.  zr counts the trailing run of zero F values.
+template <class R>
+struct FctWin {
+    R Fc, Vc, Lc, ac, am1, am2, ftm1, ftm2, dvm1, dvm2, rpm1, rmm1, rpm2, rmm2, cm2;
+    int zr;
+};
+
+// Window on cell c = m0 - 2 from F(c-1), F(c), V(c).
+template <class R>
+__device__ __forceinline__ void fct_win_init(const Inst<R>& I, FctWin<R>& w, R Fcm, R Fc, R Vc) {
+    w.Fc = Fc;
+    w.Vc = Vc;
+    const R vdt = Vc * I.dt;
+    w.Lc = (Vc >= R(0)) ? vdt * Fcm : vdt * Fc;
+    const R Hc = (Vc <= R(0)) ? vdt * Fcm : vdt * Fc;
+    w.ac = Hc - w.Lc;
+    w.zr = (Fc == R(0)) ? ((Fcm == R(0)) ? 2 : 1) : 0;
+    w.am1 = R(0); w.am2 = R(0); w.ftm1 = R(0); w.ftm2 = R(0); w.dvm1 = R(1); w.dvm2 = R(1);
+    w.rpm1 = R(0); w.rmm1 = R(0); w.rpm2 = R(0); w.rmm2 = R(0); w.cm2 = R(0);
+}
+
+// One march step on cell c of a line of nd cells, given F and V of cell c+1:
+// forms Ftd(c), the ratios of c-1 and the limiter of face c-1; when emit,
+// returns the corrected value of cell c-2 (before the final-sweep clamp).
+// Shifts the window to c+1.
 template <class R, int DIR>
-__global__ __launch_bounds__(256) void k_fct_march(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
-                                                   int ninst, int src, int ch, int fin) {
-    VOF_PROLOGUE(R)
-    const int nd = (DIR == 1) ? I.ny : I.nz;
-    const int no = (DIR == 1) ? I.nz : I.ny;
-    const int ntr = no * I.sx;
-    const int nbt = (ntr + 255) >> 8;
-    const int tb = lb % nbt, chn = lb / nbt;
-    const int t = tb * 256 + (int)threadIdx.x;
-    if (t >= ntr) return;
-    const int o = 1 + t / I.sx;
-    const int i = t - (o - 1) * I.sx;
-    if (i < 1 || i > I.nx) return;
-    const int S = (DIR == 1) ? I.sx : I.sy;
-    const int base = (DIR == 1) ? (o * I.sy + i) : (o * I.sx + i);
-    const Buf<R> Fb(sel2(I.F, src), I.ntot), Fo(sel2(I.F, src ^ 1), I.ntot), Vb((DIR == 1) ? I.v : I.w, I.ntot);
-    // line element m of the lane's line: per-lane offset base, scalar offset m*S
-    auto F = [&](int mm) { return Fb.ld(base, mm * S); };
-    auto V = [&](int mm) { return Vb.ld(base, mm * S); };
+__device__ __forceinline__ R fct_win_step(const Inst<R>& I, FctWin<R>& w, int c, int nd, R Fcp, R Vcp,
+                                          bool emit) {
     const R kdv = (DIR == 1) ? I.dtdxdz : I.dtdxdy;
     const R qf = (DIR == 1) ? I.dx : I.dz;
     const R s4 = (DIR == 1) ? I.dy : I.dz;
     const R dt = I.dt, vol = I.vol;
+    w.zr = (Fcp == R(0)) ? w.zr + 1 : 0;
+    const R dvc = vol - kdv * (Vcp - w.Vc);
+    R Lcp, acp, ftc, cm1, out = R(0);
+    if (w.zr >= 7) {
+        // F(c-5..c+1) all zero: every value formed this step is (signed) zero.
+        Lcp = R(0); acp = R(0); ftc = R(0);
+        w.rpm1 = R(0); w.rmm1 = R(0); cm1 = R(0);
+    } else {
+        const R vdt = Vcp * dt;
+        Lcp = (Vcp >= R(0)) ? vdt * w.Fc : vdt * Fcp;
+        const R Hcp = (Vcp <= R(0)) ? vdt * w.Fc : vdt * Fcp;
+        acp = Hcp - Lcp;
+        ftc = R(0);
+        if (c >= 1 && c <= nd) {
+            R oo = (w.Fc + fct_stage1_scale<R, DIR>(I, w.Lc - Lcp)) * vol / dvc;
+            if (oo > R(1) || oo < R(0)) oo = clamp01(oo);
+            ftc = oo;
+        }
+        // Zalesak ratios of cell c-1
+        w.rpm1 = R(0); w.rmm1 = R(0);
+        if (c - 1 >= 1 && c - 1 <= nd) {
+            const R fmx = vmax(w.ftm1, vmax(w.ftm2, ftc));
+            const R fmn = vmin(w.ftm1, vmin(w.ftm2, ftc));
+            const R pp = vmax(R(0), w.am1) - vmin(R(0), w.ac);
+            if (pp > R(0)) w.rpm1 = vmin(R(1), (fmx - w.ftm1) * qf / pp);
+            const R pm = vmax(R(0), w.ac) - vmin(R(0), w.am1);
+            if (pm > R(0)) w.rmm1 = vmin(R(1), (w.ftm1 - fmn) * qf / pm);
+        }
+        // limiter on face c-1
+        cm1 = R(0);
+        if (c - 1 >= 2 && c - 1 <= nd + 1) cm1 = (w.am1 >= R(0)) ? vmin(w.rpm1, w.rmm2) : vmin(w.rpm2, w.rmm1);
+        // corrected value of cell c-2
+        if (emit) out = clamp01(w.ftm2 - ((w.am1 * cm1 - w.am2 * w.cm2) / s4) * vol / w.dvm2);
+    }
+    // shift the window
+    w.Fc = Fcp; w.Vc = Vcp; w.Lc = Lcp;
+    w.am2 = w.am1; w.am1 = w.ac; w.ac = acp;
+    w.ftm2 = w.ftm1; w.ftm1 = ftc;
+    w.dvm2 = w.dvm1; w.dvm1 = dvc;
+    w.rpm2 = w.rpm1; w.rmm2 = w.rmm1;
+    w.cm2 = cm1;
+    return out;
+}
+
+// Lane -> line mapping shared by the two march kernels: lanes along x over the
+// flattened (x, other) plane.  The wave-uniform parts (line length nd, stride
+// S between line elements, chunk index chn) are formed before any lane exits,
+// so that they stay in scalar registers; false for lanes without an interior
+// line.
+struct MarchLine {
+    int nd, S, chn, nbt;  // wave-uniform
+    int i, o, base;       // per lane
+};
+
+template <class R, int DIR>
+__device__ __forceinline__ MarchLine march_uniform(const Inst<R>& I, int lb) {
+    MarchLine L;
+    L.nd = (DIR == 1) ? I.ny : I.nz;
+    L.S = (DIR == 1) ? I.sx : I.sy;
+    const int no = (DIR == 1) ? I.nz : I.ny;
+    L.nbt = (no * I.sx + 255) >> 8;
+    L.chn = lb / L.nbt;
+    return L;
+}
+
+template <class R, int DIR>
+__device__ __forceinline__ bool march_lane(const Inst<R>& I, int lb, MarchLine& L) {
+    const int no = (DIR == 1) ? I.nz : I.ny;
+    const int t = (lb - L.chn * L.nbt) * 256 + (int)threadIdx.x;
+    if (t >= no * I.sx) return false;
+    L.o = 1 + t / I.sx;
+    L.i = t - (L.o - 1) * I.sx;
+    if (L.i < 1 || L.i > I.nx) return false;
+    L.base = (DIR == 1) ? (L.o * I.sy + L.i) : (L.o * I.sx + L.i);
+    return true;
+}
+
+// Store of line element mm (wave-uniform): per-lane line base + scalar mm*S,
+// as the loads.
+template <class R, int DIR>
+__device__ __forceinline__ void march_put(const Inst<R>& I, const Buf<R>& Fo, const MarchLine& L, int S, int mm,
+                                          R val, bool gh) {
+    Fo.st(val, L.base, mm * S);
+    if (!gh) return;
+    if (DIR == 1) store_F_ghosts(I, Fo, L.i, mm, L.o, val);
+    else store_F_ghosts(I, Fo, L.i, L.o, mm, val);
+}
+
+// Any chunk length: the line is streamed through the window with F and V
+// fetched three cells ahead.
+template <class R, int DIR>
+__global__ __launch_bounds__(256) void k_fct_march(const EnsArg<R> ea,
+                                                   int src, int ch, int fin) {
+    VOF_PROLOGUE(R)
+    MarchLine L = march_uniform<R, DIR>(I, lb);
+    const int S = L.S, nd = L.nd, chn = L.chn;
+    if (!march_lane<R, DIR>(I, lb, L)) return;
+    const Buf<R> Fb(sel2(I.F, src), I.ntot), Fo(sel2(I.F, src ^ 1), I.ntot), Vb((DIR == 1) ? I.v : I.w, I.ntot);
+    auto cl = [nd](int mm) { return mm < 0 ? 0 : (mm > nd + 1 ? nd + 1 : mm); };
+    auto F = [&](int mm) { return Fb.ld(L.base, cl(mm) * S); };
+    auto V = [&](int mm) { return Vb.ld(L.base, cl(mm) * S); };
     const bool gh = fin != 0;
     const int m0 = 1 + chn * ch;
     const int m1 = min(nd, m0 + ch - 1);
-    auto cl = [nd](int mm) { return mm < 0 ? 0 : (mm > nd + 1 ? nd + 1 : mm); };
-    auto put = [&](int mm, R val) {
-        if (DIR == 1) store_F(I, Fo, i, mm, o, val, gh);
-        else store_F(I, Fo, i, o, mm, val, gh);
-    };
-
     int c = m0 - 2;
-    const R Fcm = F(cl(c - 1));
-    R Fc = F(cl(c));
-    R Vc = V(cl(c));
-    R Lc, ac;
-    {
-        const R vdt = Vc * dt;
-        Lc = (Vc >= R(0)) ? vdt * Fcm : vdt * Fc;
-        const R Hc = (Vc <= R(0)) ? vdt * Fcm : vdt * Fc;
-        ac = Hc - Lc;
-    }
-    int zr = (Fc == R(0)) ? ((Fcm == R(0)) ? 2 : 1) : 0;
-    R am1 = R(0), am2 = R(0), ftm1 = R(0), ftm2 = R(0), dvm1 = R(1), dvm2 = R(1);
-    R rpm1 = R(0), rmm1 = R(0), rpm2 = R(0), rmm2 = R(0), cm2 = R(0);
-    R Fn = F(cl(c + 1));
-    R Vn = V(cl(c + 1));
-    R Fn2 = F(cl(c + 2));
-    R Vn2 = V(cl(c + 2));
+    FctWin<R> w;
+    fct_win_init(I, w, F(c - 1), F(c), V(c));
+    R Fn = F(c + 1), Vn = V(c + 1), Fn2 = F(c + 2), Vn2 = V(c + 2);
     for (; c <= m1 + 2; ++c) {
         const R Fcp = Fn, Vcp = Vn;
         Fn = Fn2;
         Vn = Vn2;
-        Fn2 = F(cl(c + 3));
-        Vn2 = V(cl(c + 3));
-        zr = (Fcp == R(0)) ? zr + 1 : 0;
-        const R dvc = vol - kdv * (Vcp - Vc);
-        R Lcp, acp, ftc, cm1;
-        if (zr >= 7) {
-            // F(c-5..c+1) all zero: every value formed this step is (signed) zero.
-            Lcp = R(0); acp = R(0); ftc = R(0);
-            rpm1 = R(0); rmm1 = R(0); cm1 = R(0);
-            if (c - 2 >= m0) put(c - 2, R(0));
-        } else {
-            const R vdt = Vcp * dt;
-            Lcp = (Vcp >= R(0)) ? vdt * Fc : vdt * Fcp;
-            const R Hcp = (Vcp <= R(0)) ? vdt * Fc : vdt * Fcp;
-            acp = Hcp - Lcp;
-            ftc = R(0);
-            if (c >= 1 && c <= nd) {
-                R oo = (Fc + fct_stage1_scale<R, DIR>(I, Lc - Lcp)) * vol / dvc;
-                if (oo > R(1) || oo < R(0)) oo = clamp01(oo);
-                ftc = oo;
-            }
-            // Zalesak ratios of cell c-1
-            rpm1 = R(0); rmm1 = R(0);
-            if (c - 1 >= 1 && c - 1 <= nd) {
-                const R fmx = vmax(ftm1, vmax(ftm2, ftc));
-                const R fmn = vmin(ftm1, vmin(ftm2, ftc));
-                const R pp = vmax(R(0), am1) - vmin(R(0), ac);
-                if (pp > R(0)) rpm1 = vmin(R(1), (fmx - ftm1) * qf / pp);
-                const R pm = vmax(R(0), ac) - vmin(R(0), am1);
-                if (pm > R(0)) rmm1 = vmin(R(1), (ftm1 - fmn) * qf / pm);
-            }
-            // limiter on face c-1
-            cm1 = R(0);
-            if (c - 1 >= 2 && c - 1 <= nd + 1) cm1 = (am1 >= R(0)) ? vmin(rpm1, rmm2) : vmin(rpm2, rmm1);
-            // corrected value of cell c-2
-            if (c - 2 >= m0) {
-                const R fn = clamp01(ftm2 - ((am1 * cm1 - am2 * cm2) / s4) * vol / dvm2);
-                put(c - 2, gh ? clamp01(fn) : fn);
-            }
-        }
-        // shift the window
-        Fc = Fcp; Vc = Vcp; Lc = Lcp;
-        am2 = am1; am1 = ac; ac = acp;
-        ftm2 = ftm1; ftm1 = ftc;
-        dvm2 = dvm1; dvm1 = dvc;
-        rpm2 = rpm1; rmm2 = rmm1;
-        cm2 = cm1;
+        Fn2 = F(c + 3);
+        Vn2 = V(c + 3);
+        const bool emit = c - 2 >= m0;
+        const R fn = fct_win_step<R, DIR>(I, w, c, nd, Fcp, Vcp, emit);
+        if (emit) march_put<R, DIR>(I, Fo, L, S, c - 2, gh ? clamp01(fn) : fn, gh);
+    }
+}
+
+// Chunks of CH cells: the whole window of a chunk (F on CH+6 cells, V on CH+5)
+// is loaded into registers before the march, so every load of a wave is in
+// flight at once.  A wave whose F window is zero on every lane stores zeros
+// without loading V (all values of the chunk are +0, as in the march).
+template <class R, int DIR, int CH>
+__global__ __launch_bounds__(256) void k_fct_line(const EnsArg<R> ea,
+                                                  int src, int fin) {
+    VOF_PROLOGUE(R)
+    MarchLine L = march_uniform<R, DIR>(I, lb);
+    const int S = L.S, nd = L.nd, chn = L.chn;
+    if (!march_lane<R, DIR>(I, lb, L)) return;
+    const Buf<R> Fb(sel2(I.F, src), I.ntot), Fo(sel2(I.F, src ^ 1), I.ntot), Vb((DIR == 1) ? I.v : I.w, I.ntot);
+    auto cl = [nd](int mm) { return mm < 0 ? 0 : (mm > nd + 1 ? nd + 1 : mm); };
+    const bool gh = fin != 0;
+    const int m0 = 1 + chn * CH;
+    const int m1 = min(nd, m0 + CH - 1);
+    constexpr int NF = CH + 6, NV = CH + 5;
+    R fw[NF], vw[NV];  // F(m0-3+e), V(m0-2+e)
+#pragma unroll
+    for (int e = 0; e < NF; ++e) fw[e] = Fb.ld(L.base, cl(m0 - 3 + e) * S);
+    bool gas = true;
+#pragma unroll
+    for (int e = 0; e < NF; ++e) gas = gas && (fw[e] == R(0));
+    if (__all(gas)) {
+#pragma unroll
+        for (int e = 0; e < CH; ++e)
+            if (m0 + e <= m1) march_put<R, DIR>(I, Fo, L, S, m0 + e, R(0), gh);
+        return;
+    }
+#pragma unroll
+    for (int e = 0; e < NV; ++e) vw[e] = Vb.ld(L.base, cl(m0 - 2 + e) * S);
+    FctWin<R> w;
+    fct_win_init(I, w, fw[0], fw[1], vw[0]);
+#pragma unroll
+    for (int s = 0; s < CH + 4; ++s) {  // cell c = m0 - 2 + s
+        const int c = m0 - 2 + s;
+        const R fn = fct_win_step<R, DIR>(I, w, c, nd, fw[s + 2], vw[s + 1], s >= 4);
+        if (s >= 4 && c - 2 <= m1) march_put<R, DIR>(I, Fo, L, S, c - 2, gh ? clamp01(fn) : fn, gh);
     }
 }
 
@@ -644,8 +798,8 @@ __global__ __launch_bounds__(256) void k_fct_march(const Inst<R>* __restrict__ t
 // written once per ghost: k faces (full planes), then j faces and i faces of
 // the interior slabs.
 template <class R>
-__global__ __launch_bounds__(256) void k_ghost_F(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
-                                                 int ninst, int fsel) {
+__global__ __launch_bounds__(256) void k_ghost_F(const EnsArg<R> ea,
+                                                 int fsel) {
     VOF_PROLOGUE(R)
     const int nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
     const int nk = 2 * sx * (ny + 2), nj = 2 * sx * nz, ni = 2 * ny * nz;
@@ -674,8 +828,8 @@ __global__ __launch_bounds__(256) void k_ghost_F(const Inst<R>* __restrict__ tab
 // post_process_f (:502-507) on interior cells, for stage-by-stage tests; in a
 // step it is folded into the store of the last FCT sweep.
 template <class R>
-__global__ __launch_bounds__(256) void k_clamp_F(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
-                                                 int ninst, int fsel, int kz) {
+__global__ __launch_bounds__(256) void k_clamp_F(const EnsArg<R> ea,
+                                                 int fsel, int kz) {
     VOF_PROLOGUE(R)
     WMap m;
     if (!wmap<R, 1>(I, lb, kz, 1, I.nz, m)) return;
@@ -694,8 +848,8 @@ __global__ __launch_bounds__(256) void k_clamp_F(const Inst<R>* __restrict__ tab
 // j >= 2, k >= 2, as the predictor writes them).  They run after the
 // predictor and before the rhs.
 template <class R>
-__global__ __launch_bounds__(256) void k_csf_smooth(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
-                                                    int ninst, int fsel, int kz) {
+__global__ __launch_bounds__(256) void k_csf_smooth(const EnsArg<R> ea,
+                                                    int fsel, int kz) {
     VOF_PROLOGUE(R)
     WMap m;
     if (!wmap<R, 1>(I, lb, kz, 1, I.nz, m)) return;
@@ -708,8 +862,8 @@ __global__ __launch_bounds__(256) void k_csf_smooth(const Inst<R>* __restrict__ 
 }
 
 template <class R>
-__global__ __launch_bounds__(256) void k_curvature(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
-                                                   int ninst, int kz) {
+__global__ __launch_bounds__(256) void k_curvature(const EnsArg<R> ea,
+                                                   int kz) {
     VOF_PROLOGUE(R)
     WMap m;
     if (!wmap<R, 1>(I, lb, kz, 1, I.nz, m)) return;
@@ -718,8 +872,8 @@ __global__ __launch_bounds__(256) void k_curvature(const Inst<R>* __restrict__ t
 }
 
 template <class R>
-__global__ __launch_bounds__(256) void k_csf(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
-                                             int ninst, int fsel, int kz) {
+__global__ __launch_bounds__(256) void k_csf(const EnsArg<R> ea,
+                                             int fsel, int kz) {
     VOF_PROLOGUE(R)
     WMap m;
     if (!wmap<R, 1>(I, lb, kz, 1, I.nz, m)) return;
@@ -738,8 +892,8 @@ __global__ __launch_bounds__(256) void k_csf(const Inst<R>* __restrict__ tab, co
 // Mean-pressure removal (extension; no reference counterpart): subtract the
 // FP64 interior mean, stored by the reduction, from interior p.
 template <class R>
-__global__ __launch_bounds__(256) void k_sub_mean(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
-                                                  int ninst, int psel, const double* __restrict__ means, int kz) {
+__global__ __launch_bounds__(256) void k_sub_mean(const EnsArg<R> ea,
+                                                  int psel, const double* __restrict__ means, int kz) {
     VOF_PROLOGUE(R)
     WMap m;
     if (!wmap<R, 1>(I, lb, kz, 1, I.nz, m)) return;
@@ -774,13 +928,17 @@ __device__ __forceinline__ double wave_max(double x) {
 }
 
 template <class R>
-__global__ __launch_bounds__(256) void k_reduce1(const Inst<R>* __restrict__ tab, const int* __restrict__ boff,
-                                                 int ninst, int fsel, int psel, int kz, int mode,
+__global__ __launch_bounds__(256) void k_reduce1(const EnsArg<R> ea,
+                                                 int fsel, int psel, int kz, int mode,
                                                  double* __restrict__ part) {
     const int b_ = blockIdx.x;
-    const int inst_ = (ninst == 1) ? 0 : find_inst(boff, ninst, b_);
-    const int lb = b_ - boff[inst_];
-    const Inst<R>& I = tab[inst_];
+    int lb = b_;
+    Inst<R> I = ea.one;
+    if (ea.ninst != 1) {
+        const int inst_ = find_inst(ea.boff, ea.ninst, b_);
+        lb = b_ - ea.boff[inst_];
+        I = ea.tab[inst_];
+    }
     double s[NRED] = {0.0, 0.0, 0.0, 0.0, 1e300, -1e300, 0.0};
     WMap m;
     if (wmap<R, 0>(I, lb, kz, 0, I.nz + 1, m) && m.own) {
@@ -880,57 +1038,74 @@ void launch_copy(const void* a, void* b, long n16, hipStream_t st) {
 // Host launchers.
 template <class R>
 void launch_predict(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st) {
-    hipLaunchKernelGGL(k_predict<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, fsel, g.kz);
+    hipLaunchKernelGGL(k_predict<R>, dim3(g.nblk), dim3(256), 0, st, ens_arg(e, g), fsel, g.kz);
 }
 template <class R>
 void launch_rhs(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st) {
-    hipLaunchKernelGGL(k_rhs<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, fsel, g.kz);
+    hipLaunchKernelGGL(k_rhs<R>, dim3(g.nblk), dim3(256), 0, st, ens_arg(e, g), fsel, g.kz);
 }
 template <class R>
 void launch_jacobi(const Ens<R>& e, const Geo& g, int psrc, hipStream_t st) {
-    hipLaunchKernelGGL(k_jacobi<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, psrc, g.kz);
+    const dim3 grid(g.nblk), blk(256);
+    switch (g.kz) {  // chunks of 4, 8 or 16 planes: operands preloaded
+        case 4: hipLaunchKernelGGL((k_jacobi_blk<R, 4>), grid, blk, 0, st, ens_arg(e, g), psrc); break;
+        case 8: hipLaunchKernelGGL((k_jacobi_blk<R, 8>), grid, blk, 0, st, ens_arg(e, g), psrc); break;
+        case 16: hipLaunchKernelGGL((k_jacobi_blk<R, 16>), grid, blk, 0, st, ens_arg(e, g), psrc); break;
+        default: hipLaunchKernelGGL(k_jacobi<R>, grid, blk, 0, st, ens_arg(e, g), psrc, g.kz);
+    }
 }
 template <class R>
 void launch_jacobi2(const Ens<R>& e, const Geo& g, int psrc, hipStream_t st) {
-    hipLaunchKernelGGL(k_jacobi2<R>, dim3(g.nblk), dim3(1024), 0, st, e.tab, g.boff, e.ninst, psrc, g.kz);
+    hipLaunchKernelGGL(k_jacobi2<R>, dim3(g.nblk), dim3(1024), 0, st, ens_arg(e, g), psrc, g.kz);
 }
 template <class R>
 void launch_correct(const Ens<R>& e, const Geo& g, int fsel, int psel, hipStream_t st) {
-    hipLaunchKernelGGL(k_correct<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, fsel, psel, g.kz);
+    hipLaunchKernelGGL(k_correct<R>, dim3(g.nblk), dim3(256), 0, st, ens_arg(e, g), fsel, psel, g.kz);
+}
+// Chunks of 8, 16 or 32 cells use the register-blocked kernel; any other
+// length the streaming one.
+template <class R, int DIR>
+void launch_march(const Ens<R>& e, const Geo& g, int src, int fin, hipStream_t st) {
+    const dim3 grid(g.nblk), blk(256);
+    switch (g.kz) {
+        case 8: hipLaunchKernelGGL((k_fct_line<R, DIR, 8>), grid, blk, 0, st, ens_arg(e, g), src, fin); break;
+        case 16: hipLaunchKernelGGL((k_fct_line<R, DIR, 16>), grid, blk, 0, st, ens_arg(e, g), src, fin); break;
+        case 32: hipLaunchKernelGGL((k_fct_line<R, DIR, 32>), grid, blk, 0, st, ens_arg(e, g), src, fin); break;
+        default:
+            hipLaunchKernelGGL((k_fct_march<R, DIR>), grid, blk, 0, st, ens_arg(e, g), src, g.kz, fin);
+    }
 }
 template <class R>
 void launch_fct(const Ens<R>& e, const Geo& g, int dir, int src, int fin, hipStream_t st) {
     if (dir == 0)
-        hipLaunchKernelGGL(k_fct_x<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, src, g.kz, fin);
+        hipLaunchKernelGGL(k_fct_x<R>, dim3(g.nblk), dim3(256), 0, st, ens_arg(e, g), src, g.kz, fin);
     else if (dir == 1)
-        hipLaunchKernelGGL((k_fct_march<R, 1>), dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, src, g.kz,
-                           fin);
+        launch_march<R, 1>(e, g, src, fin, st);
     else
-        hipLaunchKernelGGL((k_fct_march<R, 2>), dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, src, g.kz,
-                           fin);
+        launch_march<R, 2>(e, g, src, fin, st);
 }
 template <class R>
 void launch_clamp_F(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st) {
-    hipLaunchKernelGGL(k_clamp_F<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, fsel, g.kz);
+    hipLaunchKernelGGL(k_clamp_F<R>, dim3(g.nblk), dim3(256), 0, st, ens_arg(e, g), fsel, g.kz);
 }
 template <class R>
 void launch_ghost_F(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st) {
-    hipLaunchKernelGGL(k_ghost_F<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, fsel);
+    hipLaunchKernelGGL(k_ghost_F<R>, dim3(g.nblk), dim3(256), 0, st, ens_arg(e, g), fsel);
 }
 template <class R>
 void launch_csf(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st) {
-    hipLaunchKernelGGL(k_csf_smooth<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, fsel, g.kz);
-    hipLaunchKernelGGL(k_curvature<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, g.kz);
-    hipLaunchKernelGGL(k_csf<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, fsel, g.kz);
+    hipLaunchKernelGGL(k_csf_smooth<R>, dim3(g.nblk), dim3(256), 0, st, ens_arg(e, g), fsel, g.kz);
+    hipLaunchKernelGGL(k_curvature<R>, dim3(g.nblk), dim3(256), 0, st, ens_arg(e, g), g.kz);
+    hipLaunchKernelGGL(k_csf<R>, dim3(g.nblk), dim3(256), 0, st, ens_arg(e, g), fsel, g.kz);
 }
 template <class R>
 void launch_sub_mean(const Ens<R>& e, const Geo& g, int psel, const double* means, hipStream_t st) {
-    hipLaunchKernelGGL(k_sub_mean<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, psel, means, g.kz);
+    hipLaunchKernelGGL(k_sub_mean<R>, dim3(g.nblk), dim3(256), 0, st, ens_arg(e, g), psel, means, g.kz);
 }
 template <class R>
 void launch_reduce(const Ens<R>& e, const Geo& g, int fsel, int psel, int mode, double* part,
                    const long long* ncell, double* out, double* means, hipStream_t st) {
-    hipLaunchKernelGGL(k_reduce1<R>, dim3(g.nblk), dim3(256), 0, st, e.tab, g.boff, e.ninst, fsel, psel, g.kz, mode,
+    hipLaunchKernelGGL(k_reduce1<R>, dim3(g.nblk), dim3(256), 0, st, ens_arg(e, g), fsel, psel, g.kz, mode,
                        part);
     hipLaunchKernelGGL(k_reduce2, dim3(e.ninst), dim3(256), 0, st, g.boff, (const double*)part, ncell, out, means);
 }

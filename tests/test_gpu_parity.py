@@ -137,6 +137,46 @@ def run_steps_pair(vofmod, orc, steps, first=1, graph=False, ic=1, box=None, **k
     return s, o
 
 
+@pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("shape", [3, 4, 8, 16, 32])
+def test_launch_shapes_match_oracle(vofmod, orc, prec, shape):
+    """Every kernel variant behind the launch shapes: the Jacobi z-chunk
+    (3: streaming march; 4, 8, 16: operands preloaded) and the y/z FCT chunk
+    (3: streaming march; 8, 16, 32: register-blocked lines), on a grid whose
+    sizes leave partial chunks and with a large CFL on a random interface."""
+    s, o = make_pair(vofmod, orc, nx=37, ny=45, nz=27, precision=prec, dt=2e-3)
+    s.set_tuning(kz_jacobi=min(shape, 16), fct_chunk=shape, jacobi_tblock=1)
+    rng = np.random.default_rng(SEED + 13 + shape)
+    st = random_state(s.shape, rng, s.dtype, vel=1.5)
+    load_state(s, o, st)
+    s.debug_stage(vofmod.STAGE_PREDICT)
+    s.debug_stage(vofmod.STAGE_RHS)
+    for name in ("nu_rho", "advect_u", "advect_v", "advect_w"):
+        o.stage(name)
+    for it in range(2):
+        s.debug_stage(vofmod.STAGE_JACOBI)
+        o.stage("jacobi")
+        assert_same(s.download("P"), o.get("P"), f"P sweep {it}")
+    for gs, on in ((vofmod.STAGE_FCT_Y, "fct_y"), (vofmod.STAGE_FCT_Z, "fct_z")):
+        load_state(s, o, st)
+        s.debug_stage(gs)
+        o.stage(on)
+        assert_same(interior(s.download("F")), interior(o.get("F")), on)
+    s.close()
+    o.close()
+    # whole steps (the last sweep of each step also writes the F ghosts)
+    s, o = make_pair(vofmod, orc, nx=37, ny=45, nz=27, precision=prec, dt=2e-3)
+    s.set_tuning(kz_jacobi=min(shape, 16), fct_chunk=shape)
+    load_state(s, o, st)
+    s.step(1, 3)
+    o.step(1, 3)
+    for k in ("F", "U", "V", "W"):
+        assert_same(s.download(k), o.get(k), f"{k} after 3 steps")
+    assert_same(interior(s.download("P")), interior(o.get("P")), "P after 3 steps")
+    s.close()
+    o.close()
+
+
 def test_jacobi2_tiles_and_chunks(vofmod, orc):
     """Two-sweep launches on a grid with several x tiles, row bands and z chunks."""
     s, o = make_pair(vofmod, orc, nx=131, ny=33, nz=21, precision=4, dt=1e-3)
