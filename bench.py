@@ -21,6 +21,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "cuda-lbm-ibm-fvm_amd"))
 
@@ -58,6 +60,9 @@ def parse():
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="run one plain copy of this many bytes first (byte-counter calibration)")
     ap.add_argument("--jacobi-tblock", type=int, default=0)
+    ap.add_argument("--study", choices=["precision"], default=None,
+                    help="instead of the benchmark line, print the FP32-vs-FP64 study of the 200^3 dam break "
+                         "(1000 steps) and the 384^3 ripple layer (500 steps)")
     return ap.parse_args()
 
 
@@ -81,26 +86,7 @@ def make_solver(vof, n, prec):
     return s
 
 
-SEED = 20261203
-
-
-def ensemble_config(vof, ninst=64, seed=SEED):
-    """64-instance dam-break ensemble: instance i draws, from seed+i, a cubic grid
-    N in {32,48,64,80,96}, liquid block fractions of the 0.1 m edge in
-    U[0.2,0.6] (x, y) and U[0.3,0.8] (z), gas density U[25,100], sigma
-    U[0.003,0.02]; liquid 1000 kg/m^3, g = (0,0,-9.81), dt = 4e-6 s."""
-    import numpy as np
-
-    prms, boxes = [], []
-    for i in range(ninst):
-        r = np.random.default_rng(seed + i)
-        n = int(r.choice([32, 48, 64, 80, 96]))
-        fx, fy, fz = r.uniform(0.2, 0.6), r.uniform(0.2, 0.6), r.uniform(0.3, 0.8)
-        rho_g = float(r.uniform(25.0, 100.0))
-        sigma = float(r.uniform(0.003, 0.02))
-        prms.append(vof.default_params(n=n, g=(0.0, 0.0, -9.81), rho_g=rho_g, sigma=sigma))
-        boxes.append((0.1 * fx, 0.1 * fy, 0.1 * fz))
-    return prms, boxes
+from vof_cases import SEED, ensemble_config, field_errors, ripple_F  # noqa: E402
 
 
 def batched_leg(vof, steps=100, warmup=10):
@@ -211,6 +197,61 @@ def adjoint_leg(vof, n=96, steps=400, every=20, seed=SEED):
     return out
 
 
+def precision_study(vof):
+    """FP32 fields (FP64 sums) against an all-FP64 run of the same case, both on
+    the GPU: per-field errors after the last step (p with its interior mean
+    removed, as p is defined up to a constant), the Jacobi residual and liquid
+    volume every `every` steps, and the graph-replayed step rate of each
+    precision with the step's algorithmic bytes (236 B/cell FP32, x2 FP64)
+    against the 8 TB/s HBM peak."""
+    cases = [
+        ("dam_break_200", "dam-break 200^3, box IC x<0.04 y<0.04 z<0.06 m, g=(0,0,-9.81), dt=4e-6 s", 200, 1000,
+         50, "box"),
+        ("ripple_384", "liquid layer 0.06 m deep at 384^3, free surface = 4 plane waves of up to 2 cells "
+         "(wavelengths U[8,32] cells, seed 20261203), g=(0,0,-9.81), dt=4e-6 s", 384, 500, 25, "ripple"),
+    ]
+    out = {"study": "precision", "cases": {}}
+    for key, desc, n, steps, every, ic in cases:
+        runs = {}
+        for prec in (8, 4):
+            s = vof.Solver(vof.default_params(n=n, precision=prec, g=(0.0, 0.0, -9.81)))
+            if ic == "box":
+                s.init_box(0.04, 0.04, 0.06)
+            else:
+                s.upload("F", ripple_F(n, dtype=s.dtype))
+            s.autotune(1, 3, None)  # trials run on a snapshot; the state is restored
+            hist = [dict(step=0, **{k: s.diagnostics()[k] for k in ("liquid_volume", "residual_l2")})]
+            for c in range(steps // every):
+                s.step(1 + c * every, every, graph=True)
+                d = s.diagnostics()
+                hist.append(dict(step=(c + 1) * every, liquid_volume=d["liquid_volume"],
+                                 residual_l2=d["residual_l2"], nonfinite=d["nonfinite"]))
+            fields = {k: s.download(k) for k in ("F", "U", "V", "W", "P")}
+            ms = s.time_steps(1 + steps, 10, 100, graph=True) / 100
+            print(f"precision study: {key} {'FP64' if prec == 8 else 'FP32'} done", file=sys.stderr, flush=True)
+            s.close()
+            esz = 2 if prec == 8 else 1
+            runs[prec] = dict(history=hist, fields=fields, ms_per_step=ms, cell_updates_per_s=n ** 3 / (ms * 1e-3),
+                              step_hbm_frac=236 * esz * n ** 3 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS)
+        f64, f32 = runs[8]["fields"], runs[4]["fields"]
+        errs = {}
+        for k in ("F", "U", "V", "W"):
+            errs[k] = field_errors(f32[k], f64[k])
+        p32 = f32["P"][1:-1, 1:-1, 1:-1].astype(np.float64)
+        p64 = f64["P"][1:-1, 1:-1, 1:-1]
+        errs["P_mean_removed"] = field_errors(p32 - p32.mean(), p64 - p64.mean(), interior=False)
+        v0 = runs[8]["history"][0]["liquid_volume"]
+        out["cases"][key] = {
+            "workload": desc, "steps": steps,
+            "fp32_vs_fp64": errs,
+            "liquid_volume_drift": {str(prec): runs[prec]["history"][-1]["liquid_volume"] / v0 - 1.0 for prec in (4, 8)},
+            "history": {str(prec): runs[prec]["history"] for prec in (4, 8)},
+            "rate": {str(prec): {k: runs[prec][k] for k in ("ms_per_step", "cell_updates_per_s", "step_hbm_frac")}
+                     for prec in (4, 8)},
+        }
+    return out
+
+
 def cpu_baseline(n, prec, budget_s):
     """Time the oracle on the same workload: one warm-up step, then as many
     steps as fit the budget (at least one)."""
@@ -249,6 +290,9 @@ def main():
     import vof
 
     vof.load_library()
+    if a.study == "precision":
+        print(json.dumps(precision_study(vof)), flush=True)
+        return
     if a.calibrate_copy:
         vof.copy_bandwidth(a.calibrate_copy, 1)
     n, prec = a.n, a.precision
