@@ -654,6 +654,68 @@ __global__ __launch_bounds__(256) void k_jacobi_adj(const Inst<R>* __restrict__ 
     arhs[c] = first ? acc : arhs[c] + acc;
 }
 
+// The same reverse sweep for every launch after a step's first (the input is
+// h): a column march.  Lanes cover the flattened (x,y) plane in overlapped
+// waves that store their middle 62 lanes; z neighbours are carried in
+// registers, x neighbours come from the adjacent lanes (the neighbour terms
+// are summed in the order above), and the division by ap is a multiplication
+// by 1/ap, formed once per lane for the three kinds of plane (k = 1, interior,
+// k = nz).  The adjoint is compared with its references to a tolerance, not
+// bit for bit, so the reciprocal is allowed here.
+// acc: 0 leaves rhs_bar alone, 1 adds the input h, 2 adds the input and the
+// output h (the next launch's input, which then adds nothing): rhs_bar is read
+// and written once per pair of sweeps.
+template <class R>
+__global__ __launch_bounds__(256) void k_jacobi_adj_col(const Inst<R>* __restrict__ tab,
+                                                        const double* __restrict__ in, double* __restrict__ out,
+                                                        double* __restrict__ arhs, int out_g, int acc, int kz) {
+    const Inst<R> I = tab[0];
+    constexpr int H = 1, S = 64 - 2 * H;
+    const int nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
+    const int P = sx * (ny + 2);
+    const int wpp = (P + S - 1) / S, bpp = (wpp + 3) >> 2;
+    const int zb = int(blockIdx.x) / bpp;
+    const int wv = (int(blockIdx.x) - zb * bpp) * 4 + (int(threadIdx.x) >> 6);
+    if (wv >= wpp) return;  // wave-uniform
+    const int lane = int(threadIdx.x) & 63;
+    const int t = wv * S - H + lane;
+    const int tc = min(max(t, 0), P - 1);
+    const int j = tc / sx, i = tc - j * sx;
+    const bool cell = lane >= H && lane < 64 - H && t == tc && i >= 1 && i <= nx && j >= 1 && j <= ny;
+    const int tjm = max(tc - sx, 0), tjp = min(tc + sx, P - 1);
+    const int k0 = 1 + zb * kz, k1 = min(nz, k0 + kz - 1);
+    const double ax = I.axx, ay = I.ayy, az = I.azz;
+    const double r_lo = 1.0 / jac_ap(I, i, j, 1), r_hi = 1.0 / jac_ap(I, i, j, nz);
+    const double r_in = (nz > 2) ? 1.0 / jac_ap(I, i, j, 2) : 0.0;
+    const Buf<double> hb(in, I.ntot);
+    int o = k0 * sy;
+    double hkm = hb.ld(tc, o - sy), hk = hb.ld(tc, o);
+    double n_kp = hb.ld(tc, o + sy), n_jm = hb.ld(tjm, o), n_jp = hb.ld(tjp, o);
+    for (int k = k0; k <= k1; ++k, o += sy) {
+        const double hkp = n_kp, hjm = n_jm, hjp = n_jp;
+        const int o2 = min(o + sy, nz * sy);
+        n_kp = hb.ld(tc, o2 + sy);
+        n_jm = hb.ld(tjm, o2);
+        n_jp = hb.ld(tjp, o2);
+        const double hw = lane_up(hk), he = lane_dn(hk);
+        if (cell) {
+            double sm = 0.0;
+            if (i < nx) sm -= ax * he;
+            if (i > 1) sm -= ax * hw;
+            if (j < ny) sm -= ay * hjp;
+            if (j > 1) sm -= ay * hjm;
+            if (k < nz) sm -= az * hkp;
+            if (k > 1) sm -= az * hkm;
+            const int c = o + tc;
+            const double ho = sm * ((k == 1) ? r_lo : ((k == nz) ? r_hi : r_in));
+            out[c] = out_g ? sm : ho;
+            if (acc) arhs[c] += (acc == 2) ? hk + ho : hk;
+        }
+        hkm = hk;
+        hk = hkp;
+    }
+}
+
 // Transpose of rhs = rho/dt * div(u*) on interior cells: each interior face
 // i >= 2 (j >= 2, k >= 2) gathers from its two cells; arho += arhs div / dt.
 template <class R>
@@ -1013,18 +1075,19 @@ void adj_centroid(const Inst<R>* tab, const Inst<R>& h, double J, double sumF, d
 template <class R>
 void adj_fct(const Inst<R>* tab, const Inst<R>& h, int dir, const R* F, const R* V, const double* aFo,
              double* aFi, double* aV, int fin, hipStream_t st) {
-    if (dir == 0) {  // lanes along x; z chunk of 8 planes, fewer while under 1024 work-groups
+    if (dir == 0) {  // lanes along x; z chunk of 8 planes, fewer while under 2048 work-groups
         constexpr int S = 64 - 2 * kAdjXHalo;
         const int P = h.sx * (h.ny + 2);
         const int bpp = ((P + S - 1) / S + 3) / 4;
         int kz = 8;
-        while (kz > 1 && bpp * ((h.nz + kz - 1) / kz) < 1024) kz /= 2;
+        while (kz > 1 && bpp * ((h.nz + kz - 1) / kz) < 2048) kz /= 2;
         hipLaunchKernelGGL(k_fct_adj_x<R>, dim3(bpp * ((h.nz + kz - 1) / kz)), dim3(256), 0, st, tab, F, V, aFo, aFi,
                            aV, kz, fin);
         return;
     }
     // y and z: marches with lanes along x; chunk length 16 cells, 8 when that
-    // leaves too few work-groups to fill the chip
+    // leaves too few work-groups to fill the chip (shorter chunks cost more in
+    // the 12 warm-up positions each chunk recomputes than they gain in waves)
     const int n = (dir == 1) ? h.ny : h.nz;
     const long lines = (dir == 1) ? long(h.nx) * h.nz : long(h.nx) * h.ny;
     const int nbt = adj_blocks(lines);
@@ -1043,8 +1106,15 @@ void adj_correct(const Inst<R>* tab, const Inst<R>& h, const R* F, const R* p, c
 }
 template <class R>
 void adj_jacobi(const Inst<R>* tab, const Inst<R>& h, const double* in, double* out, double* arhs, int ns, int in_g,
-                int out_g, int first, hipStream_t st) {
-    if (ns == 2)
+                int out_g, int first, int acc, hipStream_t st) {
+    if (ns == 1 && !in_g && !first) {
+        // column march; z chunk of 8 planes, fewer while under 2048 work-groups
+        const int bpp = ((h.sx * (h.ny + 2) + 61) / 62 + 3) / 4;
+        int kz = 8;
+        while (kz > 1 && bpp * ((h.nz + kz - 1) / kz) < 2048) kz /= 2;
+        hipLaunchKernelGGL(k_jacobi_adj_col<R>, dim3(bpp * ((h.nz + kz - 1) / kz)), dim3(256), 0, st, tab, in, out,
+                           arhs, out_g, acc, kz);
+    } else if (ns == 2)
         hipLaunchKernelGGL((k_jacobi_adj<R, 2>), grid_interior(h), dim3(256), 0, st, tab, in, out, arhs, in_g, out_g,
                            first);
     else
@@ -1111,7 +1181,7 @@ template void copy_fields<double>(const FieldCopy<double>&, long, hipStream_t);
                                  const double*, double*, double*, double*, double*, double*, double*, double*,     \
                                  double*, hipStream_t);                                                            \
     template void adj_jacobi<R>(const Inst<R>*, const Inst<R>&, const double*, double*, double*, int, int, int,   \
-                                int, hipStream_t);                                                                 \
+                                int, int, hipStream_t);                                                            \
     template void adj_rhs<R>(const Inst<R>*, const Inst<R>&, const R*, const R*, const R*, const R*,              \
                              const double*, double*, double*, double*, double*, hipStream_t);                      \
     template void adj_predict<R>(const Inst<R>*, const Inst<R>&, const R*, const R*, const R*, const R*,          \

@@ -210,6 +210,9 @@ struct Ensemble : EnsBase {
         double *aus = nullptr, *avs = nullptr, *aws = nullptr, *arhs = nullptr, *arho = nullptr;
         double *akap = nullptr, *afs = nullptr;  // CSF only
         double* seed[5] = {};  // F, u, v, w, p seeds of the final state (nullptr: zero)
+        hipStream_t st2 = nullptr;  // taped forward passes, overlapped with the reverse
+        hipEvent_t ev_start = nullptr, ev_tape[2] = {}, ev_rev[2] = {};
+        R* csf_scratch = nullptr;  // smoothed F and curvature of the reverse (VOF_FLAG_CSF)
         double* seed_mem = nullptr;
         double* part = nullptr;
         double* acc = nullptr;
@@ -220,8 +223,12 @@ struct Ensemble : EnsBase {
 
     void adj_release() override {
         Adj& a = adj;
-        for (void* p : {(void*)a.store, (void*)a.d, (void*)a.seed_mem, (void*)a.part, (void*)a.acc})
+        for (void* p : {(void*)a.store, (void*)a.d, (void*)a.seed_mem, (void*)a.part, (void*)a.acc,
+                        (void*)a.csf_scratch})
             if (p) (void)hipFree(p);
+        for (hipEvent_t ev : {a.ev_start, a.ev_tape[0], a.ev_tape[1], a.ev_rev[0], a.ev_rev[1]})
+            if (ev) (void)hipEventDestroy(ev);
+        if (a.st2) (void)hipStreamDestroy(a.st2);
         a = Adj();
     }
 
@@ -795,9 +802,16 @@ struct Ensemble : EnsBase {
     // of the interval being reversed [nck, nck+K+1), the final state nck+K+1,
     // and the K step tapes (u*, v*, w*, F after the first and the second sweep)
     // [nck+K+2, nck+2K+2).
+    // Store layout (states of 5 fields): checkpoints [0, nck); two interval
+    // buffers of K+1 states each (the states entering the steps of an interval
+    // and its end state); the final state of the forward; two tape buffers of
+    // K tapes.  Buffer b = interval & 1, so that one interval can be recomputed
+    // while the next one up is reversed.
     R* adj_state(size_t idx) const { return adj.store + idx * 5 * adj.nt; }
-    size_t adj_final() const { return size_t(adj.nck) + size_t(adj.K) + 1; }
-    R* adj_tape(size_t m) const { return adj_state(adj_final() + 1 + m); }
+    R* adj_ival(int b, size_t m) const { return adj_state(size_t(adj.nck) + size_t(b) * size_t(adj.K + 1) + m); }
+    size_t adj_final() const { return size_t(adj.nck) + 2 * size_t(adj.K + 1); }
+    R* adj_tape(int b, size_t m) const { return adj_state(adj_final() + 1 + size_t(b) * size_t(adj.K) + m); }
+    static size_t adj_store_states(int nck, int K) { return size_t(nck) + 4 * size_t(K) + 3; }
 
     vof_status save_state(R* dst) {
         const auto& I = ih[0];
@@ -851,7 +865,7 @@ struct Ensemble : EnsBase {
         const int K = std::min(every, n);
         const int nck = (n + K - 1) / K;
         const size_t nt = size_t(ih[0].ntot);
-        const size_t states = size_t(nck) + 2 * size_t(K) + 2;
+        const size_t states = adj_store_states(nck, K);
         if (adj.store && (adj.nt != nt || adj.store_states < states)) {
             (void)hipFree(adj.store);
             adj.store = nullptr;
@@ -939,6 +953,12 @@ struct Ensemble : EnsBase {
     // kernel never writes (the F ghosts, the unused u* wall faces) stay zero.
     vof_status adj_alloc() {
         const size_t nt = adj.nt;
+        if (!adj.st2) {
+            VOF_HIP(hipStreamCreateWithFlags(&adj.st2, hipStreamNonBlocking));
+            for (hipEvent_t* ev : {&adj.ev_start, &adj.ev_tape[0], &adj.ev_tape[1], &adj.ev_rev[0], &adj.ev_rev[1]})
+                VOF_HIP(hipEventCreateWithFlags(ev, hipEventDisableTiming));
+        }
+        if ((flags & VOF_FLAG_CSF) && !adj.csf_scratch) VOF_HIP(hipMalloc(&adj.csf_scratch, 2 * nt * sizeof(R)));
         const size_t kArrays = (flags & VOF_FLAG_CSF) ? 18 : 16;
         if (!adj.d) {
             VOF_HIP(hipMalloc(&adj.d, kArrays * nt * sizeof(double)));
@@ -988,14 +1008,18 @@ struct Ensemble : EnsBase {
             const double* in = adj.ap[c];
             for (int l = 0; l < nl; ++l) {
                 double* out = ((nl - 1 - l) % 2 == 0) ? adj.ap[x] : adj.gt;
-                vof::adj_jacobi(e.tab, I, in, out, adj.arhs, 1, l == 0, l == nl - 1, l == 0, st);
+                // rhs_bar = sum of every launch's input h: launch 0 assigns its
+                // own; after it, odd launches add their input and output (the
+                // next launch's input), and a last odd launch only its input
+                const int acc = (l == 0) ? 1 : ((l % 2 == 1) ? ((l + 1 < nl) ? 2 : 1) : 0);
+                vof::adj_jacobi(e.tab, I, in, out, adj.arhs, 1, l == 0, l == nl - 1, l == 0, acc, st);
                 in = out;
             }
             vof::adj_rhs(e.tab, I, SF, us, vs, ws, adj.arhs, adj.aus, adj.avs, adj.aws, adj.arho, st);
         }
         if (flags & VOF_FLAG_CSF)
-            vof::adj_csf(e.tab, I, SF, I.fsm, I.kappa, adj.aus, adj.avs, adj.aws, adj.akap, adj.afs, adj.aF[f],
-                         adj.arho, adj.part, adj.acc, st);
+            vof::adj_csf(e.tab, I, SF, adj.csf_scratch, adj.csf_scratch + nt, adj.aus, adj.avs, adj.aws, adj.akap,
+                         adj.afs, adj.aF[f], adj.arho, adj.part, adj.acc, st);
         vof::adj_predict(e.tab, I, SF, Su, Sv, Sw, adj.aus, adj.avs, adj.aws, adj.arho, adj.au[x], adj.av[x],
                          adj.aw[x], adj.aF[f], adj.part, adj.acc, st);
         adj.cur = x;
@@ -1023,19 +1047,45 @@ struct Ensemble : EnsBase {
             else
                 VOF_HIP(hipMemsetAsync(init[q], 0, nt * sizeof(double), st));
         }
-        // per interval, last first: one taped forward pass from its checkpoint
-        // (states and tapes of every step), then its steps in reverse
-        for (int k = adj.nck - 1; k >= 0; --k) {
-            const int s0 = k * adj.K, len = std::min(adj.n, s0 + adj.K) - s0;
+        // Per interval, last first: a taped forward pass from its checkpoint
+        // (the state entering every step, and its tapes) into buffer k & 1,
+        // then its steps in reverse.  The taped pass of interval k-1 runs on
+        // the second stream while interval k is reversed on the first: it
+        // writes only the solver's fields and buffer (k-1) & 1, which the
+        // reverse of interval k+1 (waited for) was the last to read; the
+        // reverse reads only the store and its own arrays.
+        auto retape = [&](int k) -> vof_status {
+            const int b = k & 1, s0 = k * adj.K, len = std::min(adj.n, s0 + adj.K) - s0;
             VOF_TRY(load_state(adj_state(size_t(k)), 0, 0));
             for (int m = 0; m < len; ++m) {
-                VOF_TRY(save_state(adj_state(size_t(adj.nck) + size_t(m))));
-                enqueue_step_taped(adj.first + s0 + m, adj_tape(size_t(m)));
+                VOF_TRY(save_state(adj_ival(b, size_t(m))));
+                enqueue_step_taped(adj.first + s0 + m, adj_tape(b, size_t(m)));
             }
-            VOF_TRY(save_state(adj_state(size_t(adj.nck) + size_t(len))));
+            return save_state(adj_ival(b, size_t(len)));
+        };
+        const hipStream_t st_rev = st;
+        VOF_HIP(hipEventRecord(adj.ev_start, st_rev));  // solver fields as the caller left them
+        VOF_HIP(hipStreamWaitEvent(adj.st2, adj.ev_start, 0));
+        auto retape_on_st2 = [&](int k) -> vof_status {
+            st = adj.st2;  // every enqueue of the taped pass goes to the second stream
+            const vof_status rc = retape(k);
+            st = st_rev;
+            if (rc != VOF_OK) return rc;
+            VOF_HIP(hipEventRecord(adj.ev_tape[k & 1], adj.st2));
+            return VOF_OK;
+        };
+        VOF_TRY(retape_on_st2(adj.nck - 1));
+        for (int k = adj.nck - 1; k >= 0; --k) {
+            if (k >= 1) {
+                if (k + 1 <= adj.nck - 1) VOF_HIP(hipStreamWaitEvent(adj.st2, adj.ev_rev[(k + 1) & 1], 0));
+                VOF_TRY(retape_on_st2(k - 1));
+            }
+            VOF_HIP(hipStreamWaitEvent(st_rev, adj.ev_tape[k & 1], 0));
+            const int b = k & 1, s0 = k * adj.K, len = std::min(adj.n, s0 + adj.K) - s0;
             for (int m = len - 1; m >= 0; --m)
-                VOF_TRY(reverse_step(adj.first + s0 + m, adj_state(size_t(adj.nck) + size_t(m)),
-                                     adj_state(size_t(adj.nck) + size_t(m) + 1), adj_tape(size_t(m))));
+                VOF_TRY(reverse_step(adj.first + s0 + m, adj_ival(b, size_t(m)), adj_ival(b, size_t(m) + 1),
+                                     adj_tape(b, size_t(m))));
+            VOF_HIP(hipEventRecord(adj.ev_rev[b], st_rev));
         }
         VOF_TRY(load_state(adj_state(adj_final()), adj.f_fin, adj.p_fin));
         VOF_HIP(hipEventRecord(e1, st));
