@@ -255,6 +255,29 @@ def precision_study(vof):
     return out
 
 
+def transfer_leg(s, cells, ms_step):
+    """Host <-> device cost of handing the state over as host buffers: upload
+    and download of F, u, v, w, p through the C ABI (pageable numpy arrays,
+    the solver's stream), and the step rate if every step were such a round
+    trip.  Not part of `value`, which is measured on HBM-resident fields."""
+    names = ("F", "U", "V", "W", "P")
+    host = {k: s.download(k) for k in names}
+    s.sync()
+    t0 = time.perf_counter()
+    for k in names:
+        s.upload(k, host[k])
+    s.sync()
+    t_up = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for k in names:
+        host[k] = s.download(k)
+    t_down = time.perf_counter() - t0
+    nbytes = sum(a.nbytes for a in host.values())
+    return {"bytes_each_way": nbytes, "upload_ms": 1e3 * t_up, "download_ms": 1e3 * t_down,
+            "upload_gbs": nbytes / t_up / 1e9, "download_gbs": nbytes / t_down / 1e9,
+            "cell_updates_per_s_one_step_per_round_trip": cells / (ms_step * 1e-3 + t_up + t_down)}
+
+
 def cpu_baseline(n, prec, budget_s):
     """Time the oracle on the same workload: one warm-up step, then as many
     steps as fit the budget (at least one)."""
@@ -339,6 +362,7 @@ def main():
     step_bytes = sum(STAGE_BYTES[k] * LAUNCHES_PER_STEP[k] for k in STAGE_BYTES) * (esz // 4) * cells
     diag = s.diagnostics()
     tuning = s.get_tuning()
+    pcie = transfer_leg(s, cells, ms_step)
     s.close()
     copy_hbm = vof.copy_bandwidth(1 << 30, 10)
     copy_mall = vof.copy_bandwidth(1 << 26, 50)
@@ -382,6 +406,7 @@ def main():
         "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kern.items()},
         "final_liquid_volume": diag["liquid_volume"],
+        "pcie": pcie,
     }
     if not a.no_batched:
         out["batched"] = batched_leg(vof)
