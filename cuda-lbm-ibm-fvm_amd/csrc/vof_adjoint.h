@@ -30,12 +30,12 @@ template <class R> void adj_fct(const Inst<R>* tab, const Inst<R>& h, int dir, c
 template <class R> void adj_correct(const Inst<R>* tab, const Inst<R>& h, const R* F, const R* p, const double* au,
                                     const double* av, const double* aw, double* aus, double* avs, double* aws,
                                     double* ap, double* arho, double* au0, double* av0, double* aw0, hipStream_t st);
-// ns (1 or 2) Jacobi sweeps in reverse, on h = D^-1 gbar: in_g: the input is
-// gbar, out_g: write gbar instead of h, first: assign arhs instead of adding.
-// acc (single sweeps after a step's first): 0 arhs untouched, 1 arhs += input
-// h, 2 arhs += input h + output h.
+// One Jacobi sweep in reverse, on h = D^-1 gbar: in_g: the input is gbar,
+// out_g: write gbar instead of h, first: assign arhs instead of adding.  acc
+// (sweeps after a step's first): 0 arhs untouched, 1 arhs += input h,
+// 2 arhs += input h + output h.
 template <class R> void adj_jacobi(const Inst<R>* tab, const Inst<R>& h, const double* in, double* out,
-                                   double* arhs, int ns, int in_g, int out_g, int first, int acc, hipStream_t st);
+                                   double* arhs, int in_g, int out_g, int first, int acc, hipStream_t st);
 template <class R> void adj_rhs(const Inst<R>* tab, const Inst<R>& h, const R* F, const R* us, const R* vs,
                                 const R* ws, const double* arhs, double* aus, double* avs, double* aws, double* arho,
                                 hipStream_t st);

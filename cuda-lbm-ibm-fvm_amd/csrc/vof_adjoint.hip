@@ -599,9 +599,9 @@ __global__ __launch_bounds__(256) void k_correct_adj(const Inst<R>* __restrict__
 // coefficients (summed in the forward's order).  N is symmetric, so with
 // h = D^-1 gbar the reverse of one sweep is
 //   rhs_bar += h,   gbar_prev = -N h,   h_prev = D^-1 gbar_prev,
-// a zero-rhs Jacobi sweep on h.  A launch does one or two such sweeps; in_g:
-// the input is gbar (divided by ap on load), out_g: write gbar_prev instead of
-// h_prev (last launch), first: assign rhs_bar instead of adding.  Only interior
+// a zero-rhs Jacobi sweep on h.  A launch does one such sweep; in_g: the input
+// is gbar (divided by ap on load), out_g: write gbar_prev instead of h_prev
+// (last launch), first: assign rhs_bar instead of adding.  Only interior
 // entries are read or written.
 template <class R>
 __device__ __forceinline__ double jac_ap(const Inst<R>& I, int i, int j, int k) {
@@ -611,7 +611,7 @@ __device__ __forceinline__ double jac_ap(const Inst<R>& I, int i, int j, int k) 
     return double(-(ae + aw + an + as + af + ab));
 }
 
-template <class R, int NS>
+template <class R>
 __global__ __launch_bounds__(256) void k_jacobi_adj(const Inst<R>* __restrict__ tab, const double* __restrict__ in,
                                                     double* __restrict__ out, double* __restrict__ arhs, int in_g,
                                                     int out_g, int first) {
@@ -624,32 +624,14 @@ __global__ __launch_bounds__(256) void k_jacobi_adj(const Inst<R>* __restrict__ 
         const double x = in[kk * sy + jj * sx + ii];
         return in_g ? x / jac_ap(I, ii, jj, kk) : x;
     };
-    auto nh = [&](int ii, int jj, int kk) {  // -N h at an interior cell
-        double s = 0.0;
-        if (ii < nx) s -= ax * H(ii + 1, jj, kk);
-        if (ii > 1) s -= ax * H(ii - 1, jj, kk);
-        if (jj < ny) s -= ay * H(ii, jj + 1, kk);
-        if (jj > 1) s -= ay * H(ii, jj - 1, kk);
-        if (kk < nz) s -= az * H(ii, jj, kk + 1);
-        if (kk > 1) s -= az * H(ii, jj, kk - 1);
-        return s;
-    };
-    double acc = H(i, j, k);
-    double g;
-    if (NS == 1) {
-        g = nh(i, j, k);
-    } else {
-        // first sweep at the cell and its six neighbours, second at the cell
-        auto h1 = [&](int ii, int jj, int kk) { return nh(ii, jj, kk) / jac_ap(I, ii, jj, kk); };
-        acc += h1(i, j, k);
-        g = 0.0;
-        if (i < nx) g -= ax * h1(i + 1, j, k);
-        if (i > 1) g -= ax * h1(i - 1, j, k);
-        if (j < ny) g -= ay * h1(i, j + 1, k);
-        if (j > 1) g -= ay * h1(i, j - 1, k);
-        if (k < nz) g -= az * h1(i, j, k + 1);
-        if (k > 1) g -= az * h1(i, j, k - 1);
-    }
+    double g = 0.0;  // -N h at the cell
+    if (i < nx) g -= ax * H(i + 1, j, k);
+    if (i > 1) g -= ax * H(i - 1, j, k);
+    if (j < ny) g -= ay * H(i, j + 1, k);
+    if (j > 1) g -= ay * H(i, j - 1, k);
+    if (k < nz) g -= az * H(i, j, k + 1);
+    if (k > 1) g -= az * H(i, j, k - 1);
+    const double acc = H(i, j, k);
     out[c] = out_g ? g : g / jac_ap(I, i, j, k);
     arhs[c] = first ? acc : arhs[c] + acc;
 }
@@ -1105,21 +1087,19 @@ void adj_correct(const Inst<R>* tab, const Inst<R>& h, const R* F, const R* p, c
                        ap, arho, au0, av0, aw0);
 }
 template <class R>
-void adj_jacobi(const Inst<R>* tab, const Inst<R>& h, const double* in, double* out, double* arhs, int ns, int in_g,
+void adj_jacobi(const Inst<R>* tab, const Inst<R>& h, const double* in, double* out, double* arhs, int in_g,
                 int out_g, int first, int acc, hipStream_t st) {
-    if (ns == 1 && !in_g && !first) {
+    if (!in_g && !first) {
         // column march; z chunk of 8 planes, fewer while under 2048 work-groups
         const int bpp = ((h.sx * (h.ny + 2) + 61) / 62 + 3) / 4;
         int kz = 8;
         while (kz > 1 && bpp * ((h.nz + kz - 1) / kz) < 2048) kz /= 2;
         hipLaunchKernelGGL(k_jacobi_adj_col<R>, dim3(bpp * ((h.nz + kz - 1) / kz)), dim3(256), 0, st, tab, in, out,
                            arhs, out_g, acc, kz);
-    } else if (ns == 2)
-        hipLaunchKernelGGL((k_jacobi_adj<R, 2>), grid_interior(h), dim3(256), 0, st, tab, in, out, arhs, in_g, out_g,
+    } else {
+        hipLaunchKernelGGL(k_jacobi_adj<R>, grid_interior(h), dim3(256), 0, st, tab, in, out, arhs, in_g, out_g,
                            first);
-    else
-        hipLaunchKernelGGL((k_jacobi_adj<R, 1>), grid_interior(h), dim3(256), 0, st, tab, in, out, arhs, in_g, out_g,
-                           first);
+    }
 }
 template <class R>
 void adj_rhs(const Inst<R>* tab, const Inst<R>& h, const R* F, const R* us, const R* vs, const R* ws,
@@ -1181,7 +1161,7 @@ template void copy_fields<double>(const FieldCopy<double>&, long, hipStream_t);
                                  const double*, double*, double*, double*, double*, double*, double*, double*,     \
                                  double*, hipStream_t);                                                            \
     template void adj_jacobi<R>(const Inst<R>*, const Inst<R>&, const double*, double*, double*, int, int, int,   \
-                                int, int, hipStream_t);                                                            \
+                                int, hipStream_t);                                                                 \
     template void adj_rhs<R>(const Inst<R>*, const Inst<R>&, const R*, const R*, const R*, const R*,              \
                              const double*, double*, double*, double*, double*, hipStream_t);                      \
     template void adj_predict<R>(const Inst<R>*, const Inst<R>&, const R*, const R*, const R*, const R*,          \

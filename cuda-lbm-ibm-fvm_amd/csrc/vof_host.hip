@@ -1001,9 +1001,8 @@ struct Ensemble : EnsBase {
         if (sweeps == 0) {
             VOF_HIP(hipMemcpyAsync(adj.ap[x], adj.ap[c], nt * sizeof(double), hipMemcpyDeviceToDevice, st));
         } else {
-            // one sweep per launch (two per launch recompute the intermediate
-            // at six neighbours, which costs more than it saves); the outputs
-            // alternate so that the last lands in ap[x]
+            // one sweep per launch; the outputs alternate so that the last
+            // lands in ap[x]
             const int nl = sweeps;
             const double* in = adj.ap[c];
             for (int l = 0; l < nl; ++l) {
@@ -1012,7 +1011,7 @@ struct Ensemble : EnsBase {
                 // own; after it, odd launches add their input and output (the
                 // next launch's input), and a last odd launch only its input
                 const int acc = (l == 0) ? 1 : ((l % 2 == 1) ? ((l + 1 < nl) ? 2 : 1) : 0);
-                vof::adj_jacobi(e.tab, I, in, out, adj.arhs, 1, l == 0, l == nl - 1, l == 0, acc, st);
+                vof::adj_jacobi(e.tab, I, in, out, adj.arhs, l == 0, l == nl - 1, l == 0, acc, st);
                 in = out;
             }
             vof::adj_rhs(e.tab, I, SF, us, vs, ws, adj.arhs, adj.aus, adj.avs, adj.aws, adj.arho, st);

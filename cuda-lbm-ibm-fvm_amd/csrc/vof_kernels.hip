@@ -91,7 +91,9 @@ __device__ __forceinline__ R mom(const Inst<R>& I, R c, R xm, R xp, R ym, R yp, 
 // and w* on face k (interior faces only: i>=2, j>=2, k>=2; wall faces are never
 // written, as in the reference).  Per plane: 10 coalesced loads (u,v,w at k+1
 // and at rows j-1, j+1 of plane k+1, F at k), 8 lane shuffles.
-template <class R>
+// KZ > 0: the z chunk is KZ planes and the plane loop is unrolled, so the
+// register window moves by renaming instead of copies; KZ = 0: any chunk.
+template <class R, int KZ>
 __global__ __launch_bounds__(256) void k_predict(const EnsArg<R> ea,
                                                  int fsel, int kz) {
     VOF_PROLOGUE(R)
@@ -119,7 +121,7 @@ __global__ __launch_bounds__(256) void k_predict(const EnsArg<R> ea,
     R nu = u.ld(tc, o + sy), nv = v.ld(tc, o + sy), nw = w.ld(tc, o + sy);
     R nujm = u.ld(tjm, o + sy), nujp = u.ld(tjp, o + sy), nvjm = v.ld(tjm, o + sy), nvjp = v.ld(tjp, o + sy);
     R nwjm = w.ld(tjm, o + sy), nwjp = w.ld(tjp, o + sy), nf = F.ld(tc, o);
-    for (; k <= m.k1; ++k, o += sy) {
+    auto plane = [&]() __attribute__((always_inline)) {
         const R u_kp = nu, v_kp = nv, w_kp = nw;
         const R ujm_n = nujm, ujp_n = nujp, vjm_n = nvjm, vjp_n = nvjp, wjm_n = nwjm, wjp_n = nwjp;
         const R f = nf;
@@ -162,7 +164,15 @@ __global__ __launch_bounds__(256) void k_predict(const EnsArg<R> ea,
         vjp_km = vjp; vjm = vjm_n; vjp = vjp_n;
         wjm = wjm_n; wjp = wjp_n;
         udn_km = udn; wup_k = wup_kp;
+    };
+    if constexpr (KZ > 0) {
+        if (m.k1 - m.k0 + 1 == KZ) {  // every chunk but a short last one
+#pragma unroll
+            for (int e = 0; e < KZ; ++e, ++k, o += sy) plane();
+            return;
+        }
     }
+    for (; k <= m.k1; ++k, o += sy) plane();
 }
 
 // ---------------------------------------------------------------------------
@@ -1038,7 +1048,13 @@ void launch_copy(const void* a, void* b, long n16, hipStream_t st) {
 // Host launchers.
 template <class R>
 void launch_predict(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st) {
-    hipLaunchKernelGGL(k_predict<R>, dim3(g.nblk), dim3(256), 0, st, ens_arg(e, g), fsel, g.kz);
+    const dim3 grid(g.nblk), blk(256);
+    switch (g.kz) {  // chunks of 4, 8 or 16 planes: unrolled plane loop
+        case 4: hipLaunchKernelGGL((k_predict<R, 4>), grid, blk, 0, st, ens_arg(e, g), fsel, g.kz); break;
+        case 8: hipLaunchKernelGGL((k_predict<R, 8>), grid, blk, 0, st, ens_arg(e, g), fsel, g.kz); break;
+        case 16: hipLaunchKernelGGL((k_predict<R, 16>), grid, blk, 0, st, ens_arg(e, g), fsel, g.kz); break;
+        default: hipLaunchKernelGGL((k_predict<R, 0>), grid, blk, 0, st, ens_arg(e, g), fsel, g.kz);
+    }
 }
 template <class R>
 void launch_rhs(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st) {

@@ -61,8 +61,14 @@ typedef enum vof_field {
  * face gains dt * sigma * kappa_f * dF/dn / rho_f, kappa = -div(grad F/|grad F|)
  * from vertex normals.  sigma = 0 gives the parity step. */
 #define VOF_FLAG_CSF           0x2u
-#define VOF_FLAG_RHO_PREDICTOR 0x4u /* gravity/CSF scaled by face density in the predictor */
-#define VOF_FLAG_NO_SLIP       0x8u /* tangential wall ghosts mirror to give zero wall velocity */
+/* Reserved, not built (vof_create returns VOF_ERR_UNSUPPORTED): a density-
+ * weighted viscous term in the predictor and no-slip tangential walls.  The
+ * reference's predictor takes rho but never reads it (voftrial1.cu:233, :290,
+ * :347) and its tangential wall ghosts copy the interior (free slip,
+ * :959-1051); density enters the step through the Poisson right-hand side and
+ * the correction, and the surface force above through its face density. */
+#define VOF_FLAG_RHO_PREDICTOR 0x4u
+#define VOF_FLAG_NO_SLIP       0x8u
 #define VOF_FLAG_WATCHDOG      0x10u/* check F in [0,1] and finite u,v,w,p after every step */
 
 /* Solver parameters.  Replaces `struct Params` + `cpar_host` defaults
