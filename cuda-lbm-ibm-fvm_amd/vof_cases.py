@@ -9,7 +9,9 @@ of this library are quoted on:
   (64^3 FP64, 200^3 FP32 and FP64, 96^3 for the gradient);
 * ensemble: 64 dam breaks of uneven size and physics (seed + i);
 * ripple: a liquid layer 0.06 m deep whose free surface carries a sum of
-  plane waves of amplitude up to two cells (384^3).
+  plane waves of amplitude up to two cells (384^3);
+* drop: a density-matched ellipsoidal drop flattened against the floor, for
+  the surface-tension gradient (64^3).
 
 Base seed 20261203; instance i of the ensemble uses seed + i.
 """
@@ -83,3 +85,20 @@ def field_errors(a: np.ndarray, ref: np.ndarray, interior: bool = True) -> dict:
     d = a - ref
     nr = float(np.sqrt(np.sum(ref * ref)))
     return {"max_abs": float(np.max(np.abs(d))), "rel_l2": float(np.sqrt(np.sum(d * d)) / nr) if nr > 0 else 0.0}
+
+
+def drop_F(n: int, center=(0.05, 0.05, 0.012), semi_axes=(0.025, 0.025, 0.012), L: float = 0.1, sub: int = 4,
+           dtype=np.float64) -> np.ndarray:
+    """Volume fraction of an ellipsoidal drop on the stored (n+2)^3 grid, each
+    interior cell sampled sub^3 times; ghosts copy the nearest interior cell.
+    The default drop is flattened against the floor (its lowest point at
+    z = 0), so surface tension pulls it rounder and raises its centroid."""
+    h = L / n
+    off = (np.arange(sub) + 0.5) / sub
+    c = (np.arange(n)[:, None] + off[None, :]).ravel() * h
+    qx = ((c - center[0]) / semi_axes[0]) ** 2
+    qy = ((c - center[1]) / semi_axes[1]) ** 2
+    qz = ((c - center[2]) / semi_axes[2]) ** 2
+    inside = (qz[:, None, None] + qy[None, :, None] + qx[None, None, :]) <= 1.0
+    f = inside.reshape(n, sub, n, sub, n, sub).mean(axis=(1, 3, 5))
+    return np.pad(f, 1, mode="edge").astype(dtype)
