@@ -49,3 +49,15 @@ def test_field_errors_closed_form():
     e = field_errors(a, ref)
     assert e["max_abs"] == 0.5
     assert abs(e["rel_l2"] - 0.5 / np.sqrt(8)) < 1e-15
+
+
+def test_drop_volume_and_floor_contact():
+    # sub-sampled ellipsoid: volume within 1% of 4/3 pi a b c at 64^3; touches the floor, not the walls
+    from vof_cases import drop_F
+
+    n = 64
+    F = drop_F(n)
+    vol = F[1:-1, 1:-1, 1:-1].sum() * (0.1 / n) ** 3
+    assert abs(vol - 4.0 / 3.0 * np.pi * 0.025 * 0.025 * 0.012) / vol < 0.01
+    assert F[1].max() > 0.0 and F[1:-1, 1:-1, 1].max() == 0.0
+    np.testing.assert_array_equal(F[0], F[1])  # ghost plane copies the first interior plane
