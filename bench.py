@@ -327,9 +327,25 @@ def transfer_leg(s, cells, ms_step):
             "cell_updates_per_s_one_step_per_round_trip": cells / (ms_step * 1e-3 + t_up + t_down)}
 
 
+def _host_cpu():
+    """CPU model and logical CPU count of this host (from /proc/cpuinfo)."""
+    model, count = "unknown", os.cpu_count() or 0
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, count
+
+
 def cpu_baseline(n, prec, budget_s):
-    """Time the oracle on the same workload: one warm-up step, then as many
-    steps as fit the budget (at least one)."""
+    """Time the oracle (the reference's kernel bodies as OpenMP loops, one
+    IEEE rounding per source operation) on the same workload: one warm-up
+    step, then three timed runs of as many steps as fit a third of the budget
+    each; the median run is reported."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     import vof
@@ -338,21 +354,30 @@ def cpu_baseline(n, prec, budget_s):
     o.init_box(0.04, 0.04, 0.06)
     t0 = time.perf_counter()
     o.step(1, 1)
-    t1 = time.perf_counter()
-    per = max(t1 - t0, 1e-6)
-    k = max(1, min(1000, int(budget_s / per)))
-    t0 = time.perf_counter()
-    o.step(2, k)
-    el = time.perf_counter() - t0
+    per = max(time.perf_counter() - t0, 1e-6)
+    k = max(1, min(1000, int(budget_s / 3 / per)))
+    rates = []
+    istep = 2
+    for _ in range(3):
+        t0 = time.perf_counter()
+        o.step(istep, k)
+        rates.append(n ** 3 * k / (time.perf_counter() - t0))
+        istep += k
     cores = oracle.threads()
     o.close()
+    rate = sorted(rates)[1]
+    model, ncpu = _host_cpu()
+    esz = 2 if prec == 8 else 1
     return {
-        "value": n ** 3 * k / el,
+        "value": rate,
         "unit": "cell-updates/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"{k} steps (after 1 warm-up) of the same {n}^3 {'FP64' if prec == 8 else 'FP32'} dam break, "
-                  f"{el:.1f} s, OpenMP oracle restating voftrial1.cu's kernels on {cores} threads",
+        "gbs_algorithmic": rate * 236 * esz / 1e9,
+        "host": f"{model}, {ncpu} logical CPUs visible, OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', 'unset')}",
+        "sample": f"median of 3 runs of {k} steps (after 1 warm-up) of the same {n}^3 "
+                  f"{'FP64' if prec == 8 else 'FP32'} dam break, OpenMP oracle restating voftrial1.cu's kernels "
+                  f"on {cores} threads",
     }
 
 
