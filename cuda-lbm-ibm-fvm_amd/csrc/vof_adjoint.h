@@ -31,20 +31,24 @@ template <class R> void adj_correct(const Inst<R>* tab, const Inst<R>& h, const 
                                     const double* av, const double* aw, double* aus, double* avs, double* aws,
                                     double* ap, double* arho, double* au0, double* av0, double* aw0, hipStream_t st);
 // One Jacobi sweep in reverse, on h = D^-1 gbar: in_g: the input is gbar,
-// out_g: write gbar instead of h, first: assign arhs instead of adding.  acc
-// (sweeps after a step's first): 0 arhs untouched, 1 arhs += input h,
-// 2 arhs += input h + output h.
+// out_g: write gbar instead of h, first: assign arhs instead of adding; acc:
+// 0 arhs untouched, 1 arhs (+)= input h, 2 arhs (+)= input h + output h.
 template <class R> void adj_jacobi(const Inst<R>* tab, const Inst<R>& h, const double* in, double* out,
                                    double* arhs, int in_g, int out_g, int first, int acc, hipStream_t st);
 template <class R> void adj_rhs(const Inst<R>* tab, const Inst<R>& h, const R* F, const R* us, const R* vs,
                                 const R* ws, const double* arhs, double* aus, double* avs, double* aws, double* arho,
                                 hipStream_t st);
-// Predictor + rho/nu in reverse: au += (atomics), aF += ; adds dJ/d(rho_l, rho_g,
-// nu_l, nu_g, gx, gy, gz) to acc[0..7).  part: adj_part_size doubles.
+// Predictor + rho/nu in reverse: au += (atomics), aF += ; adds its work-group
+// partial sums of dJ/d(rho_l, rho_g, nu_l, nu_g, gx, gy, gz) to part (7 per
+// work-group; zero it before the first step), which adj_predict_sums reduces
+// into acc[0..7) once after the last step.
 template <class R> void adj_predict(const Inst<R>* tab, const Inst<R>& h, const R* F, const R* u, const R* v,
                                     const R* w, const double* aus, const double* avs, const double* aws,
                                     const double* arho, double* au, double* av, double* aw, double* aF, double* part,
                                     double* acc, hipStream_t st);
+void adj_predict_sums(int nx, int ny, int nz, const double* part, double* acc, hipStream_t st);
+// doubles of the predictor's partial-sum region (7 per work-group)
+inline size_t adj_predict_part_size(int nx, int ny, int nz) { return adj_part_size(nx, ny, nz); }
 // CSF in reverse (VOF_FLAG_CSF): recompute Fs and kappa of the step into fsm,
 // kap; then akap = (scratch), aF += and arho += the face-force VJP, afs =
 // (scratch) the curvature VJP, aF += its smoothing transpose (ghosts folded to

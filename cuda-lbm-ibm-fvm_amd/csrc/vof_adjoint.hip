@@ -71,8 +71,9 @@ __device__ __forceinline__ bool stored_cell(const Inst<R>& I, int& i, int& j, in
 }
 
 // Sum NQ per-lane values over the 256 lanes of a work-group into
-// part[blk*NQ + q] (blk = linear work-group index).  Every lane must call it.
-template <int NQ>
+// part[blk*NQ + q] (blk = linear work-group index), or add them to it when
+// ACC.  Every lane must call it.
+template <int NQ, bool ACC = false>
 __device__ __forceinline__ void block_parts(const double (&v)[NQ], double* part) {
     __shared__ double sh[NQ][256];
     for (int q = 0; q < NQ; ++q) sh[q][threadIdx.x] = v[q];
@@ -84,7 +85,10 @@ __device__ __forceinline__ void block_parts(const double (&v)[NQ], double* part)
     }
     if (threadIdx.x == 0) {
         const size_t blk = size_t(blockIdx.y) * gridDim.x + blockIdx.x;
-        for (int q = 0; q < NQ; ++q) part[blk * NQ + q] = sh[q][0];
+        for (int q = 0; q < NQ; ++q) {
+            if (ACC) part[blk * NQ + q] += sh[q][0];
+            else part[blk * NQ + q] = sh[q][0];
+        }
     }
 }
 
@@ -611,46 +615,26 @@ __device__ __forceinline__ double jac_ap(const Inst<R>& I, int i, int j, int k) 
     return double(-(ae + aw + an + as + af + ab));
 }
 
-template <class R>
-__global__ __launch_bounds__(256) void k_jacobi_adj(const Inst<R>* __restrict__ tab, const double* __restrict__ in,
-                                                    double* __restrict__ out, double* __restrict__ arhs, int in_g,
-                                                    int out_g, int first) {
-    const Inst<R>& I = tab[0];
-    int i, j, k, c;
-    if (!interior_cell(I, i, j, k, c)) return;
-    const int nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
-    const double ax = I.axx, ay = I.ayy, az = I.azz;
-    auto H = [&](int ii, int jj, int kk) {  // input as h at an interior cell
-        const double x = in[kk * sy + jj * sx + ii];
-        return in_g ? x / jac_ap(I, ii, jj, kk) : x;
-    };
-    double g = 0.0;  // -N h at the cell
-    if (i < nx) g -= ax * H(i + 1, j, k);
-    if (i > 1) g -= ax * H(i - 1, j, k);
-    if (j < ny) g -= ay * H(i, j + 1, k);
-    if (j > 1) g -= ay * H(i, j - 1, k);
-    if (k < nz) g -= az * H(i, j, k + 1);
-    if (k > 1) g -= az * H(i, j, k - 1);
-    const double acc = H(i, j, k);
-    out[c] = out_g ? g : g / jac_ap(I, i, j, k);
-    arhs[c] = first ? acc : arhs[c] + acc;
-}
-
-// The same reverse sweep for every launch after a step's first (the input is
-// h): a column march.  Lanes cover the flattened (x,y) plane in overlapped
-// waves that store their middle 62 lanes; z neighbours are carried in
-// registers, x neighbours come from the adjacent lanes (the neighbour terms
-// are summed in the order above), and the division by ap is a multiplication
-// by 1/ap, formed once per lane for the three kinds of plane (k = 1, interior,
-// k = nz).  The adjoint is compared with its references to a tolerance, not
-// bit for bit, so the reciprocal is allowed here.
+// The same reverse sweep as a column march.  Lanes cover the flattened
+// (x,y) plane in overlapped waves that store their middle 62 lanes; z
+// neighbours are carried in registers, x neighbours come from the adjacent
+// lanes (the neighbour terms are summed in the order above), and the divisions
+// by ap are multiplications by 1/ap, formed once per lane for the three kinds
+// of plane (k = 1, interior, k = nz) of the rows j-1, j, j+1.  in_g: every
+// loaded value is gbar and becomes h on load.  first: rhs_bar is assigned.
 // acc: 0 leaves rhs_bar alone, 1 adds the input h, 2 adds the input and the
-// output h (the next launch's input, which then adds nothing): rhs_bar is read
-// and written once per pair of sweeps.
+// output h (the next launch's input, which then adds nothing), so rhs_bar is
+// read and written once per pair of sweeps.  The adjoint is compared with its
+// references to a tolerance, not bit for bit, so the reciprocals are allowed
+// here.
+// ri: 1/ap of a cell whose row and column are interior, for k = 1, interior
+// planes and k = nz (formed on the host with the same sums); the other cells
+// divide on the device.
 template <class R>
 __global__ __launch_bounds__(256) void k_jacobi_adj_col(const Inst<R>* __restrict__ tab,
                                                         const double* __restrict__ in, double* __restrict__ out,
-                                                        double* __restrict__ arhs, int out_g, int acc, int kz) {
+                                                        double* __restrict__ arhs, int in_g, int out_g, int first,
+                                                        int acc, int kz, double ri_lo, double ri_in, double ri_hi) {
     const Inst<R> I = tab[0];
     constexpr int H = 1, S = 64 - 2 * H;
     const int nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
@@ -667,18 +651,31 @@ __global__ __launch_bounds__(256) void k_jacobi_adj_col(const Inst<R>* __restric
     const int tjm = max(tc - sx, 0), tjp = min(tc + sx, P - 1);
     const int k0 = 1 + zb * kz, k1 = min(nz, k0 + kz - 1);
     const double ax = I.axx, ay = I.ayy, az = I.azz;
-    const double r_lo = 1.0 / jac_ap(I, i, j, 1), r_hi = 1.0 / jac_ap(I, i, j, nz);
-    const double r_in = (nz > 2) ? 1.0 / jac_ap(I, i, j, 2) : 0.0;
+    // 1/ap of rows j-1, j, j+1 (index 0, 1, 2) for k = 1, interior planes, k = nz;
+    // the side rows only when their loads are gbar
+    double rlo[3] = {ri_lo, ri_lo, ri_lo}, rin[3] = {ri_in, ri_in, ri_in}, rhi[3] = {ri_hi, ri_hi, ri_hi};
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const int jr = min(max(j - 1 + r, 0), ny + 1);
+        if ((r != 1 && !in_g) || (i >= 2 && i <= nx - 1 && jr >= 2 && jr <= ny - 1)) continue;
+        rlo[r] = 1.0 / jac_ap(I, i, jr, 1);
+        rhi[r] = 1.0 / jac_ap(I, i, jr, nz);
+        rin[r] = (nz > 2) ? 1.0 / jac_ap(I, i, jr, 2) : 0.0;
+    }
+    auto rk = [&](int r, int k) { return (k == 1) ? rlo[r] : ((k == nz) ? rhi[r] : rin[r]); };
+    auto as_h = [&](double x, int r, int k) { return in_g ? x * rk(r, k) : x; };
     const Buf<double> hb(in, I.ntot);
     int o = k0 * sy;
-    double hkm = hb.ld(tc, o - sy), hk = hb.ld(tc, o);
-    double n_kp = hb.ld(tc, o + sy), n_jm = hb.ld(tjm, o), n_jp = hb.ld(tjp, o);
+    double hkm = as_h(hb.ld(tc, o - sy), 1, k0 - 1), hk = as_h(hb.ld(tc, o), 1, k0);
+    double n_kp = as_h(hb.ld(tc, o + sy), 1, k0 + 1);
+    double n_jm = as_h(hb.ld(tjm, o), 0, k0), n_jp = as_h(hb.ld(tjp, o), 2, k0);
     for (int k = k0; k <= k1; ++k, o += sy) {
         const double hkp = n_kp, hjm = n_jm, hjp = n_jp;
-        const int o2 = min(o + sy, nz * sy);
-        n_kp = hb.ld(tc, o2 + sy);
-        n_jm = hb.ld(tjm, o2);
-        n_jp = hb.ld(tjp, o2);
+        const int k2 = min(k + 1, nz);  // plane whose rows are fetched next
+        const int o2 = k2 * sy;
+        n_kp = as_h(hb.ld(tc, o2 + sy), 1, k2 + 1);
+        n_jm = as_h(hb.ld(tjm, o2), 0, k2);
+        n_jp = as_h(hb.ld(tjp, o2), 2, k2);
         const double hw = lane_up(hk), he = lane_dn(hk);
         if (cell) {
             double sm = 0.0;
@@ -689,9 +686,11 @@ __global__ __launch_bounds__(256) void k_jacobi_adj_col(const Inst<R>* __restric
             if (k < nz) sm -= az * hkp;
             if (k > 1) sm -= az * hkm;
             const int c = o + tc;
-            const double ho = sm * ((k == 1) ? r_lo : ((k == nz) ? r_hi : r_in));
+            const double ho = sm * rk(1, k);
             out[c] = out_g ? sm : ho;
-            if (acc) arhs[c] += (acc == 2) ? hk + ho : hk;
+            const double add = (acc == 2) ? hk + ho : hk;
+            if (first) arhs[c] = add;
+            else if (acc) arhs[c] += add;
         }
         hkm = hk;
         hk = hkp;
@@ -882,7 +881,7 @@ __global__ __launch_bounds__(256) void k_predict_adj(const Inst<R>* __restrict__
     }
     flush(k1, false);
     flush(k1 + 1, false);
-    block_parts<7>(sums, part);
+    block_parts<7, true>(sums, part);  // accumulated over the steps; summed once by adj_predict_sums
 }
 
 // ---------------------------------------------------------------------------
@@ -1089,17 +1088,17 @@ void adj_correct(const Inst<R>* tab, const Inst<R>& h, const R* F, const R* p, c
 template <class R>
 void adj_jacobi(const Inst<R>* tab, const Inst<R>& h, const double* in, double* out, double* arhs, int in_g,
                 int out_g, int first, int acc, hipStream_t st) {
-    if (!in_g && !first) {
-        // column march; z chunk of 8 planes, fewer while under 2048 work-groups
-        const int bpp = ((h.sx * (h.ny + 2) + 61) / 62 + 3) / 4;
-        int kz = 8;
-        while (kz > 1 && bpp * ((h.nz + kz - 1) / kz) < 2048) kz /= 2;
-        hipLaunchKernelGGL(k_jacobi_adj_col<R>, dim3(bpp * ((h.nz + kz - 1) / kz)), dim3(256), 0, st, tab, in, out,
-                           arhs, out_g, acc, kz);
-    } else {
-        hipLaunchKernelGGL(k_jacobi_adj<R>, grid_interior(h), dim3(256), 0, st, tab, in, out, arhs, in_g, out_g,
-                           first);
-    }
+    // column march; z chunk of 8 planes, fewer while under 2048 work-groups
+    const int bpp = ((h.sx * (h.ny + 2) + 61) / 62 + 3) / 4;
+    int kz = 8;
+    while (kz > 1 && bpp * ((h.nz + kz - 1) / kz) < 2048) kz /= 2;
+    // 1/ap of interior rows and columns, summed as jac_ap sums (field precision)
+    const R ax = h.axx, ay = h.ayy, az = h.azz, zf = (h.nz != 1) ? h.azz : R(0);
+    const double ri_lo = 1.0 / double(-(ax + ax + ay + ay + zf + R(0)));
+    const double ri_hi = 1.0 / double(-(ax + ax + ay + ay + R(0) + zf));
+    const double ri_in = 1.0 / double(-(ax + ax + ay + ay + az + az));
+    hipLaunchKernelGGL(k_jacobi_adj_col<R>, dim3(bpp * ((h.nz + kz - 1) / kz)), dim3(256), 0, st, tab, in, out, arhs,
+                       in_g, out_g, first, acc, kz, ri_lo, ri_in, ri_hi);
 }
 template <class R>
 void adj_rhs(const Inst<R>* tab, const Inst<R>& h, const R* F, const R* us, const R* vs, const R* ws,
@@ -1107,18 +1106,28 @@ void adj_rhs(const Inst<R>* tab, const Inst<R>& h, const R* F, const R* us, cons
     hipLaunchKernelGGL(k_rhs_adj<R>, grid_interior(h), dim3(256), 0, st, tab, F, us, vs, ws, arhs, aus, avs, aws,
                        arho);
 }
+int adj_predict_blocks(int nx, int ny, int nz, int* kz_out);
 template <class R>
 void adj_predict(const Inst<R>* tab, const Inst<R>& h, const R* F, const R* u, const R* v, const R* w,
                  const double* aus, const double* avs, const double* aws, const double* arho, double* au, double* av,
                  double* aw, double* aF, double* part, double* acc, hipStream_t st) {
-    // z chunk: 16 planes, fewer while that leaves under 1024 work-groups
-    const int nxy = ((h.nx + PTX - 1) / PTX) * ((h.ny + PTY - 1) / PTY);
-    int kz = 16;
-    while (kz > 2 && nxy * ((h.nz + kz - 1) / kz) < 1024) kz /= 2;
-    const int nb = nxy * ((h.nz + kz - 1) / kz);
+    int kz = 0;
+    const int nb = adj_predict_blocks(h.nx, h.ny, h.nz, &kz);
     hipLaunchKernelGGL(k_predict_adj<R>, dim3(nb), dim3(256), 0, st, tab, F, u, v, w, aus, avs, aws, arho, au, av,
                        aw, aF, part, kz);
-    hipLaunchKernelGGL(k_sum_parts<7>, dim3(1), dim3(256), 0, st, part, nb, acc);
+}
+// Work-groups of the predictor transpose (and its z chunk): 16 planes, fewer
+// while that leaves under 1024 work-groups.
+int adj_predict_blocks(int nx, int ny, int nz, int* kz_out) {
+    const int nxy = ((nx + PTX - 1) / PTX) * ((ny + PTY - 1) / PTY);
+    int kz = 16;
+    while (kz > 2 && nxy * ((nz + kz - 1) / kz) < 1024) kz /= 2;
+    if (kz_out) *kz_out = kz;
+    return nxy * ((nz + kz - 1) / kz);
+}
+void adj_predict_sums(int nx, int ny, int nz, const double* part, double* acc, hipStream_t st) {
+    hipLaunchKernelGGL(k_sum_parts<7>, dim3(1), dim3(256), 0, st, part, adj_predict_blocks(nx, ny, nz, nullptr),
+                       acc);
 }
 template <class R>
 void adj_csf(const Inst<R>* tab, const Inst<R>& h, const R* F, R* fsm, R* kap, const double* aus, const double* avs,

@@ -881,8 +881,10 @@ struct Ensemble : EnsBase {
             adj.store_states = states;
         }
         if (!adj.part) {
+            // two regions: the predictor's partial sums, accumulated over a
+            // whole backward sweep, then scratch for the other reductions
             const size_t np = vof::adj_part_size(ih[0].nx, ih[0].ny, ih[0].nz);
-            VOF_HIP(hipMalloc(&adj.part, np * sizeof(double)));
+            VOF_HIP(hipMalloc(&adj.part, 2 * np * sizeof(double)));
             VOF_HIP(hipMalloc(&adj.acc, vof::ADJ_NSCALAR * sizeof(double)));
         }
         for (double*& s : adj.seed) s = nullptr;
@@ -903,11 +905,12 @@ struct Ensemble : EnsBase {
         adj.f_fin = fsel;
         adj.p_fin = psel;
         const auto& I = ih[0];
-        const int nb = vof::adj_centroid_parts(e.tab, I, I.F[fsel], adj.part, st);
+        double* scratch = adj.part + vof::adj_part_size(I.nx, I.ny, I.nz);
+        const int nb = vof::adj_centroid_parts(e.tab, I, I.F[fsel], scratch, st);
         VOF_HIP(hipGetLastError());
         VOF_HIP(hipEventRecord(e1, st));
         std::vector<double> h(2 * size_t(nb));
-        VOF_HIP(hipMemcpyAsync(h.data(), adj.part, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        VOF_HIP(hipMemcpyAsync(h.data(), scratch, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
         VOF_HIP(hipStreamSynchronize(st));
         float ms = 0.f;
         VOF_HIP(hipEventElapsedTime(&ms, e0, e1));
@@ -1007,10 +1010,10 @@ struct Ensemble : EnsBase {
             const double* in = adj.ap[c];
             for (int l = 0; l < nl; ++l) {
                 double* out = ((nl - 1 - l) % 2 == 0) ? adj.ap[x] : adj.gt;
-                // rhs_bar = sum of every launch's input h: launch 0 assigns its
-                // own; after it, odd launches add their input and output (the
-                // next launch's input), and a last odd launch only its input
-                const int acc = (l == 0) ? 1 : ((l % 2 == 1) ? ((l + 1 < nl) ? 2 : 1) : 0);
+                // rhs_bar = sum of every launch's input h: even launches take
+                // their input and output (the next launch's input; launch 0
+                // assigns), a last even launch only its input, odd ones nothing
+                const int acc = (l % 2 == 0) ? ((l + 1 < nl) ? 2 : 1) : 0;
                 vof::adj_jacobi(e.tab, I, in, out, adj.arhs, l == 0, l == nl - 1, l == 0, acc, st);
                 in = out;
             }
@@ -1018,7 +1021,7 @@ struct Ensemble : EnsBase {
         }
         if (flags & VOF_FLAG_CSF)
             vof::adj_csf(e.tab, I, SF, adj.csf_scratch, adj.csf_scratch + nt, adj.aus, adj.avs, adj.aws, adj.akap,
-                         adj.afs, adj.aF[f], adj.arho, adj.part, adj.acc, st);
+                         adj.afs, adj.aF[f], adj.arho, adj.part + vof::adj_part_size(I.nx, I.ny, I.nz), adj.acc, st);
         vof::adj_predict(e.tab, I, SF, Su, Sv, Sw, adj.aus, adj.avs, adj.aws, adj.arho, adj.au[x], adj.av[x],
                          adj.aw[x], adj.aF[f], adj.part, adj.acc, st);
         adj.cur = x;
@@ -1036,6 +1039,8 @@ struct Ensemble : EnsBase {
         VOF_HIP(hipEventCreate(&e1));
         VOF_HIP(hipEventRecord(e0, st));
         VOF_HIP(hipMemsetAsync(adj.acc, 0, vof::ADJ_NSCALAR * sizeof(double), st));
+        // the predictor transposes of all steps add their work-group sums here
+        VOF_HIP(hipMemsetAsync(adj.part, 0, vof::adj_part_size(I.nx, I.ny, I.nz) * sizeof(double), st));
         adj.cur = 0;
         adj.fcur = 0;
         vof::adj_centroid(e.tab, I, adj.J, adj.sumF, gJ, adj.seed[0], adj.aF[0], st);
@@ -1086,6 +1091,7 @@ struct Ensemble : EnsBase {
                                      adj_tape(b, size_t(m))));
             VOF_HIP(hipEventRecord(adj.ev_rev[b], st_rev));
         }
+        vof::adj_predict_sums(I.nx, I.ny, I.nz, adj.part, adj.acc, st);
         VOF_TRY(load_state(adj_state(adj_final()), adj.f_fin, adj.p_fin));
         VOF_HIP(hipEventRecord(e1, st));
         double acc[vof::ADJ_NSCALAR];
