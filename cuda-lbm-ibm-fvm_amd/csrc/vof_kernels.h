@@ -70,9 +70,13 @@ inline int blocks_wcol(int nx, int ny, int nzrange, int kz, int h) {
 inline int blocks_jacobi2(int nx, int ny, int nz, int kz) {
     return ((nx + 59) / 60) * ((ny + 13) / 14) * ((nz + kz - 1) / kz);
 }
+// y/z line kernels: 256 lanes per work-group over the no*(nx+2) lines, the
+// count rounded up to a multiple of 8 so that under the round-robin XCD
+// dispatch the chunks of one line all run on the same XCD (their overlapping
+// windows hit its L2); one such row of work-groups per chunk of ch cells.
 inline int blocks_march(int nx, int nd, int no, int ch) {
     const int ntr = no * (nx + 2);
-    return ((ntr + 255) / 256) * ((nd + ch - 1) / ch);
+    return ((((ntr + 255) / 256) + 7) & ~7) * ((nd + ch - 1) / ch);
 }
 inline int blocks_ghost(int nx, int ny, int nz) {
     const int sx = nx + 2;

@@ -26,8 +26,14 @@ namespace vof {
 // Shared launch prologue: logical work-group, instance and local work-group.
 // The instance record is copied into (scalar) registers: buffer stores are
 // opaque to alias analysis, and a reference would be re-read after each one.
-#define VOF_PROLOGUE(R)                                                   \
-    const int b_ = xcd_remap(blockIdx.x, gridDim.x);                      \
+#define VOF_PROLOGUE(R) VOF_PROLOGUE_B(R, xcd_remap(blockIdx.x, gridDim.x))
+// Work-group b on XCD b % 8 (the dispatcher's own round robin), for kernels
+// whose work depends on the data: the FCT sweeps skip gas wave-uniformly, and
+// a contiguous range per XCD would give the XCDs that own the liquid region
+// all the limiter arithmetic.
+#define VOF_PROLOGUE_RR(R) VOF_PROLOGUE_B(R, (int)blockIdx.x)
+#define VOF_PROLOGUE_B(R, BLOCK)                                          \
+    const int b_ = (BLOCK);                                               \
     int lb = b_, inst_ = 0;                                               \
     Inst<R> I = ea.one;                                                   \
     if (ea.ninst != 1) {                                                  \
@@ -546,7 +552,7 @@ __device__ __forceinline__ void store_F(const Inst<R>& I, const Buf<R>& Fo, int 
 template <class R>
 __global__ __launch_bounds__(256) void k_fct_x(const EnsArg<R> ea,
                                                int src, int kz, int fin) {
-    VOF_PROLOGUE(R)
+    VOF_PROLOGUE_RR(R)
     WMap m;
     if (!wmap<R, 3>(I, lb, kz, 1, I.nz, m)) return;
     const int i = m.i, j = m.j;
@@ -701,7 +707,7 @@ __device__ __forceinline__ MarchLine march_uniform(const Inst<R>& I, int lb) {
     L.nd = (DIR == 1) ? I.ny : I.nz;
     L.S = (DIR == 1) ? I.sx : I.sy;
     const int no = (DIR == 1) ? I.nz : I.ny;
-    L.nbt = (no * I.sx + 255) >> 8;
+    L.nbt = (((no * I.sx + 255) >> 8) + 7) & ~7;  // multiple of 8: chunk neighbours share an XCD
     L.chn = lb / L.nbt;
     return L;
 }
@@ -734,7 +740,7 @@ __device__ __forceinline__ void march_put(const Inst<R>& I, const Buf<R>& Fo, co
 template <class R, int DIR>
 __global__ __launch_bounds__(256) void k_fct_march(const EnsArg<R> ea,
                                                    int src, int ch, int fin) {
-    VOF_PROLOGUE(R)
+    VOF_PROLOGUE_RR(R)
     MarchLine L = march_uniform<R, DIR>(I, lb);
     const int S = L.S, nd = L.nd, chn = L.chn;
     if (!march_lane<R, DIR>(I, lb, L)) return;
@@ -768,7 +774,7 @@ __global__ __launch_bounds__(256) void k_fct_march(const EnsArg<R> ea,
 template <class R, int DIR, int CH>
 __global__ __launch_bounds__(256) void k_fct_line(const EnsArg<R> ea,
                                                   int src, int fin) {
-    VOF_PROLOGUE(R)
+    VOF_PROLOGUE_RR(R)
     MarchLine L = march_uniform<R, DIR>(I, lb);
     const int S = L.S, nd = L.nd, chn = L.chn;
     if (!march_lane<R, DIR>(I, lb, L)) return;
