@@ -549,6 +549,54 @@ __device__ __forceinline__ void store_F(const Inst<R>& I, const Buf<R>& Fo, int 
 // halo 3: a wave stores its middle 58 lanes, whose whole +-3 stencil is inside
 // the wave); each wave walks kz planes.  Row ends need no care: ghost lanes
 // carry the reference's zero conventions, which decouple consecutive rows.
+//
+// One plane k of a wave, from F and u of the plane (every lane active, so that
+// each shuffle reads a live lane).
+template <class R>
+__device__ __forceinline__ void fct_x_plane(const Inst<R>& I, const Buf<R>& Fo, const R f0, const R u0, bool cell,
+                                            bool store, int i, int j, int k, int fin) {
+    if (__all(f0 == R(0))) {
+        if (store) store_F(I, Fo, i, j, k, R(0), fin != 0);
+        return;
+    }
+    const R fm = sh_up(f0);
+    const R up = sh_dn(u0);
+    const R vdt = u0 * I.dt;
+    const R L0 = (u0 >= R(0)) ? vdt * fm : vdt * f0;
+    const R H0 = (u0 <= R(0)) ? vdt * fm : vdt * f0;
+    const R a0 = H0 - L0;
+    const R L1 = sh_dn(L0);
+    const R a1 = sh_dn(a0);
+    const R dv = I.vol - I.dtdydz * (up - u0);
+    R ftd = R(0);
+    if (cell) {
+        R oo = (f0 + fct_stage1_scale<R, 0>(I, L0 - L1)) * I.vol / dv;
+        if (oo > R(1) || oo < R(0)) oo = clamp01(oo);
+        ftd = oo;
+    }
+    const R ftm = sh_up(ftd);
+    const R ftp = sh_dn(ftd);
+    R rp = R(0), rm = R(0);
+    if (cell) {
+        const R fmx = vmax(ftd, vmax(ftm, ftp));
+        const R fmn = vmin(ftd, vmin(ftm, ftp));
+        const R pp = vmax(R(0), a0) - vmin(R(0), a1);
+        if (pp > R(0)) rp = vmin(R(1), (fmx - ftd) * I.dx / pp);
+        const R pm = vmax(R(0), a1) - vmin(R(0), a0);
+        if (pm > R(0)) rm = vmin(R(1), (ftd - fmn) * I.dx / pm);
+    }
+    const R rp1 = sh_dn(rp);
+    const R rm1 = sh_dn(rm);
+    const R c1 = cell ? ((a1 >= R(0)) ? vmin(rp1, rm) : vmin(rp, rm1)) : R(0);
+    const R ac1 = a1 * c1;
+    const R ac0 = sh_up(ac1);
+    if (store) {
+        const R fn = clamp01(ftd - ((ac1 - ac0) / I.dy) * I.vol / dv);
+        store_F(I, Fo, i, j, k, fin ? clamp01(fn) : fn, fin != 0);
+    }
+}
+
+// Any chunk length: planes streamed with F and u fetched one plane ahead.
 template <class R>
 __global__ __launch_bounds__(256) void k_fct_x(const EnsArg<R> ea,
                                                int src, int kz, int fin) {
@@ -565,46 +613,42 @@ __global__ __launch_bounds__(256) void k_fct_x(const EnsArg<R> ea,
         const int on = min(k + 1, m.k1) * I.sy;
         nf = Fi.ld(m.tc, on);
         nu = u.ld(m.tc, on);
-        if (__all(f0 == R(0))) {
-            if (store) store_F(I, Fo, i, j, k, R(0), fin != 0);
-            continue;
-        }
-        const R fm = sh_up(f0);
-        const R up = sh_dn(u0);
-        const R vdt = u0 * I.dt;
-        const R L0 = (u0 >= R(0)) ? vdt * fm : vdt * f0;
-        const R H0 = (u0 <= R(0)) ? vdt * fm : vdt * f0;
-        const R a0 = H0 - L0;
-        const R L1 = sh_dn(L0);
-        const R a1 = sh_dn(a0);
-        const R dv = I.vol - I.dtdydz * (up - u0);
-        R ftd = R(0);
-        if (cell) {
-            R oo = (f0 + fct_stage1_scale<R, 0>(I, L0 - L1)) * I.vol / dv;
-            if (oo > R(1) || oo < R(0)) oo = clamp01(oo);
-            ftd = oo;
-        }
-        const R ftm = sh_up(ftd);
-        const R ftp = sh_dn(ftd);
-        R rp = R(0), rm = R(0);
-        if (cell) {
-            const R fmx = vmax(ftd, vmax(ftm, ftp));
-            const R fmn = vmin(ftd, vmin(ftm, ftp));
-            const R pp = vmax(R(0), a0) - vmin(R(0), a1);
-            if (pp > R(0)) rp = vmin(R(1), (fmx - ftd) * I.dx / pp);
-            const R pm = vmax(R(0), a1) - vmin(R(0), a0);
-            if (pm > R(0)) rm = vmin(R(1), (ftd - fmn) * I.dx / pm);
-        }
-        const R rp1 = sh_dn(rp);
-        const R rm1 = sh_dn(rm);
-        const R c1 = cell ? ((a1 >= R(0)) ? vmin(rp1, rm) : vmin(rp, rm1)) : R(0);
-        const R ac1 = a1 * c1;
-        const R ac0 = sh_up(ac1);
-        if (store) {
-            const R fn = clamp01(ftd - ((ac1 - ac0) / I.dy) * I.vol / dv);
-            store_F(I, Fo, i, j, k, fin ? clamp01(fn) : fn, fin != 0);
-        }
+        fct_x_plane(I, Fo, f0, u0, cell, store, i, j, k, fin);
     }
+}
+
+// Chunks of KZ planes: F of the whole chunk is loaded before the first plane,
+// and u only when F is nonzero somewhere on the wave's chunk (an all-gas chunk
+// stores zeros, which is what every one of its planes would store).
+template <class R, int KZ>
+__global__ __launch_bounds__(256) void k_fct_xb(const EnsArg<R> ea,
+                                                int src, int fin) {
+    VOF_PROLOGUE_RR(R)
+    WMap m;
+    if (!wmap<R, 3>(I, lb, KZ, 1, I.nz, m)) return;
+    const int i = m.i, j = m.j;
+    const bool cell = (m.t == m.tc) && i >= 1 && i <= I.nx && j >= 1 && j <= I.ny;
+    const bool store = cell && m.own;
+    const Buf<R> Fi(sel2(I.F, src), I.ntot), Fo(sel2(I.F, src ^ 1), I.ntot), u(I.u, I.ntot);
+    R fw[KZ], uw[KZ];
+#pragma unroll
+    for (int e = 0; e < KZ; ++e) fw[e] = Fi.ld(m.tc, min(m.k0 + e, m.k1) * I.sy);
+    bool gas = true;
+#pragma unroll
+    for (int e = 0; e < KZ; ++e) gas = gas && (fw[e] == R(0));
+    if (__all(gas)) {
+        if (store) {
+#pragma unroll
+            for (int e = 0; e < KZ; ++e)
+                if (m.k0 + e <= m.k1) store_F(I, Fo, i, j, m.k0 + e, R(0), fin != 0);
+        }
+        return;
+    }
+#pragma unroll
+    for (int e = 0; e < KZ; ++e) uw[e] = u.ld(m.tc, min(m.k0 + e, m.k1) * I.sy);
+#pragma unroll
+    for (int e = 0; e < KZ; ++e)
+        if (m.k0 + e <= m.k1) fct_x_plane(I, Fo, fw[e], uw[e], cell, store, i, j, m.k0 + e, fin);
 }
 
 // y and z sweeps: lanes run along x (coalesced), each lane marches along the
@@ -1099,9 +1143,15 @@ void launch_march(const Ens<R>& e, const Geo& g, int src, int fin, hipStream_t s
 }
 template <class R>
 void launch_fct(const Ens<R>& e, const Geo& g, int dir, int src, int fin, hipStream_t st) {
-    if (dir == 0)
-        hipLaunchKernelGGL(k_fct_x<R>, dim3(g.nblk), dim3(256), 0, st, ens_arg(e, g), src, g.kz, fin);
-    else if (dir == 1)
+    if (dir == 0) {
+        const dim3 grid(g.nblk), blk(256);
+        switch (g.kz) {  // chunks of 4, 8 or 16 planes: F (and u) preloaded
+            case 4: hipLaunchKernelGGL((k_fct_xb<R, 4>), grid, blk, 0, st, ens_arg(e, g), src, fin); break;
+            case 8: hipLaunchKernelGGL((k_fct_xb<R, 8>), grid, blk, 0, st, ens_arg(e, g), src, fin); break;
+            case 16: hipLaunchKernelGGL((k_fct_xb<R, 16>), grid, blk, 0, st, ens_arg(e, g), src, fin); break;
+            default: hipLaunchKernelGGL(k_fct_x<R>, grid, blk, 0, st, ens_arg(e, g), src, g.kz, fin);
+        }
+    } else if (dir == 1)
         launch_march<R, 1>(e, g, src, fin, st);
     else
         launch_march<R, 2>(e, g, src, fin, st);
