@@ -17,6 +17,8 @@
 //  * the Jacobi right-hand side is formed once per step;
 //  * each FCT direction is one fused kernel, the last one of a step also
 //    applies post_process_f and writes the F ghost layer.
+#include <algorithm>
+
 #include "vof_device.h"
 #include "vof_kernels.h"
 
@@ -414,21 +416,83 @@ __device__ __forceinline__ R corr(const Inst<R>& I, R st, R r0, R r1, R p0, R p1
     return st - I.dt / r * (p0 - p1) * hi;
 }
 
+// Per-lane constants of the correction's column mapping.
+struct CorrLane {
+    int i, j, ci, cj, ts, t;
+    bool iin, jin;
+};
+
+template <class R>
+__device__ __forceinline__ CorrLane corr_lane(const Inst<R>& I, const WMap& m) {
+    CorrLane L;
+    L.i = m.i;
+    L.j = m.j;
+    L.cj = min(max(m.j, 1), I.ny);
+    L.ci = min(max(m.i, 1), I.nx);
+    L.iin = (m.i >= 1 && m.i <= I.nx);
+    L.jin = (m.j >= 1 && m.j <= I.ny);
+    L.ts = L.cj * I.sx + m.i;  // load position: row clamped into the interior
+    L.t = m.t;
+    return L;
+}
+
+// Plane k (offset o) of the correction, from p, rho, u*, v*, w* of the plane at
+// the lane's row, p and rho one row below and one plane below; every lane of
+// the wave active (shuffles).
+template <class R>
+__device__ __forceinline__ void correct_plane(const Inst<R>& I, const WMap& m, const CorrLane& L, int k, int o, R pk,
+                                              R pjm, R rk, R rjm, R usk, R vsk, R wsk, R p_km, R r_km,
+                                              const Buf<R>& u, const Buf<R>& v, const Buf<R>& w) {
+    const int i = L.i, j = L.j, nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
+    const int ci = L.ci, cj = L.cj, ts = L.ts, t = L.t;
+    const bool iin = L.iin, jin = L.jin;
+    const R pim = sh_up(pk), rim = sh_up(rk);
+    const R uc = corr(I, usk, rk, rim, pk, pim, I.dxi);
+    const R vc = corr(I, vsk, rk, rjm, pk, pjm, I.dyi);
+    const R wc = corr(I, wsk, rk, r_km, pk, p_km, I.dzi);
+    const R v_r = sh_dn(vc), v_l = sh_up(vc), w_r = sh_dn(wc), w_l = sh_up(wc);
+    if (m.own) {
+        // u: walls at i = 1, nx+1; plane i = 0 keeps/copies its own values
+        R uval = R(0);
+        bool uw = true;
+        if (i >= 2 && i <= nx) uval = uc;
+        else if (i == 0) { uw = !jin; if (uw) uval = u.ld(ts, o); }
+        if (uw) u.st(uval, t, o);
+        // v: walls at j = 1, ny+1; row j = 0 keeps/copies; x ghosts copy the neighbour
+        R vval = R(0);
+        bool vw = true;
+        if (j >= 2 && j <= ny) vval = iin ? vc : (i == 0 ? v_r : v_l);
+        else if (j == 0) { vw = !iin; if (vw) vval = v.ld(ci, o); }
+        if (vw) v.st(vval, t, o);
+        // w: wall at k = 1
+        const R wval = (k == 1) ? R(0) : (iin ? wc : (i == 0 ? w_r : w_l));
+        w.st(wval, t, o);
+        if (k == 1) {  // ghost plane 0: u, v copy plane 1; w copies its own plane-0 interior
+            u.st((i == 0) ? u.ld(ts, sy) : uval, t, 0);
+            v.st((j == 0) ? v.ld(ci, sy) : vval, t, 0);
+            if (!(iin && jin)) w.st(w.ld(cj * sx + ci, 0), t, 0);
+        }
+        if (k == nz) {  // ghost plane nz+1: u, v copy plane nz; w wall
+            u.st((i == 0) ? u.ld(ts, o) : uval, t, o + sy);
+            v.st((j == 0) ? v.ld(ci, o) : vval, t, o + sy);
+            w.st(R(0), t, o + sy);
+        }
+    }
+}
+
+// Any chunk length: operands of plane k fetched one plane ahead.
 template <class R>
 __global__ __launch_bounds__(256) void k_correct(const EnsArg<R> ea,
                                                  int fsel, int psel, int kz) {
     VOF_PROLOGUE(R)
     WMap m;
     if (!wmap<R, 1>(I, lb, kz, 1, I.nz, m)) return;
-    const int i = m.i, j = m.j, nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
-    const int cj = min(max(j, 1), ny), ci = min(max(i, 1), nx);
-    const bool iin = (i >= 1 && i <= nx), jin = (j >= 1 && j <= ny);
-    const int ts = cj * sx + i;  // load position: row clamped into the interior
-    const int tsm = ts - sx;
+    const CorrLane L = corr_lane(I, m);
+    const int nz = I.nz, sx = I.sx, sy = I.sy;
+    const int ts = L.ts, tsm = ts - sx;
     const Buf<R> F(sel2(I.F, fsel), I.ntot), p(sel2(I.p, psel), I.ntot);
     const Buf<R> us(I.us, I.ntot), vs(I.vs, I.ntot), ws(I.ws, I.ntot);
     const Buf<R> u(I.u, I.ntot), v(I.v, I.ntot), w(I.w, I.ntot);
-    const int t = m.t;
     int o = m.k0 * sy;
     const int olim = nz * sy;
     R p_km = p.ld(ts, o - sy);
@@ -446,40 +510,47 @@ __global__ __launch_bounds__(256) void k_correct(const EnsArg<R> ea,
             n_f = F.ld(ts, o2); n_fjm = F.ld(tsm, o2);
             n_us = us.ld(ts, o2); n_vs = vs.ld(ts, o2); n_ws = ws.ld(ts, o2);
         }
-        const R pim = sh_up(pk), rim = sh_up(rk);
-        const R uc = corr(I, usk, rk, rim, pk, pim, I.dxi);
-        const R vc = corr(I, vsk, rk, rjm, pk, pjm, I.dyi);
-        const R wc = corr(I, wsk, rk, r_km, pk, p_km, I.dzi);
-        const R v_r = sh_dn(vc), v_l = sh_up(vc), w_r = sh_dn(wc), w_l = sh_up(wc);
-        if (m.own) {
-            // u: walls at i = 1, nx+1; plane i = 0 keeps/copies its own values
-            R uval = R(0);
-            bool uw = true;
-            if (i >= 2 && i <= nx) uval = uc;
-            else if (i == 0) { uw = !jin; if (uw) uval = u.ld(ts, o); }
-            if (uw) u.st(uval, t, o);
-            // v: walls at j = 1, ny+1; row j = 0 keeps/copies; x ghosts copy the neighbour
-            R vval = R(0);
-            bool vw = true;
-            if (j >= 2 && j <= ny) vval = iin ? vc : (i == 0 ? v_r : v_l);
-            else if (j == 0) { vw = !iin; if (vw) vval = v.ld(ci, o); }
-            if (vw) v.st(vval, t, o);
-            // w: wall at k = 1
-            const R wval = (k == 1) ? R(0) : (iin ? wc : (i == 0 ? w_r : w_l));
-            w.st(wval, t, o);
-            if (k == 1) {  // ghost plane 0: u, v copy plane 1; w copies its own plane-0 interior
-                u.st((i == 0) ? u.ld(ts, sy) : uval, t, 0);
-                v.st((j == 0) ? v.ld(ci, sy) : vval, t, 0);
-                if (!(iin && jin)) w.st(w.ld(cj * sx + ci, 0), t, 0);
-            }
-            if (k == nz) {  // ghost plane nz+1: u, v copy plane nz; w wall
-                u.st((i == 0) ? u.ld(ts, o) : uval, t, o + sy);
-                v.st((j == 0) ? v.ld(ci, o) : vval, t, o + sy);
-                w.st(R(0), t, o + sy);
-            }
-        }
+        correct_plane(I, m, L, k, o, pk, pjm, rk, rjm, usk, vsk, wsk, p_km, r_km, u, v, w);
         p_km = pk;
         r_km = rk;
+    }
+}
+
+// Chunks of KZ planes with every operand of the chunk (p and F on KZ+1 planes
+// of the lane's row, the rest on KZ planes) loaded before the first plane.
+template <class R, int KZ>
+__global__ __launch_bounds__(256) void k_correct_blk(const EnsArg<R> ea, int fsel, int psel) {
+    VOF_PROLOGUE(R)
+    WMap m;
+    if (!wmap<R, 1>(I, lb, KZ, 1, I.nz, m)) return;
+    const CorrLane L = corr_lane(I, m);
+    const int nz = I.nz, sx = I.sx, sy = I.sy, k0 = m.k0, k1 = m.k1;
+    const int ts = L.ts, tsm = ts - sx;
+    const Buf<R> F(sel2(I.F, fsel), I.ntot), p(sel2(I.p, psel), I.ntot);
+    const Buf<R> us(I.us, I.ntot), vs(I.vs, I.ntot), ws(I.ws, I.ntot);
+    const Buf<R> u(I.u, I.ntot), v(I.v, I.ntot), w(I.w, I.ntot);
+    R pc[KZ + 1], fc[KZ + 1], pjm[KZ], fjm[KZ], usw[KZ], vsw[KZ], wsw[KZ];  // pc, fc[e]: plane k0-1+e
+#pragma unroll
+    for (int e = 0; e < KZ + 1; ++e) {
+        const int o = min(k0 - 1 + e, nz) * sy;
+        pc[e] = p.ld(ts, o);
+        fc[e] = F.ld(ts, o);
+    }
+#pragma unroll
+    for (int e = 0; e < KZ; ++e) {
+        const int o = min(k0 + e, nz) * sy;
+        pjm[e] = p.ld(tsm, o);
+        fjm[e] = F.ld(tsm, o);
+        usw[e] = us.ld(ts, o);
+        vsw[e] = vs.ld(ts, o);
+        wsw[e] = ws.ld(ts, o);
+    }
+#pragma unroll
+    for (int e = 0; e < KZ; ++e) {
+        const int k = k0 + e;
+        if (k > k1) break;
+        correct_plane(I, m, L, k, k * sy, pc[e + 1], pjm[e], rho_of(I, fc[e + 1]), rho_of(I, fjm[e]), usw[e], vsw[e],
+                      wsw[e], pc[e], rho_of(I, fc[e]), u, v, w);
     }
 }
 
@@ -1083,15 +1154,19 @@ __global__ __launch_bounds__(256) void k_reduce2(const int* __restrict__ boff, c
 }
 
 // ---------------------------------------------------------------------------
-// Plain streaming copy (16 B per lane per access, grid-stride), used to
-// measure the attainable HBM bandwidth and to calibrate the byte counters.
+// Plain streaming copy (16 B per lane), used to measure the attainable HBM
+// bandwidth and to calibrate the byte counters.  One work-group per 4 KiB, so
+// every access of the copy is in flight from the start (a grid-stride loop
+// over a few thousand work-groups reaches only ~70% of the rate at 1 GiB).
 __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ a, float4* __restrict__ b, long n) {
     const long stride = (long)gridDim.x * blockDim.x;
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) b[t] = a[t];
 }
 
 void launch_copy(const void* a, void* b, long n16, hipStream_t st) {
-    hipLaunchKernelGGL(k_copy, dim3(256 * 32), dim3(256), 0, st, (const float4*)a, (float4*)b, n16);
+    const long nb = std::min<long>((n16 + 255) / 256, 1L << 30);
+    hipLaunchKernelGGL(k_copy, dim3(unsigned(std::max<long>(nb, 1))), dim3(256), 0, st, (const float4*)a, (float4*)b,
+                       n16);
 }
 
 // ---------------------------------------------------------------------------
@@ -1126,7 +1201,12 @@ void launch_jacobi2(const Ens<R>& e, const Geo& g, int psrc, hipStream_t st) {
 }
 template <class R>
 void launch_correct(const Ens<R>& e, const Geo& g, int fsel, int psel, hipStream_t st) {
-    hipLaunchKernelGGL(k_correct<R>, dim3(g.nblk), dim3(256), 0, st, ens_arg(e, g), fsel, psel, g.kz);
+    const dim3 grid(g.nblk), blk(256);
+    switch (g.kz) {  // chunks of 4 or 8 planes: operands preloaded
+        case 4: hipLaunchKernelGGL((k_correct_blk<R, 4>), grid, blk, 0, st, ens_arg(e, g), fsel, psel); break;
+        case 8: hipLaunchKernelGGL((k_correct_blk<R, 8>), grid, blk, 0, st, ens_arg(e, g), fsel, psel); break;
+        default: hipLaunchKernelGGL(k_correct<R>, grid, blk, 0, st, ens_arg(e, g), fsel, psel, g.kz);
+    }
 }
 // Chunks of 8, 16 or 32 cells use the register-blocked kernel; any other
 // length the streaming one.
