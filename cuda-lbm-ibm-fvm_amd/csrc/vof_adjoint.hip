@@ -182,12 +182,17 @@ __global__ __launch_bounds__(256) void k_centroid_adj(const Inst<R>* __restrict_
 // are neither read nor written: a ghost F only enters a flux through a wall
 // face, whose velocity is zero, so its adjoint is zero.  fin: the output went
 // through clamp01 a second time (last sweep of a step).
-template <class R, int DIR>
+//
+// The chunk length CH is a compile-time constant: F and V on the CH+13
+// positions the walk reads and aFo on its CH+12 output positions are loaded
+// into registers before the walk, so that every load of a wave is in flight at
+// once, and the walk is unrolled over the register windows.
+template <class R, int DIR, int CH>
 __global__ __launch_bounds__(256) void k_fct_adj(const Inst<R>* __restrict__ tab, const R* __restrict__ F,
                                                  const R* __restrict__ V, const double* __restrict__ aFo,
-                                                 double* __restrict__ aFi, double* __restrict__ aV, int ch,
-                                                 int fin) {
+                                                 double* __restrict__ aFi, double* __restrict__ aV, int fin) {
     static_assert(DIR == 1 || DIR == 2, "marches serve the y and z sweeps");
+    constexpr int ch = CH;
     const Inst<R>& I = tab[0];
     const int n = (DIR == 1) ? I.ny : I.nz;
     const int na = I.nx;  // lanes along x
@@ -215,13 +220,29 @@ __global__ __launch_bounds__(256) void k_fct_adj(const Inst<R>* __restrict__ tab
         return d * dy * dx / vol;
     };
     const double ds1 = (DIR == 1) ? dy / dxdy : dy * dx / vol;
-    auto ldF = [&](int m) { return (m >= 0 && m <= n + 1) ? double(F[base + m * S]) : 0.0; };
-    auto ldV = [&](int m) { return (m >= 0 && m <= n + 1) ? double(V[base + m * S]) : 0.0; };
-
     const int j0 = m0 - 6, j1 = m1 + 6;
+    // F, V at positions j0-1+e (zero outside 0..n+1); aFo at cell j0-3+e (zero
+    // outside 1..n).  Loads use positions clamped into range.
+    R fw[CH + 13], vw[CH + 13];
+    double gw[CH + 12];
+#pragma unroll
+    for (int e = 0; e < CH + 13; ++e) {
+        const int mm = j0 - 1 + e;
+        const int mc = min(max(mm, 0), n + 1);
+        const R fv = F[base + mc * S], vv = V[base + mc * S];
+        const bool in = (mm >= 0 && mm <= n + 1);
+        fw[e] = in ? fv : R(0);
+        vw[e] = in ? vv : R(0);
+    }
+#pragma unroll
+    for (int e = 0; e < CH + 12; ++e) {
+        const int c = j0 - 3 + e;
+        const double g = aFo[base + min(max(c, 1), n) * S];
+        gw[e] = (c >= 1 && c <= n) ? g : 0.0;
+    }
     // forward windows
-    double F0 = ldF(j0 - 1), F1 = 0, F2 = 0, F3 = 0, F4 = 0, F5 = 0, F6 = 0;
-    double V0 = ldV(j0 - 1), V1 = 0, V2 = 0, V3 = 0, V4 = 0, V5 = 0;
+    double F0 = double(fw[0]), F1 = 0, F2 = 0, F3 = 0, F4 = 0, F5 = 0, F6 = 0;
+    double V0 = double(vw[0]), V1 = 0, V2 = 0, V3 = 0, V4 = 0, V5 = 0;
     double L0 = 0, L1 = 0;
     double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
     double dv1 = 1, dv2 = 1, dv3 = 1, dv4 = 1, dv5 = 1;
@@ -238,10 +259,13 @@ __global__ __launch_bounds__(256) void k_fct_adj(const Inst<R>* __restrict__ tab
     double GL4 = 0, GL5 = 0;            // L (stage-1 part)
     double GF5 = 0, GF6 = 0;            // F
     double GV4 = 0, GV5 = 0;            // V
-    for (int j = j0; j <= j1; ++j) {
+#pragma unroll
+    for (int e = 0; e < CH + 12; ++e) {
+        const int j = j0 + e;
+        if (j > j1) continue;  // short last chunk (wave-uniform)
         // ---- advance the windows by one position
-        F6 = F5; F5 = F4; F4 = F3; F3 = F2; F2 = F1; F1 = F0; F0 = ldF(j);
-        V5 = V4; V4 = V3; V3 = V2; V2 = V1; V1 = V0; V0 = ldV(j);
+        F6 = F5; F5 = F4; F4 = F3; F3 = F2; F2 = F1; F1 = F0; F0 = double(fw[e + 1]);
+        V5 = V4; V4 = V3; V3 = V2; V2 = V1; V1 = V0; V0 = double(vw[e + 1]);
         L1 = L0;
         a4 = a3; a3 = a2; a2 = a1; a1 = a0;
         dv5 = dv4; dv4 = dv3; dv3 = dv2; dv2 = dv1;
@@ -289,7 +313,7 @@ __global__ __launch_bounds__(256) void k_fct_adj(const Inst<R>* __restrict__ tab
         if (j - 3 >= 1 && j - 3 <= n) {
             const double num = (a2 * C2 - a3 * C3) / s4;
             const double yv = T3 - num * vol / dv3;
-            double g = aFo[base + (j - 3) * S] * dclamp(yv);
+            double g = gw[e] * dclamp(yv);
             if (fin) g *= dclamp(clamp01d(yv));
             if (g != 0.0) {
                 const double k = vol / (s4 * dv3);
@@ -1066,17 +1090,16 @@ void adj_fct(const Inst<R>* tab, const Inst<R>& h, int dir, const R* F, const R*
                            aV, kz, fin);
         return;
     }
-    // y and z: marches with lanes along x; chunk length 16 cells, 8 when that
-    // leaves too few work-groups to fill the chip (shorter chunks cost more in
-    // the 12 warm-up positions each chunk recomputes than they gain in waves)
+    // y and z: marches with lanes along x in chunks of 8 cells (a chunk's
+    // windows, 21 positions of F and V and 20 output adjoints, stay in
+    // registers; 16-cell chunks would spill them)
     const int n = (dir == 1) ? h.ny : h.nz;
     const long lines = (dir == 1) ? long(h.nx) * h.nz : long(h.nx) * h.ny;
     const int nbt = adj_blocks(lines);
-    int ch = 16;
-    if (long(nbt) * ((n + ch - 1) / ch) < 1024) ch = 8;
+    constexpr int ch = 8;
     const dim3 grid(nbt * ((n + ch - 1) / ch));
-    if (dir == 1) hipLaunchKernelGGL((k_fct_adj<R, 1>), grid, dim3(256), 0, st, tab, F, V, aFo, aFi, aV, ch, fin);
-    else hipLaunchKernelGGL((k_fct_adj<R, 2>), grid, dim3(256), 0, st, tab, F, V, aFo, aFi, aV, ch, fin);
+    if (dir == 1) hipLaunchKernelGGL((k_fct_adj<R, 1, ch>), grid, dim3(256), 0, st, tab, F, V, aFo, aFi, aV, fin);
+    else hipLaunchKernelGGL((k_fct_adj<R, 2, ch>), grid, dim3(256), 0, st, tab, F, V, aFo, aFi, aV, fin);
 }
 template <class R>
 void adj_correct(const Inst<R>* tab, const Inst<R>& h, const R* F, const R* p, const double* au, const double* av,
