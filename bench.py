@@ -426,19 +426,25 @@ def main():
     _, stages = s.time_steps(istep, 0, st_steps, stages=True)
     istep += st_steps
     per_step = {k: v / st_steps for k, v in stages.items()}
+    tuning = s.get_tuning()
+    # With two sweeps per launch (jacobi_tblock 2) a Jacobi launch still reads p
+    # and rhs and writes p once: 12 B/cell per launch, 5 launches per step.
+    launches = dict(LAUNCHES_PER_STEP)
+    sweeps_per_launch = 2 if tuning.get("jacobi_tblock", 1) == 2 else 1
+    launches["jacobi"] = LAUNCHES_PER_STEP["jacobi"] // sweeps_per_launch
     kern = {}
     for k, b in STAGE_BYTES.items():
-        t_launch = per_step[k] / LAUNCHES_PER_STEP[k] * 1e-3
+        t_launch = per_step[k] / launches[k] * 1e-3
         if t_launch <= 0:
             continue
         byts = b * (esz // 4) * cells
         kern[k] = {"ms_per_launch": t_launch * 1e3, "bytes_per_launch": byts, "gbs": byts / t_launch / 1e9,
                    "frac": byts / t_launch / 1e9 / HBM_PEAK_GBS}
+    kern["jacobi"]["sweeps_per_launch"] = sweeps_per_launch
     dom = max(kern, key=lambda k: per_step[k])
     d = kern[dom]
-    step_bytes = sum(STAGE_BYTES[k] * LAUNCHES_PER_STEP[k] for k in STAGE_BYTES) * (esz // 4) * cells
+    step_bytes = sum(STAGE_BYTES[k] * launches[k] for k in STAGE_BYTES) * (esz // 4) * cells
     diag = s.diagnostics()
-    tuning = s.get_tuning()
     pcie = transfer_leg(s, cells, ms_step)
     s.close()
     copy_hbm = vof.copy_bandwidth(1 << 30, 10)
@@ -475,6 +481,7 @@ def main():
             "traffic": (load_traffic(dom) if (n == 200 and prec == 4) else None),
             "bytes_per_launch": d["bytes_per_launch"],
             "ms_per_launch": d["ms_per_launch"],
+            "sweeps_per_launch": d.get("sweeps_per_launch"),
             "attainable_copy_gbs": {"hbm_1GiB": round(copy_hbm, 1), "mall_64MiB": round(copy_mall, 1)},
         },
         "tuning": tuning,
