@@ -427,8 +427,9 @@ def main():
     istep += st_steps
     per_step = {k: v / st_steps for k, v in stages.items()}
     tuning = s.get_tuning()
-    # With two sweeps per launch (jacobi_tblock 2) a Jacobi launch still reads p
-    # and rhs and writes p once: 12 B/cell per launch, 5 launches per step.
+    # With two sweeps per launch (jacobi_tblock 2, temporal blocking) a launch
+    # performs two sweeps of 12 algorithmic B/cell each: the "effective" figure
+    # of the roofline, beside the counter bytes (about half of it).
     launches = dict(LAUNCHES_PER_STEP)
     sweeps_per_launch = 2 if tuning.get("jacobi_tblock", 1) == 2 else 1
     launches["jacobi"] = LAUNCHES_PER_STEP["jacobi"] // sweeps_per_launch
@@ -437,13 +438,13 @@ def main():
         t_launch = per_step[k] / launches[k] * 1e-3
         if t_launch <= 0:
             continue
-        byts = b * (esz // 4) * cells
+        byts = b * (sweeps_per_launch if k == "jacobi" else 1) * (esz // 4) * cells
         kern[k] = {"ms_per_launch": t_launch * 1e3, "bytes_per_launch": byts, "gbs": byts / t_launch / 1e9,
                    "frac": byts / t_launch / 1e9 / HBM_PEAK_GBS}
     kern["jacobi"]["sweeps_per_launch"] = sweeps_per_launch
     dom = max(kern, key=lambda k: per_step[k])
     d = kern[dom]
-    step_bytes = sum(STAGE_BYTES[k] * launches[k] for k in STAGE_BYTES) * (esz // 4) * cells
+    step_bytes = sum(STAGE_BYTES[k] * LAUNCHES_PER_STEP[k] for k in STAGE_BYTES) * (esz // 4) * cells
     diag = s.diagnostics()
     pcie = transfer_leg(s, cells, ms_step)
     s.close()

@@ -330,12 +330,29 @@ __global__ __launch_bounds__(256) void k_jacobi_blk(const EnsArg<R> ea, int psrc
 // order, so two launches of this kernel equal four single sweeps bit for bit.
 constexpr int kJ2Rows = 16, kJ2OutRows = 14, kJ2OutLanes = 60;
 
+// One point of a sweep; ap = -(ae+aw+an+as+af+ab) in that order, formed once
+// per lane for each kind of plane (jac_ap).
 template <class R>
 __device__ __forceinline__ R jac_point(R b, R ae, R aw, R an, R as, R af, R ab, R pe, R pw, R pn, R ps, R pf,
-                                       R pb) {
-    const R ap = -(ae + aw + an + as + af + ab);
+                                       R pb, R ap) {
     return (b - ae * pe - aw * pw - an * pn - as * ps - af * pf - ab * pb) / ap;
 }
+
+// ap of a lane on the four kinds of plane (interior, k = 1, k = nz, nz = 1),
+// summed in the reference's order (:430) with the zero coefficients in place.
+template <class R>
+struct JacAp {
+    R in, lo, hi, one;
+    __device__ __forceinline__ JacAp(R ae, R aw, R an, R as, R azz, int nz) {
+        in = -(ae + aw + an + as + azz + azz);
+        lo = -(ae + aw + an + as + ((nz != 1) ? azz : R(0)) + R(0));
+        hi = -(ae + aw + an + as + R(0) + ((nz != 1) ? azz : R(0)));
+        one = -(ae + aw + an + as + R(0) + R(0));
+    }
+    __device__ __forceinline__ R at(int s, int nz) const {
+        return (s != 1 && s != nz) ? in : (nz == 1 ? one : (s == 1 ? lo : hi));
+    }
+};
 
 template <class R>
 __global__ __launch_bounds__(1024) void k_jacobi2(const EnsArg<R> ea,
@@ -361,6 +378,7 @@ __global__ __launch_bounds__(1024) void k_jacobi2(const EnsArg<R> ea,
     const R ae = (i != nx) ? I.axx : R(0), aw = (i != 1) ? I.axx : R(0);
     const R an = (j != ny) ? I.ayy : R(0), as = (j != 1) ? I.ayy : R(0);
     const R azz = I.azz;
+    const JacAp<R> apk(ae, aw, an, as, azz, nz);
     const int olast = (nz + 1) * sy;
     auto po = [&](int s) { return min(max(s, 0) * sy, olast); };  // clamped plane offset
     // march state: sweep-0 column at s-1, s; sweep-1 column at s-2, s-1; rhs at s-1
@@ -379,7 +397,7 @@ __global__ __launch_bounds__(1024) void k_jacobi2(const EnsArg<R> ea,
         R q1 = R(0);
         if (in1 && s >= 1 && s <= nz) {
             const R af = (s != nz) ? azz : R(0), ab = (s != 1) ? azz : R(0);
-            q1 = jac_point(b, ae, aw, an, as, af, ab, q0e, q0w, q0jp, q0jm, q0p, q0m);
+            q1 = jac_point(b, ae, aw, an, as, af, ab, q0e, q0w, q0jp, q0jm, q0p, q0m, apk.at(s, nz));
         }
         const int slot = s % 3, pslot = (s + 2) % 3;
         ring[slot][w][lane] = q1;
@@ -390,11 +408,83 @@ __global__ __launch_bounds__(1024) void k_jacobi2(const EnsArg<R> ea,
         if (st2 && s2 >= k0) {
             const R q1s = ring[pslot][w - 1][lane], q1n = ring[pslot][w + 1][lane];
             const R af = (s2 != nz) ? azz : R(0), ab = (s2 != 1) ? azz : R(0);
-            pt.st(jac_point(bm, ae, aw, an, as, af, ab, q1e, q1w, q1n, q1s, q1, q1mm), tc, s2 * sy);
+            pt.st(jac_point(bm, ae, aw, an, as, af, ab, q1e, q1w, q1n, q1s, q1, q1mm, apk.at(s2, nz)), tc, s2 * sy);
         }
         q0m = q0; q0 = q0p;
         q1mm = q1m; q1m = q1;
         bm = b;
+    }
+}
+
+// The same two sweeps for z-chunks of KZ planes, with every operand of the
+// chunk loaded before the march (p on the KZ+4 planes of the lane's column,
+// p on the two neighbouring rows and rhs on KZ+2 planes): the march then waits
+// on memory once per chunk instead of once per plane between its barriers.
+// The chunk bounds are uniform over the work-group, so every wave reaches
+// every barrier.
+template <class R, int KZ>
+__global__ __launch_bounds__(1024) void k_jacobi2_blk(const EnsArg<R> ea, int psrc) {
+    VOF_PROLOGUE(R)
+    __shared__ R ring[3][kJ2Rows][64];
+    const int nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
+    const int nxt = (nx + kJ2OutLanes - 1) / kJ2OutLanes;
+    const int nbd = (ny + kJ2OutRows - 1) / kJ2OutRows;
+    const int per_z = nxt * nbd;
+    const int zc = lb / per_z, rem = lb - zc * per_z;
+    const int bd = rem / nxt, xt = rem - bd * nxt;
+    const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+    const int i = kJ2OutLanes * xt - 1 + lane;
+    const int j = kJ2OutRows * bd + w;
+    const int k0 = 1 + zc * KZ, k1 = min(nz, k0 + KZ - 1);
+    const bool in1 = (i >= 1 && i <= nx && j >= 1 && j <= ny);
+    const bool st2 = in1 && lane >= 2 && lane <= 61 && w >= 1 && w <= kJ2OutRows;
+    const int ic = min(max(i, 0), nx + 1), jc = min(max(j, 0), ny + 1);
+    const int tc = jc * sx + ic;
+    const int tjm = max(jc - 1, 0) * sx + ic, tjp = min(jc + 1, ny + 1) * sx + ic;
+    const Buf<R> p(sel2(I.p, psrc), I.ntot), pt(sel2(I.p, psrc ^ 1), I.ntot), rhs(I.rhs, I.ntot);
+    const R ae = (i != nx) ? I.axx : R(0), aw = (i != 1) ? I.axx : R(0);
+    const R an = (j != ny) ? I.ayy : R(0), as = (j != 1) ? I.ayy : R(0);
+    const R azz = I.azz;
+    const JacAp<R> apk(ae, aw, an, as, azz, nz);
+    const int olast = (nz + 1) * sy;
+    auto po = [&](int s) { return min(max(s, 0) * sy, olast); };  // clamped plane offset
+    R pc[KZ + 4];                          // column, planes k0-2+e
+    R pjm[KZ + 2], pjp[KZ + 2], bb[KZ + 2];  // planes k0-1+e
+#pragma unroll
+    for (int e = 0; e < KZ + 4; ++e) pc[e] = p.ld(tc, po(k0 - 2 + e));
+#pragma unroll
+    for (int e = 0; e < KZ + 2; ++e) {
+        const int o = po(k0 - 1 + e);
+        pjm[e] = p.ld(tjm, o);
+        pjp[e] = p.ld(tjp, o);
+        bb[e] = rhs.ld(tc, o);
+    }
+    // sweep 1 at plane s, sweep 2 at plane s-1, for s = k0-1 .. k1+1
+    R q1mm = R(0), q1m = R(0);
+#pragma unroll
+    for (int e = 0; e < KZ + 2; ++e) {
+        const int s = k0 - 1 + e;
+        if (s > k1 + 1) break;  // uniform over the work-group
+        const R q0 = pc[e + 1];
+        const R q0w = sh_up(q0), q0e = sh_dn(q0);
+        R q1 = R(0);
+        if (in1 && s >= 1 && s <= nz) {
+            const R af = (s != nz) ? azz : R(0), ab = (s != 1) ? azz : R(0);
+            q1 = jac_point(bb[e], ae, aw, an, as, af, ab, q0e, q0w, pjp[e], pjm[e], pc[e + 2], pc[e], apk.at(s, nz));
+        }
+        const int slot = s % 3, pslot = (s + 2) % 3;
+        ring[slot][w][lane] = q1;
+        __syncthreads();
+        const int s2 = s - 1;
+        const R q1w = sh_up(q1m), q1e = sh_dn(q1m);
+        if (e >= 1 && st2 && s2 >= k0) {
+            const R q1s = ring[pslot][w - 1][lane], q1n = ring[pslot][w + 1][lane];
+            const R af = (s2 != nz) ? azz : R(0), ab = (s2 != 1) ? azz : R(0);
+            pt.st(jac_point(bb[e - 1], ae, aw, an, as, af, ab, q1e, q1w, q1n, q1s, q1, q1mm, apk.at(s2, nz)), tc,
+                  s2 * sy);
+        }
+        q1mm = q1m;
+        q1m = q1;
     }
 }
 
@@ -1197,7 +1287,19 @@ void launch_jacobi(const Ens<R>& e, const Geo& g, int psrc, hipStream_t st) {
 }
 template <class R>
 void launch_jacobi2(const Ens<R>& e, const Geo& g, int psrc, hipStream_t st) {
-    hipLaunchKernelGGL(k_jacobi2<R>, dim3(g.nblk), dim3(1024), 0, st, ens_arg(e, g), psrc, g.kz);
+    const dim3 grid(g.nblk), blk(1024);
+    // chunks of 4, 8 (and, in FP32, 16) planes: operands preloaded; 16 FP64
+    // planes would exceed the 128 VGPRs of a 1024-lane work-group
+    if (g.kz == 4) {
+        hipLaunchKernelGGL((k_jacobi2_blk<R, 4>), grid, blk, 0, st, ens_arg(e, g), psrc);
+    } else if (g.kz == 8) {
+        hipLaunchKernelGGL((k_jacobi2_blk<R, 8>), grid, blk, 0, st, ens_arg(e, g), psrc);
+    } else if (g.kz == 16 && sizeof(R) == 4) {
+        if constexpr (sizeof(R) == 4)
+            hipLaunchKernelGGL((k_jacobi2_blk<R, 16>), grid, blk, 0, st, ens_arg(e, g), psrc);
+    } else {
+        hipLaunchKernelGGL(k_jacobi2<R>, grid, blk, 0, st, ens_arg(e, g), psrc, g.kz);
+    }
 }
 template <class R>
 void launch_correct(const Ens<R>& e, const Geo& g, int fsel, int psel, hipStream_t st) {
