@@ -181,7 +181,8 @@ def test_launch_shapes_match_oracle(vofmod, orc, prec, shape):
 @pytest.mark.parametrize("kz", [3, 4, 8, 16])
 def test_jacobi2_tiles_and_chunks(vofmod, orc, kz, prec):
     """Two-sweep launches on a grid with several x tiles, row bands and z chunks
-    (partial last chunks); kz 3 streams the planes, 4, 8 and 16 preload them."""
+    (partial last chunks); kz 3 and FP64 stream the planes, FP32 with 4, 8 and
+    16 preloads them."""
     print(f"two-sweep jacobi kz={kz} prec={prec}", flush=True)
     s, o = make_pair(vofmod, orc, nx=131, ny=33, nz=21, precision=prec, dt=1e-3)
     s.set_tuning(kz_jacobi=kz)
