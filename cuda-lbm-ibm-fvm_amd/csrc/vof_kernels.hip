@@ -217,6 +217,41 @@ __global__ __launch_bounds__(256) void k_rhs(const EnsArg<R> ea,
     }
 }
 
+// The same for z-chunks of KZ planes with the chunk's operands (u*, v* on the
+// lane's and the next row, F on KZ planes, w* on KZ+1) loaded first.
+template <class R, int KZ>
+__global__ __launch_bounds__(256) void k_rhs_blk(const EnsArg<R> ea, int fsel) {
+    VOF_PROLOGUE(R)
+    WMap m;
+    if (!wmap<R, 1>(I, lb, KZ, 1, I.nz, m)) return;
+    const int i = m.i, j = m.j, sx = I.sx, sy = I.sy, nz = I.nz, k0 = m.k0, k1 = m.k1;
+    const int P = sx * (I.ny + 2);
+    const bool out = m.own && i >= 1 && i <= I.nx && j >= 1 && j <= I.ny;
+    const int tc = m.tc, tjp = min(tc + sx, P - 1);
+    const Buf<R> F(sel2(I.F, fsel), I.ntot), us(I.us, I.ntot), vs(I.vs, I.ntot), ws(I.ws, I.ntot), rhs(I.rhs, I.ntot);
+    R usw[KZ], vsw[KZ], vpw[KZ], fw[KZ], wsw[KZ + 1];  // planes k0+e
+#pragma unroll
+    for (int e = 0; e < KZ; ++e) {
+        const int o = min(k0 + e, nz) * sy;
+        usw[e] = us.ld(tc, o);
+        vsw[e] = vs.ld(tc, o);
+        vpw[e] = vs.ld(tjp, o);
+        fw[e] = F.ld(tc, o);
+    }
+#pragma unroll
+    for (int e = 0; e < KZ + 1; ++e) wsw[e] = ws.ld(tc, min(k0 + e, nz + 1) * sy);
+#pragma unroll
+    for (int e = 0; e < KZ; ++e) {
+        const int k = k0 + e;
+        if (k > k1) break;
+        const R usp = sh_dn(usw[e]);
+        if (out)
+            rhs.st(rho_of(I, fw[e]) / I.dt *
+                       ((usp - usw[e]) * I.dxi + (vpw[e] - vsw[e]) * I.dyi + (wsw[e + 1] - wsw[e]) * I.dzi),
+                   tc, k * sy);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // One Jacobi sweep (solve_p_jacobi_kernel, :401-439) with the precomputed rhs.
 // Boundary neighbours carry a zero coefficient times the (zero) ghost p, kept
@@ -1273,7 +1308,12 @@ void launch_predict(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st) {
 }
 template <class R>
 void launch_rhs(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st) {
-    hipLaunchKernelGGL(k_rhs<R>, dim3(g.nblk), dim3(256), 0, st, ens_arg(e, g), fsel, g.kz);
+    const dim3 grid(g.nblk), blk(256);
+    switch (g.kz) {  // chunks of 4 or 8 planes: operands preloaded
+        case 4: hipLaunchKernelGGL((k_rhs_blk<R, 4>), grid, blk, 0, st, ens_arg(e, g), fsel); break;
+        case 8: hipLaunchKernelGGL((k_rhs_blk<R, 8>), grid, blk, 0, st, ens_arg(e, g), fsel); break;
+        default: hipLaunchKernelGGL(k_rhs<R>, grid, blk, 0, st, ens_arg(e, g), fsel, g.kz);
+    }
 }
 template <class R>
 void launch_jacobi(const Ens<R>& e, const Geo& g, int psrc, hipStream_t st) {
