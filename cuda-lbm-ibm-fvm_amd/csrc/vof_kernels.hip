@@ -13,10 +13,16 @@
 //    z-neighbours carried in registers; x-neighbours come from the adjacent
 //    lane (wave shuffles) in overlapped waves of 64 lanes that store the middle
 //    64-2H; y-neighbours are one extra coalesced row load;
+//  * most kernels have a register-blocked form for chunks of 4/8/16 planes or
+//    cells: every load of a chunk is issued before its first point is formed;
 //  * derived fields (rho, nu) are recomputed from F in registers;
-//  * the Jacobi right-hand side is formed once per step;
+//  * the Jacobi right-hand side is formed once per step, and two sweeps can
+//    share a launch (LDS ring between the rows of a 16-row tile);
 //  * each FCT direction is one fused kernel, the last one of a step also
-//    applies post_process_f and writes the F ghost layer.
+//    applies post_process_f and writes the F ghost layer;
+//  * work-groups go to XCDs in contiguous ranges (neighbouring tiles share an
+//    L2), except in the FCT sweeps, whose gas skip makes the work depend on
+//    the data and which use the dispatcher's round robin.
 #include <algorithm>
 
 #include "vof_device.h"
