@@ -262,13 +262,20 @@ def test_remove_p_mean(vofmod, orc):
     assert abs(s.diagnostics()["p_mean"]) < 1e-9
 
 
-def test_batch_equals_plain(vofmod):
-    """Instances of a ragged batch reproduce plain runs bit for bit."""
+@pytest.mark.parametrize("tuning", [None, dict(jacobi_tblock=2, kz_jacobi=8, kz_correct=8, kz_rhs=8, kz_fct_x=8),
+                                    dict(jacobi_tblock=2, kz_jacobi=3, fct_chunk=16, kz_fct_x=16, kz_predict=4)])
+def test_batch_equals_plain(vofmod, tuning):
+    """Instances of a ragged batch reproduce plain runs bit for bit, also when
+    the batch runs other launch shapes (two-sweep Jacobi launches, preloaded or
+    streamed, other chunk lengths) than the plain runs' defaults."""
+    print(f"batch vs plain, batch tuning {tuning}", flush=True)
     sizes = [(12, 12, 12), (20, 16, 18), (9, 14, 11)]
     prms = [vofmod.default_params(nx=a, ny=b, nz=c, g=(0, 0, -9.81), rho_g=25.0 + 10 * i, dt=2e-5)
             for i, (a, b, c) in enumerate(sizes)]
     boxes = [(0.04, 0.05, 0.06), (0.03, 0.04, 0.07), (0.05, 0.02, 0.05)]
     bt = vofmod.Batch(prms)
+    if tuning:
+        bt.set_tuning(**tuning)
     bt.init_box(*zip(*boxes))
     bt.step(1, 11, graph=True)
     for i, prm in enumerate(prms):
