@@ -68,14 +68,18 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 // (base, size in bytes): the per-lane offset is a 32-bit VGPR that stays
 // fixed while a lane marches, and the plane offset is a scalar (SGPR)
 // operand, so a load or store costs no per-lane 64-bit address arithmetic.
-// Accesses outside [0, size) return 0 / are dropped by the hardware.
+// Accesses outside [0, size) return 0 / are dropped by the hardware.  Sizes
+// and byte offsets are 32-bit unsigned quantities (a field of up to 4 GiB;
+// the host rejects larger grids), formed in unsigned arithmetic.
 constexpr int kRsrcWord3 = 0x00020000;  // raw, untyped 32-bit records (gfx9 family)
+
+__device__ __forceinline__ int byte_off(int elems, unsigned esz) { return (int)((unsigned)elems * esz); }
 
 template <class R>
 struct Buf {
     __amdgpu_buffer_rsrc_t r;
     __device__ __forceinline__ Buf(const R* p, int n)
-        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<R*>(p), (short)0, n * (int)sizeof(R), kRsrcWord3)) {}
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<R*>(p), (short)0, byte_off(n, sizeof(R)), kRsrcWord3)) {}
     // vo: per-lane element offset, so: wave-uniform element offset
     __device__ __forceinline__ R ld(int vo, int so) const;
     __device__ __forceinline__ void st(R v, int vo, int so) const;
@@ -83,15 +87,15 @@ struct Buf {
 
 template <>
 __device__ __forceinline__ float Buf<float>::ld(int vo, int so) const {
-    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, vo * 4, so * 4, 0));
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, byte_off(vo, 4u), byte_off(so, 4u), 0));
 }
 template <>
 __device__ __forceinline__ void Buf<float>::st(float v, int vo, int so) const {
-    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, vo * 4, so * 4, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, byte_off(vo, 4u), byte_off(so, 4u), 0);
 }
 template <>
 __device__ __forceinline__ double Buf<double>::ld(int vo, int so) const {
-    const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, vo * 8, so * 8, 0);
+    const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, byte_off(vo, 8u), byte_off(so, 8u), 0);
     return __hiloint2double((int)x[1], (int)x[0]);
 }
 template <>
@@ -99,7 +103,7 @@ __device__ __forceinline__ void Buf<double>::st(double v, int vo, int so) const 
     decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)) x;
     x[0] = (unsigned)__double2loint(v);
     x[1] = (unsigned)__double2hiint(v);
-    __builtin_amdgcn_raw_buffer_store_b64(x, r, vo * 8, so * 8, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(x, r, byte_off(vo, 8u), byte_off(so, 8u), 0);
 }
 
 // ---------------------------------------------------------------------------

@@ -1149,6 +1149,10 @@ vof_status validate(const vof_params& p) {
     const double ntot = double(p.nx + 2) * (p.ny + 2) * (p.nz + 2);
     if (ntot >= 2147483647.0) { set_err("grid too large: %.0f stored cells exceed 32-bit indexing", ntot); return VOF_ERR_INVALID; }
     if (p.precision != 4 && p.precision != 8) { set_err("precision must be 4 or 8, got %d", p.precision); return VOF_ERR_INVALID; }
+    if (ntot * p.precision > 4294967295.0) {  // kernels address each field through one 32-bit buffer resource
+        set_err("grid too large: a field of %.0f bytes exceeds the 4 GiB one buffer resource addresses", ntot * p.precision);
+        return VOF_ERR_INVALID;
+    }
     if (p.jacobi_sweeps < 0) { set_err("jacobi_sweeps must be >= 0"); return VOF_ERR_INVALID; }
     if (!(p.dt > 0.0) || !(p.Lx > 0.0) || !(p.Ly > 0.0) || !(p.Lz > 0.0)) { set_err("dt and L must be positive"); return VOF_ERR_INVALID; }
     if (p.flags & (VOF_FLAG_RHO_PREDICTOR | VOF_FLAG_NO_SLIP)) {

@@ -45,7 +45,9 @@ def test_default_params_are_the_reference_defaults(vofmod):
 
 
 @pytest.mark.parametrize("bad", [dict(nx=0), dict(precision=3), dict(dt=-1.0), dict(jacobi_sweeps=-1),
-                                 dict(nx=2000, ny=2000, nz=2000)])
+                                 dict(nx=2000, ny=2000, nz=2000),
+                                 dict(nx=1100, ny=1100, nz=1100, precision=4),   # < 2^31 cells, > 4 GiB per field
+                                 dict(nx=900, ny=900, nz=900, precision=8)])
 def test_invalid_parameters_rejected_before_device_use(vofmod, bad):
     with pytest.raises(vofmod.VofError) as e:
         vofmod.Solver(vofmod.default_params(**bad))

@@ -11,6 +11,7 @@
 * The drop-in executable on the reference's default case (200^3, -ic 1): the
   VTK file it writes is byte-identical to the oracle's field written by the
   reference's writer.
+* A 660^3 FP64 grid (fields beyond 2 GiB): one step bitwise against the oracle.
 """
 import os
 import subprocess
@@ -47,6 +48,20 @@ def test_reference_case_64_fp32_2000_steps_bitwise(vofmod, orc):
     s, o = _pair(vofmod, orc, dict(n=64, precision=4), lambda x: x.init_ic(1))
     s.step(1, 2000, graph=True)
     o.step(1, 2000)
+    for f in ("F", "U", "V", "W", "P"):
+        assert_same(s.download(f), o.get(f), f)
+    s.close()
+    o.close()
+
+
+def test_fields_beyond_2gib_fp64_one_step_bitwise(vofmod, orc):
+    """660^3 in FP64: every field is 2.3 GB, so the kernels' byte offsets pass
+    2^31 (they are 32-bit unsigned values); one dam-break step is bitwise
+    equal to the oracle's."""
+    print("660^3 FP64 dam break, 1 step (fields beyond 2 GiB)", flush=True)
+    s, o = _pair(vofmod, orc, dict(n=660, precision=8, g=(0.0, 0.0, -9.81)), lambda x: x.init_box(0.04, 0.04, 0.06))
+    s.step(1, 1)
+    o.step(1, 1)
     for f in ("F", "U", "V", "W", "P"):
         assert_same(s.download(f), o.get(f), f)
     s.close()
