@@ -2,9 +2,13 @@
 // (main, voftrial1.cu:1228-1424) over the C ABI of libvof_hip.so.
 //
 // Same flags, same stdout lines and exit codes, same output files
-// (output/step-NNNNN.vtk, byte-identical writer).  Extension flags select the
-// grid, precision, gravity, IC box, step mode and surface tension (--csf,
-// --sigma); without them the run is the reference's default case.  Deliberate deviation: --nstep 0, which is an
+// (output/step-NNNNN.vtk, byte-identical writer).  Files are written while the
+// following steps run (vof_write_vtk_async; --sync-output writes them in the
+// loop, as the reference does).  --load-state FILE starts from a restart file
+// instead of the IC, --save-state FILE writes one after the last step.
+// Extension flags select the grid, precision,
+// gravity, IC box, step mode and surface tension (--csf, --sigma); without them
+// the run is the reference's default case.  Deliberate deviation: --nstep 0, which is an
 // integer modulo by zero in the reference (:1403), is rejected with an error.
 #include <sys/stat.h>
 
@@ -48,9 +52,9 @@ int main(int argc, char** argv) {
     int max_steps = 0;
     vof_params prm;
     vof_params_default(&prm);
-    bool use_graph = false, diag = false, have_box = false;
+    bool use_graph = false, diag = false, have_box = false, sync_output = false;
     double box[3] = {0, 0, 0};
-    std::string outdir = "output";
+    std::string outdir = "output", load_state, save_state;
 
     for (int a = 1; a < argc; ++a) {  // :1235-1251 (unknown flags ignored)
         const std::string arg = argv[a];
@@ -97,6 +101,12 @@ int main(int argc, char** argv) {
             use_graph = true;
         } else if (arg == "--diag") {
             diag = true;
+        } else if (arg == "--sync-output") {
+            sync_output = true;
+        } else if (arg == "--load-state" && a + 1 < argc) {
+            load_state = argv[++a];
+        } else if (arg == "--save-state" && a + 1 < argc) {
+            save_state = argv[++a];
         } else if (arg == "--outdir" && a + 1 < argc) {
             outdir = argv[++a];
         }
@@ -123,7 +133,10 @@ int main(int argc, char** argv) {
 
     vof_solver* s = nullptr;
     CLI_CHECK(vof_create(&prm, &s));
-    if (have_box)
+    int64_t istep0 = 0;
+    if (!load_state.empty())
+        CLI_CHECK(vof_load_state(s, load_state.c_str(), &istep0));
+    else if (have_box)
         CLI_CHECK(vof_init_box(s, box[0], box[1], box[2]));
     else
         CLI_CHECK(vof_init_ic(s, initial_condition));
@@ -131,7 +144,7 @@ int main(int argc, char** argv) {
 
     ::mkdir(outdir.c_str(), 0777);  // :1338 (mkdir -p output)
 
-    int istep = 0;
+    int istep = int(istep0);
     auto t0 = std::chrono::steady_clock::now();  // :1359
 
     while (!g_stop && (max_steps == 0 || istep < max_steps)) {  // :1361
@@ -146,7 +159,7 @@ int main(int argc, char** argv) {
             oss << outdir << "/step-" << std::setw(5) << std::setfill('0') << istep << ".vtk";
             std::cout << ">>> Exporting step-" << std::setw(5) << std::setfill('0') << istep << " result..."
                       << std::endl;
-            CLI_CHECK(vof_write_vtk(s, oss.str().c_str()));
+            CLI_CHECK(sync_output ? vof_write_vtk(s, oss.str().c_str()) : vof_write_vtk_async(s, oss.str().c_str()));
             if (diag) {
                 vof_diag d;
                 CLI_CHECK(vof_diagnostics(s, &d));
@@ -157,7 +170,9 @@ int main(int argc, char** argv) {
         }
     }
 
-    CLI_CHECK(vof_sync(s));  // :1414
+    CLI_CHECK(vof_output_wait(s));  // every file written
+    CLI_CHECK(vof_sync(s));         // :1414
+    if (!save_state.empty()) CLI_CHECK(vof_save_state(s, save_state.c_str(), istep));
     auto t1 = std::chrono::steady_clock::now();
     const double elapsed = std::chrono::duration_cast<std::chrono::duration<double>>(t1 - t0).count();
     std::cout << ">>> Finished at step " << istep << " in " << elapsed << " s" << std::endl;  // :1420

@@ -15,7 +15,9 @@
 #include <cstring>
 #include <map>
 #include <new>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/vof_capi.h"
@@ -152,6 +154,7 @@ struct EnsBase {
     virtual vof_status step(int64_t first, int n, bool graph) = 0;
     virtual vof_status download(int inst, vof_field f, void* host, size_t count) = 0;
     virtual vof_status upload(int inst, vof_field f, const void* host, size_t count) = 0;
+    virtual vof_status state_io(bool save, std::FILE* fp, int64_t* istep) = 0;  // restart file, instance 0
     virtual vof_status diagnostics(int inst, vof_diag* d) = 0;
     virtual vof_status sync() = 0;
     virtual vof_status time_steps(int64_t first, int warmup, int steps, bool graph, double* ms,
@@ -164,6 +167,8 @@ struct EnsBase {
     virtual int ninst() const = 0;
     virtual vof_status autotune(int64_t first, int steps, const char* cache, vof_tuning* out) = 0;
     virtual const vof_params& params(int inst) const = 0;
+    virtual const void* field_dev(int inst, vof_field f) = 0;  // current device buffer of a stored field
+    virtual hipStream_t stream() const = 0;
     virtual vof_status adj_forward(int64_t first, int n, int every, double* J) = 0;
     virtual vof_status adj_seed(vof_field f, const double* host, size_t count) = 0;
     virtual vof_status adj_backward(double gJ, vof_adj_scalars* out) = 0;
@@ -556,6 +561,11 @@ struct Ensemble : EnsBase {
         }
     }
 
+    const void* field_dev(int inst, vof_field f) override {
+        return (inst >= 0 && inst < ninst()) ? field_ptr(inst, f) : nullptr;
+    }
+    hipStream_t stream() const override { return st; }
+
     vof_status download(int inst, vof_field f, void* host, size_t count) override {
         if (inst < 0 || inst >= ninst()) { set_err("instance %d out of range", inst); return VOF_ERR_INVALID; }
         const auto& I = ih[size_t(inst)];
@@ -581,6 +591,57 @@ struct Ensemble : EnsBase {
         if (!src) { set_err("download: unknown field %d", int(f)); return VOF_ERR_INVALID; }
         VOF_HIP(hipMemcpyAsync(host, src, count * sizeof(R), hipMemcpyDeviceToHost, st));
         VOF_HIP(hipStreamSynchronize(st));
+        return VOF_OK;
+    }
+
+    // Restart file.  A step reads F (both buffers: the intermediate FCT sweeps
+    // read ghost cells of the second one, written in an earlier step), u, v, w
+    // and p of the current buffer; u*, v*, w*, rhs and the other p buffer are
+    // rewritten before they are read, their never-written entries are zero.
+    // Layout: 8-byte magic, then uint32 version, precision, nx, ny, nz, nfields,
+    // int64 istep, then nfields arrays of (nx+2)(ny+2)(nz+2) values each:
+    // F (current), F (other), u, v, w, p.  Native byte order (little-endian).
+    vof_status state_io(bool save, std::FILE* fp, int64_t* istep) override {
+        static const char kMagic[8] = {'V', 'O', 'F', 'S', 'T', 'A', 'T', 'E'};
+        const auto& I = ih[0];
+        const size_t n = size_t(I.ntot);
+        uint32_t hdr[6] = {1u, uint32_t(sizeof(R)), uint32_t(I.nx), uint32_t(I.ny), uint32_t(I.nz), 6u};
+        R* fields[6] = {I.F[fsel], I.F[fsel ^ 1], I.u, I.v, I.w, I.p[psel]};
+        std::vector<R> buf(n);
+        if (save) {
+            if (std::fwrite(kMagic, 1, 8, fp) != 8 || std::fwrite(hdr, sizeof(hdr), 1, fp) != 1 ||
+                std::fwrite(istep, sizeof(int64_t), 1, fp) != 1) {
+                set_err("state file: write failed");
+                return VOF_ERR_INVALID;
+            }
+            for (R* f : fields) {
+                VOF_HIP(hipMemcpyAsync(buf.data(), f, n * sizeof(R), hipMemcpyDeviceToHost, st));
+                VOF_HIP(hipStreamSynchronize(st));
+                if (std::fwrite(buf.data(), sizeof(R), n, fp) != n) { set_err("state file: write failed"); return VOF_ERR_INVALID; }
+            }
+            return VOF_OK;
+        }
+        char magic[8];
+        uint32_t h[6];
+        int64_t is = 0;
+        if (std::fread(magic, 1, 8, fp) != 8 || std::memcmp(magic, kMagic, 8) != 0 ||
+            std::fread(h, sizeof(h), 1, fp) != 1 || std::fread(&is, sizeof(is), 1, fp) != 1) {
+            set_err("state file: not a VOF state file");
+            return VOF_ERR_INVALID;
+        }
+        if (h[0] != hdr[0] || h[1] != hdr[1] || h[2] != hdr[2] || h[3] != hdr[3] || h[4] != hdr[4] || h[5] != hdr[5]) {
+            set_err("state file: version %u, %u-byte values, grid %ux%ux%u, %u fields; this solver needs version %u, "
+                    "%u-byte values, grid %ux%ux%u, %u fields", h[0], h[1], h[2], h[3], h[4], h[5], hdr[0], hdr[1],
+                    hdr[2], hdr[3], hdr[4], hdr[5]);
+            return VOF_ERR_INVALID;
+        }
+        std::vector<std::vector<R>> data(6, std::vector<R>(n));
+        for (auto& d : data)
+            if (std::fread(d.data(), sizeof(R), n, fp) != n) { set_err("state file: truncated"); return VOF_ERR_INVALID; }
+        for (int q = 0; q < 6; ++q)
+            VOF_HIP(hipMemcpyAsync(fields[q], data[size_t(q)].data(), n * sizeof(R), hipMemcpyHostToDevice, st));
+        VOF_HIP(hipStreamSynchronize(st));
+        *istep = is;
         return VOF_OK;
     }
 
@@ -1209,8 +1270,49 @@ vof_status init_ref_ic(EnsBase* e, int ic) {
 
 }  // namespace
 
+// Asynchronous VTK output of F.  F is copied on the solver's stream into a
+// device staging slot (in step order; a device copy of one field), then into
+// pinned host memory on a second stream while later steps run, and a host
+// thread writes the file from there.  Two slots: a slot is reused once its
+// previous file is written.  Failures of a writer thread are kept here and
+// reported by the next call or by vof_output_wait.
+struct AsyncVtk {
+    size_t bytes = 0;
+    void* dev[2] = {};
+    void* host[2] = {};
+    hipEvent_t staged[2] = {}, copied[2] = {};
+    hipStream_t cst = nullptr;
+    std::thread th[2];
+    std::mutex mu;
+    std::string err;
+    int next = 0;
+
+    void join_all() {
+        for (auto& t : th)
+            if (t.joinable()) t.join();
+    }
+    bool take_error(std::string* out) {
+        std::lock_guard<std::mutex> g(mu);
+        if (err.empty()) return false;
+        *out = err;
+        err.clear();
+        return true;
+    }
+    ~AsyncVtk() {
+        join_all();
+        for (int i = 0; i < 2; ++i) {
+            if (staged[i]) (void)hipEventDestroy(staged[i]);
+            if (copied[i]) (void)hipEventDestroy(copied[i]);
+            if (dev[i]) (void)hipFree(dev[i]);
+            if (host[i]) (void)hipHostFree(host[i]);
+        }
+        if (cst) (void)hipStreamDestroy(cst);
+    }
+};
+
 struct vof_solver {
     EnsBase* e;
+    AsyncVtk* out = nullptr;
 };
 struct vof_batch {
     EnsBase* e;
@@ -1247,6 +1349,7 @@ vof_status vof_create(const vof_params* p, vof_solver** out) {
 
 void vof_destroy(vof_solver* s) {
     if (!s) return;
+    delete s->out;  // waits for pending files before the solver's stream goes away
     delete s->e;
     delete s;
 }
@@ -1336,6 +1439,96 @@ vof_status vof_write_vtk(vof_solver* s, const char* path) {
         rc = vof::write_legacy_vtk_f32(path, h.data(), p.nx + 2, p.ny + 2, p.nz + 2);
     }
     if (rc != 0) { set_err("Failed to open output file: %s", path); return VOF_ERR_INVALID; }
+    return VOF_OK;
+}
+
+vof_status vof_write_vtk_async(vof_solver* s, const char* path) {
+    VOF_NEED(s);
+    if (!path) { set_err("null path"); return VOF_ERR_INVALID; }
+    const vof_params& p = s->e->params(0);
+    const size_t esz = (p.precision == 8) ? 8 : 4;
+    if (!s->out) {
+        auto* a = new (std::nothrow) AsyncVtk();
+        if (!a) { set_err("async output: out of host memory"); return VOF_ERR_ALLOC; }
+        a->bytes = size_t(s->e->total_cells(0)) * esz;
+        hipError_t e = hipStreamCreateWithFlags(&a->cst, hipStreamNonBlocking);
+        for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+            if ((e = hipMalloc(&a->dev[i], a->bytes)) != hipSuccess) break;
+            if ((e = hipHostMalloc(&a->host[i], a->bytes, hipHostMallocDefault)) != hipSuccess) break;
+            if ((e = hipEventCreateWithFlags(&a->staged[i], hipEventDisableTiming)) != hipSuccess) break;
+            e = hipEventCreateWithFlags(&a->copied[i], hipEventDisableTiming);
+        }
+        if (e != hipSuccess) {
+            delete a;
+            set_err("async output: allocation failed: %s", hipGetErrorString(e));
+            return VOF_ERR_ALLOC;
+        }
+        s->out = a;
+    }
+    AsyncVtk& a = *s->out;
+    const int k = a.next;
+    if (a.th[k].joinable()) a.th[k].join();  // the slot's previous file is written
+    std::string err;
+    if (a.take_error(&err)) { set_err("%s", err.c_str()); return VOF_ERR_INVALID; }
+    const void* F = s->e->field_dev(0, VOF_FIELD_F);
+    hipStream_t st = s->e->stream();
+    VOF_HIP(hipMemcpyAsync(a.dev[k], F, a.bytes, hipMemcpyDeviceToDevice, st));
+    VOF_HIP(hipEventRecord(a.staged[k], st));
+    VOF_HIP(hipStreamWaitEvent(a.cst, a.staged[k], 0));
+    VOF_HIP(hipMemcpyAsync(a.host[k], a.dev[k], a.bytes, hipMemcpyDeviceToHost, a.cst));
+    VOF_HIP(hipEventRecord(a.copied[k], a.cst));
+    AsyncVtk* ap = s->out;
+    const std::string file(path);
+    const int nxT = p.nx + 2, nyT = p.ny + 2, nzT = p.nz + 2;
+    const bool dbl = (p.precision == 8);
+    a.th[k] = std::thread([ap, k, file, nxT, nyT, nzT, dbl]() {
+        const hipError_t e = hipEventSynchronize(ap->copied[k]);
+        int rc = -1;
+        if (e == hipSuccess)
+            rc = dbl ? vof::write_legacy_vtk_f64(file.c_str(), static_cast<const double*>(ap->host[k]), nxT, nyT, nzT)
+                     : vof::write_legacy_vtk_f32(file.c_str(), static_cast<const float*>(ap->host[k]), nxT, nyT, nzT);
+        if (e != hipSuccess || rc != 0) {
+            std::lock_guard<std::mutex> g(ap->mu);
+            if (ap->err.empty())
+                ap->err = (e != hipSuccess) ? std::string("async output: ") + hipGetErrorString(e)
+                                            : "Failed to open output file: " + file;
+        }
+    });
+    a.next = k ^ 1;
+    return VOF_OK;
+}
+
+vof_status vof_save_state(vof_solver* s, const char* path, int64_t istep) {
+    VOF_NEED(s);
+    if (!path) { set_err("null path"); return VOF_ERR_INVALID; }
+    if (s->e->ninst() != 1) { set_err("state files hold one instance"); return VOF_ERR_INVALID; }
+    VOF_HIP(hipStreamSynchronize(s->e->stream()));
+    const std::string tmp = std::string(path) + ".tmp";
+    std::FILE* fp = std::fopen(tmp.c_str(), "wb");
+    if (!fp) { set_err("Failed to open state file: %s", path); return VOF_ERR_INVALID; }
+    vof_status rc = s->e->state_io(true, fp, &istep);
+    if (std::fclose(fp) != 0 && rc == VOF_OK) { set_err("state file: write failed"); rc = VOF_ERR_INVALID; }
+    if (rc == VOF_OK && std::rename(tmp.c_str(), path) != 0) { set_err("state file: cannot rename to %s", path); rc = VOF_ERR_INVALID; }
+    if (rc != VOF_OK) std::remove(tmp.c_str());
+    return rc;
+}
+
+vof_status vof_load_state(vof_solver* s, const char* path, int64_t* istep) {
+    VOF_NEED(s);
+    if (!path || !istep) { set_err("null argument"); return VOF_ERR_INVALID; }
+    std::FILE* fp = std::fopen(path, "rb");
+    if (!fp) { set_err("Failed to open state file: %s", path); return VOF_ERR_INVALID; }
+    const vof_status rc = s->e->state_io(false, fp, istep);
+    std::fclose(fp);
+    return rc;
+}
+
+vof_status vof_output_wait(vof_solver* s) {
+    VOF_NEED(s);
+    if (!s->out) return VOF_OK;
+    s->out->join_all();
+    std::string err;
+    if (s->out->take_error(&err)) { set_err("%s", err.c_str()); return VOF_ERR_INVALID; }
     return VOF_OK;
 }
 

@@ -122,6 +122,10 @@ def load_library(path: str = LIB_PATH):
         "vof_upload": (c_int, [S, c_int, c_void_p, c_size_t]),
         "vof_diagnostics": (c_int, [S, POINTER(VofDiag)]),
         "vof_write_vtk": (c_int, [S, c_char_p]),
+        "vof_write_vtk_async": (c_int, [S, c_char_p]),
+        "vof_output_wait": (c_int, [S]),
+        "vof_save_state": (c_int, [S, c_char_p, c_int64]),
+        "vof_load_state": (c_int, [S, c_char_p, POINTER(c_int64)]),
         "vof_write_vtk_host": (c_int, [c_char_p, c_void_p, c_int, c_int, c_int, c_int]),
         "vof_copy_bandwidth": (c_int, [c_int64, c_int, POINTER(c_double)]),
         "vof_sync": (c_int, [S]),
@@ -240,6 +244,24 @@ class Solver:
 
     def write_vtk(self, path: str) -> None:
         _check(self.lib.vof_write_vtk(self.h, path.encode()))
+
+    def write_vtk_async(self, path: str) -> None:
+        """The same file, written while later steps run (see output_wait)."""
+        _check(self.lib.vof_write_vtk_async(self.h, path.encode()))
+
+    def save_state(self, path: str, istep: int) -> None:
+        """Restart file of the state after step `istep` (both F buffers, u, v, w, p)."""
+        _check(self.lib.vof_save_state(self.h, path.encode(), int(istep)))
+
+    def load_state(self, path: str) -> int:
+        """Load a restart file; returns the step it was saved after."""
+        ist = c_int64()
+        _check(self.lib.vof_load_state(self.h, path.encode(), byref(ist)))
+        return ist.value
+
+    def output_wait(self) -> None:
+        """Wait for every pending write_vtk_async file; raises its error if one failed."""
+        _check(self.lib.vof_output_wait(self.h))
 
     def sync(self) -> None:
         _check(self.lib.vof_sync(self.h))

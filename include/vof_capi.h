@@ -144,6 +144,27 @@ vof_status vof_diagnostics(vof_solver* s, vof_diag* d);
 /* Legacy ASCII VTK of F, byte-identical to write_legacy_vtk (:907-941). */
 vof_status vof_write_vtk(vof_solver* s, const char* path);
 
+/* The same file written without stalling the steps (no counterpart in the
+ * reference, whose loop writes synchronously, :1403-1411): F as of the calls
+ * enqueued so far is copied on the handle's stream into a device staging slot,
+ * then to pinned host memory on a second stream while later steps run, and a
+ * host thread writes the file.  Two slots; a third pending file waits for the
+ * oldest.  Staging buffers are allocated on the first call.  Errors of a
+ * writer (e.g. the file cannot be opened) are returned by the next
+ * vof_write_vtk_async or by vof_output_wait; vof_destroy waits for every
+ * pending file. */
+vof_status vof_write_vtk_async(vof_solver* s, const char* path);
+vof_status vof_output_wait(vof_solver* s);
+
+/* Restart file (no counterpart in the reference, whose VTK output cannot
+ * restart a run): the solver state after step `istep` -- both F buffers, u,
+ * v, w and p -- in a binary file with a versioned header (grid, precision).
+ * vof_load_state into a solver created with the same parameters and
+ * continuing with vof_step(s, *istep + 1, ...) gives the bits of the
+ * uninterrupted run.  Single solvers only. */
+vof_status vof_save_state(vof_solver* s, const char* path, int64_t istep);
+vof_status vof_load_state(vof_solver* s, const char* path, int64_t* istep);
+
 /* Host-only VTK writer on a caller buffer (same bytes as vof_write_vtk; no
  * device access).  precision 4: data is float*, 8: double*. */
 vof_status vof_write_vtk_host(const char* path, const void* data, int precision, int nxTot, int nyTot,

@@ -22,7 +22,9 @@ def declared_functions():
 def test_header_declares_the_boundary():
     names = declared_functions()
     for must in ("vof_create", "vof_destroy", "vof_init_ic", "vof_step", "vof_download", "vof_upload",
-                 "vof_diagnostics", "vof_write_vtk", "vof_batch_create", "vof_batch_step"):
+                 "vof_diagnostics", "vof_write_vtk", "vof_write_vtk_async", "vof_output_wait",
+                 "vof_save_state", "vof_load_state",
+                 "vof_batch_create", "vof_batch_step"):
         assert must in names
 
 

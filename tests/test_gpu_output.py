@@ -1,0 +1,93 @@
+"""Output path and restart files on the GPU.
+
+* vof_write_vtk_async: files written while later steps run are byte for byte
+  the synchronous writer's.
+* vof_save_state / vof_load_state: a run interrupted after step n and resumed
+  from the file continues with the bits of the uninterrupted run.
+* The drop-in executable: --save-state / --load-state split a run without
+  changing its VTK files; asynchronous and --sync-output files are equal.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from test_gpu_parity import assert_same
+
+pytestmark = pytest.mark.gpu
+
+
+def test_async_vtk_equals_sync_vtk(vofmod, tmp_path):
+    """Files written by vof_write_vtk_async while later steps run are byte for
+    byte those of vof_write_vtk at the same steps (three pending files, so a
+    staging slot is reused); a failing write is reported by output_wait."""
+    print("async vs sync VTK, 48^3", flush=True)
+    prm = dict(n=48, precision=4, g=(0.0, 0.0, -9.81), dt=2e-5)
+    a = vofmod.Solver(vofmod.default_params(**prm))
+    b = vofmod.Solver(vofmod.default_params(**prm))
+    for s in (a, b):
+        s.init_box(0.04, 0.04, 0.06)
+    for k in range(1, 4):
+        a.step(5 * k - 4, 5, graph=True)
+        a.write_vtk_async(str(tmp_path / f"a{k}.vtk"))
+        b.step(5 * k - 4, 5, graph=True)
+        b.write_vtk(str(tmp_path / f"b{k}.vtk"))
+    a.output_wait()
+    for k in range(1, 4):
+        assert (tmp_path / f"a{k}.vtk").read_bytes() == (tmp_path / f"b{k}.vtk").read_bytes(), k
+    a.write_vtk_async(str(tmp_path / "missing_dir" / "x.vtk"))
+    with pytest.raises(vofmod.VofError, match="Failed to open output file"):
+        a.output_wait()
+    a.output_wait()  # the error is reported once
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_restart_continues_bitwise(vofmod, tmp_path, prec):
+    print(f"restart after 7 of 12 steps, 40x36x44, prec {prec}", flush=True)
+    prm = dict(nx=40, ny=36, nz=44, precision=prec, g=(0.0, 0.0, -9.81), dt=2e-5)
+    a = vofmod.Solver(vofmod.default_params(**prm))
+    a.init_box(0.04, 0.05, 0.06)
+    a.step(1, 7, graph=True)
+    path = str(tmp_path / "state.bin")
+    a.save_state(path, 7)
+    a.step(8, 5, graph=True)
+    b = vofmod.Solver(vofmod.default_params(**prm))
+    assert b.load_state(path) == 7
+    b.step(8, 5, graph=True)
+    for f in ("F", "U", "V", "W", "P"):
+        assert_same(b.download(f), a.download(f), f)
+    c = vofmod.Solver(vofmod.default_params(**dict(prm, nx=38)))
+    with pytest.raises(vofmod.VofError, match="grid"):
+        c.load_state(path)
+    with pytest.raises(vofmod.VofError, match="Failed to open state file"):
+        c.load_state(str(tmp_path / "none.bin"))
+    for s in (a, b, c):
+        s.close()
+
+
+def test_cli_restart_and_async_output(vofmod, tmp_path):
+    """`3dvof --n 48 --max_steps 20 --nstep 5`, once straight through, once
+    with --sync-output, and once split at step 10 with --save-state /
+    --load-state: every run writes the same step-00020.vtk."""
+    print("CLI restart / async output, 48^3", flush=True)
+    def run(d, *extra):
+        d.mkdir()
+        r = subprocess.run([vofmod.CLI_PATH, "--n", "48", "--nstep", "5", *extra], cwd=d, capture_output=True,
+                           text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        return r.stdout
+    run(tmp_path / "a", "--max_steps", "20")
+    run(tmp_path / "b", "--max_steps", "20", "--sync-output")
+    st = str(tmp_path / "state.bin")
+    run(tmp_path / "c", "--max_steps", "10", "--save-state", st)
+    out = run(tmp_path / "d", "--max_steps", "20", "--load-state", st)
+    assert "Exporting step-00015" in out and "Exporting step-00005" not in out
+    ref = (tmp_path / "a" / "output" / "step-00020.vtk").read_bytes()
+    for d in ("b", "d"):
+        assert (tmp_path / d / "output" / "step-00020.vtk").read_bytes() == ref, d
+    assert sorted(os.listdir(tmp_path / "a" / "output")) == [f"step-{k:05d}.vtk" for k in (5, 10, 15, 20)]
+    assert (tmp_path / "b" / "output" / "step-00010.vtk").read_bytes() == \
+        (tmp_path / "c" / "output" / "step-00010.vtk").read_bytes()
