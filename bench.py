@@ -63,9 +63,10 @@ def parse():
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="run one plain copy of this many bytes first (byte-counter calibration)")
     ap.add_argument("--jacobi-tblock", type=int, default=0)
-    ap.add_argument("--study", choices=["precision"], default=None,
+    ap.add_argument("--study", choices=["precision", "long"], default=None,
                     help="instead of the benchmark line, print the FP32-vs-FP64 study of the 200^3 dam break "
-                         "(1000 steps) and the 384^3 ripple layer (500 steps)")
+                         "(1000 steps) and the 384^3 ripple layer (500 steps); 'long': the 200^3 dam break "
+                         "over 20000 steps")
     return ap.parse_args()
 
 
@@ -252,20 +253,25 @@ def csf_adjoint_leg(vof, n=64, steps=200, every=20, sigma=0.07):
     return out
 
 
-def precision_study(vof):
+def precision_study(vof, long_run=False):
     """FP32 fields (FP64 sums) against an all-FP64 run of the same case, both on
     the GPU: per-field errors after the last step (p with its interior mean
-    removed, as p is defined up to a constant), the Jacobi residual and liquid
-    volume every `every` steps, and the graph-replayed step rate of each
-    precision with the step's algorithmic bytes (236 B/cell FP32, x2 FP64)
-    against the 8 TB/s HBM peak."""
-    cases = [
-        ("dam_break_200", "dam-break 200^3, box IC x<0.04 y<0.04 z<0.06 m, g=(0,0,-9.81), dt=4e-6 s", 200, 1000,
-         50, "box"),
-        ("ripple_384", "liquid layer 0.06 m deep at 384^3, free surface = 4 plane waves of up to 2 cells "
-         "(wavelengths U[8,32] cells, seed 20261203), g=(0,0,-9.81), dt=4e-6 s", 384, 500, 25, "ripple"),
-    ]
-    out = {"study": "precision", "cases": {}}
+    removed, as p is defined up to a constant), the Jacobi residual, liquid
+    volume, F range and non-finite count every `every` steps, and the
+    graph-replayed step rate of each precision with the step's algorithmic
+    bytes (236 B/cell FP32, x2 FP64) against the 8 TB/s HBM peak.  long_run:
+    the 200^3 dam break for 20000 steps (0.08 s of flow) instead."""
+    if long_run:
+        cases = [("dam_break_200_20000", "dam-break 200^3, box IC x<0.04 y<0.04 z<0.06 m, g=(0,0,-9.81), dt=4e-6 s, "
+                  "20000 steps", 200, 20000, 1000, "box")]
+    else:
+        cases = [
+            ("dam_break_200", "dam-break 200^3, box IC x<0.04 y<0.04 z<0.06 m, g=(0,0,-9.81), dt=4e-6 s", 200, 1000,
+             50, "box"),
+            ("ripple_384", "liquid layer 0.06 m deep at 384^3, free surface = 4 plane waves of up to 2 cells "
+             "(wavelengths U[8,32] cells, seed 20261203), g=(0,0,-9.81), dt=4e-6 s", 384, 500, 25, "ripple"),
+        ]
+    out = {"study": "long run" if long_run else "precision", "cases": {}}
     for key, desc, n, steps, every, ic in cases:
         runs = {}
         for prec in (8, 4):
@@ -280,7 +286,8 @@ def precision_study(vof):
                 s.step(1 + c * every, every, graph=True)
                 d = s.diagnostics()
                 hist.append(dict(step=(c + 1) * every, liquid_volume=d["liquid_volume"],
-                                 residual_l2=d["residual_l2"], nonfinite=d["nonfinite"]))
+                                 residual_l2=d["residual_l2"], nonfinite=d["nonfinite"], f_min=d["f_min"],
+                                 f_max=d["f_max"]))
             fields = {k: s.download(k) for k in ("F", "U", "V", "W", "P")}
             ms = s.time_steps(1 + steps, 10, 100, graph=True) / 100
             print(f"precision study: {key} {'FP64' if prec == 8 else 'FP32'} done", file=sys.stderr, flush=True)
@@ -393,8 +400,8 @@ def main():
     import vof
 
     vof.load_library()
-    if a.study == "precision":
-        print(json.dumps(precision_study(vof)), flush=True)
+    if a.study in ("precision", "long"):
+        print(json.dumps(precision_study(vof, long_run=(a.study == "long"))), flush=True)
         return
     if a.calibrate_copy:
         vof.copy_bandwidth(a.calibrate_copy, 1)
