@@ -69,7 +69,8 @@ typedef enum vof_field {
  * the correction, and the surface force above through its face density. */
 #define VOF_FLAG_RHO_PREDICTOR 0x4u
 #define VOF_FLAG_NO_SLIP       0x8u
-#define VOF_FLAG_WATCHDOG      0x10u/* check F in [0,1] and finite u,v,w,p after every step */
+#define VOF_FLAG_WATCHDOG      0x10u/* after every vof_step call: F in [0,1] (up to clamp01's half-ulp
+                                         leaks) and u,v,w,p,F finite, else VOF_ERR_NUMERICS */
 
 /* Solver parameters.  Replaces `struct Params` + `cpar_host` defaults
  * (voftrial1.cu:27-41, :1258-1291).  Physical inputs are double; they are

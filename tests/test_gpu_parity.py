@@ -262,6 +262,23 @@ def test_remove_p_mean(vofmod, orc):
     assert abs(s.diagnostics()["p_mean"]) < 1e-9
 
 
+def test_watchdog(vofmod):
+    """VOF_FLAG_WATCHDOG: a healthy run passes; a NaN planted in u (from the
+    host) reaches the checked fields within one step and vof_step returns
+    VOF_ERR_NUMERICS with the count in the message."""
+    s = vofmod.Solver(vofmod.default_params(n=24, precision=4, g=(0.0, 0.0, -9.81), dt=2e-5,
+                                            flags=vofmod.FLAG_WATCHDOG))
+    s.init_box(0.04, 0.04, 0.06)
+    s.step(1, 20, graph=True)
+    u = s.download("U")
+    u[12, 12, 12] = np.nan
+    s.upload("U", u)
+    with pytest.raises(vofmod.VofError, match="watchdog: instance 0 has [1-9][0-9]* non-finite") as e:
+        s.step(21, 1)
+    assert e.value.status == 5
+    s.close()
+
+
 @pytest.mark.parametrize("tuning", [None, dict(jacobi_tblock=2, kz_jacobi=8, kz_correct=8, kz_rhs=8, kz_fct_x=8),
                                     dict(jacobi_tblock=2, kz_jacobi=3, fct_chunk=16, kz_fct_x=16, kz_predict=4)])
 def test_batch_equals_plain(vofmod, tuning):
