@@ -635,12 +635,22 @@ struct Ensemble : EnsBase {
                     hdr[2], hdr[3], hdr[4], hdr[5]);
             return VOF_ERR_INVALID;
         }
-        std::vector<std::vector<R>> data(6, std::vector<R>(n));
-        for (auto& d : data)
-            if (std::fread(d.data(), sizeof(R), n, fp) != n) { set_err("state file: truncated"); return VOF_ERR_INVALID; }
-        for (int q = 0; q < 6; ++q)
-            VOF_HIP(hipMemcpyAsync(fields[q], data[size_t(q)].data(), n * sizeof(R), hipMemcpyHostToDevice, st));
-        VOF_HIP(hipStreamSynchronize(st));
+        // the whole file must be there before the state is touched; then one
+        // field at a time through one host buffer
+        const long data_at = std::ftell(fp);
+        if (data_at < 0 || std::fseek(fp, 0, SEEK_END) != 0) { set_err("state file: cannot seek"); return VOF_ERR_INVALID; }
+        const long end = std::ftell(fp);
+        if (end < 0 || uint64_t(end - data_at) != uint64_t(6) * n * sizeof(R)) {
+            set_err("state file: %ld data bytes, expected %llu", end - data_at,
+                    (unsigned long long)(uint64_t(6) * n * sizeof(R)));
+            return VOF_ERR_INVALID;
+        }
+        if (std::fseek(fp, data_at, SEEK_SET) != 0) { set_err("state file: cannot seek"); return VOF_ERR_INVALID; }
+        for (R* f : fields) {
+            if (std::fread(buf.data(), sizeof(R), n, fp) != n) { set_err("state file: read failed"); return VOF_ERR_INVALID; }
+            VOF_HIP(hipMemcpyAsync(f, buf.data(), n * sizeof(R), hipMemcpyHostToDevice, st));
+            VOF_HIP(hipStreamSynchronize(st));
+        }
         *istep = is;
         return VOF_OK;
     }
@@ -1481,7 +1491,7 @@ vof_status vof_write_vtk_async(vof_solver* s, const char* path) {
     const std::string file(path);
     const int nxT = p.nx + 2, nyT = p.ny + 2, nzT = p.nz + 2;
     const bool dbl = (p.precision == 8);
-    a.th[k] = std::thread([ap, k, file, nxT, nyT, nzT, dbl]() {
+    auto write = [ap, k, file, nxT, nyT, nzT, dbl]() {
         const hipError_t e = hipEventSynchronize(ap->copied[k]);
         int rc = -1;
         if (e == hipSuccess)
@@ -1493,7 +1503,12 @@ vof_status vof_write_vtk_async(vof_solver* s, const char* path) {
                 ap->err = (e != hipSuccess) ? std::string("async output: ") + hipGetErrorString(e)
                                             : "Failed to open output file: " + file;
         }
-    });
+    };
+    try {
+        a.th[k] = std::thread(write);
+    } catch (...) {  // no thread to be had: write the file here instead
+        write();
+    }
     a.next = k ^ 1;
     return VOF_OK;
 }

@@ -64,7 +64,16 @@ def test_restart_continues_bitwise(vofmod, tmp_path, prec):
         c.load_state(path)
     with pytest.raises(vofmod.VofError, match="Failed to open state file"):
         c.load_state(str(tmp_path / "none.bin"))
-    for s in (a, b, c):
+    # a truncated file is refused before any field is touched
+    cut = tmp_path / "cut.bin"
+    cut.write_bytes(open(path, "rb").read()[:-100])
+    d = vofmod.Solver(vofmod.default_params(**prm))
+    d.init_box(0.04, 0.05, 0.06)
+    F0 = d.download("F")
+    with pytest.raises(vofmod.VofError, match="data bytes"):
+        d.load_state(str(cut))
+    assert_same(d.download("F"), F0, "F after a refused load")
+    for s in (a, b, c, d):
         s.close()
 
 
