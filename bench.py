@@ -41,6 +41,12 @@ STAGE_BYTES = {
     "fct_z": 12,
 }
 LAUNCHES_PER_STEP = {"predictor": 1, "rhs": 1, "jacobi": 10, "correction": 1, "fct_x": 1, "fct_y": 1, "fct_z": 1}
+# The z sweep and the x sweep after it in one launch (sweep orders y-z-x and
+# z-x-y, two steps of every three): credited with both sweeps' 24 B/cell, the
+# "effective" figure as for two Jacobi sweeps per launch; what it must move
+# (read F, w, u; write F) is 16.
+FCT_ZX_BYTES = 24
+FCT_ZX_COMPULSORY_BYTES = 16
 
 
 def parse():
@@ -429,7 +435,7 @@ def main():
     ms_step = 1e3 * wall / a.steps
 
     # per-stage device times from HIP events on the solver's stream
-    st_steps = max(3, a.stage_steps)
+    st_steps = 3 * max(1, (a.stage_steps + 2) // 3)  # whole rotations of the sweep order
     _, stages = s.time_steps(istep, 0, st_steps, stages=True)
     istep += st_steps
     per_step = {k: v / st_steps for k, v in stages.items()}
@@ -440,8 +446,14 @@ def main():
     launches = dict(LAUNCHES_PER_STEP)
     sweeps_per_launch = 2 if tuning.get("jacobi_tblock", 1) == 2 else 1
     launches["jacobi"] = LAUNCHES_PER_STEP["jacobi"] // sweeps_per_launch
+    stage_bytes = dict(STAGE_BYTES)
+    zx = tuning.get("fct_zx_fused", 1) == 2 and tuning.get("kz_fct_x") in (4, 8, 16)
+    if zx:  # per step: x and z alone in one step of three, fused in two
+        launches["fct_x"] = launches["fct_z"] = 1 / 3
+        launches["fct_zx"] = 2 / 3
+        stage_bytes["fct_zx"] = FCT_ZX_BYTES
     kern = {}
-    for k, b in STAGE_BYTES.items():
+    for k, b in stage_bytes.items():
         t_launch = per_step[k] / launches[k] * 1e-3
         if t_launch <= 0:
             continue
@@ -449,6 +461,9 @@ def main():
         kern[k] = {"ms_per_launch": t_launch * 1e3, "bytes_per_launch": byts, "gbs": byts / t_launch / 1e9,
                    "frac": byts / t_launch / 1e9 / HBM_PEAK_GBS}
     kern["jacobi"]["sweeps_per_launch"] = sweeps_per_launch
+    if "fct_zx" in kern:
+        kern["fct_zx"]["sweeps_per_launch"] = 2
+        kern["fct_zx"]["compulsory_bytes_per_launch"] = FCT_ZX_COMPULSORY_BYTES * (esz // 4) * cells
     dom = max(kern, key=lambda k: per_step[k])
     d = kern[dom]
     step_bytes = sum(STAGE_BYTES[k] * LAUNCHES_PER_STEP[k] for k in STAGE_BYTES) * (esz // 4) * cells

@@ -78,13 +78,14 @@ int box_extent(int n, double L, double b) {
 }
 
 // Tuning cache: a JSON object, one entry per line,
-//   "gfx950|200x200x200|p4|n1|c8000000|s10": {"kz_predict": 4, ..., "kz_fct_x": 4},
-const char* const kTuneFields[7] = {"kz_predict", "kz_rhs", "kz_jacobi", "kz_correct",
-                                    "fct_chunk", "jacobi_tblock", "kz_fct_x"};
+//   "gfx950|200x200x200|p4|n1|c8000000|s10": {"kz_predict": 4, ..., "fct_zx_fused": 2},
+constexpr int kNumTuneFields = 8;
+const char* const kTuneFields[kNumTuneFields] = {"kz_predict", "kz_rhs", "kz_jacobi", "kz_correct",
+                                                 "fct_chunk", "jacobi_tblock", "kz_fct_x", "fct_zx_fused"};
 
 int32_t* tune_slot(vof_tuning* t, int f) {
-    int32_t* s[7] = {&t->kz_predict, &t->kz_rhs, &t->kz_jacobi, &t->kz_correct,
-                     &t->fct_chunk, &t->jacobi_tblock, &t->kz_fct_x};
+    int32_t* s[kNumTuneFields] = {&t->kz_predict, &t->kz_rhs, &t->kz_jacobi, &t->kz_correct,
+                                  &t->fct_chunk, &t->jacobi_tblock, &t->kz_fct_x, &t->fct_zx_fused};
     return s[f];
 }
 
@@ -103,7 +104,7 @@ bool load_tuning(const char* path, const std::string& key, vof_tuning* t) {
     for (const auto& l : read_lines(path)) {
         if (l.find(tag) == std::string::npos) continue;
         vof_tuning r = *t;
-        for (int f = 0; f < 7; ++f) {
+        for (int f = 0; f < kNumTuneFields; ++f) {
             const std::string name = std::string("\"") + kTuneFields[f] + "\":";
             const size_t at = l.find(name);
             if (at == std::string::npos) return false;
@@ -126,7 +127,7 @@ void store_tuning(const char* path, const std::string& key, vof_tuning t) {
         keep.push_back(l);
     }
     std::string entry = " " + tag + " {";
-    for (int f = 0; f < 7; ++f) {
+    for (int f = 0; f < kNumTuneFields; ++f) {
         char b[64];
         std::snprintf(b, sizeof(b), "%s\"%s\": %d", f ? ", " : "", kTuneFields[f], int(*tune_slot(&t, f)));
         entry += b;
@@ -182,8 +183,8 @@ struct Ensemble : EnsBase {
     std::vector<vof_params> prm;
     std::vector<vof::Inst<R>> ih;
     vof::Ens<R> e;
-    vof::Geo g_pred, g_rhs, g_jac, g_jac2, g_corr, g_fx, g_fy, g_fz, g_ghost, g_red, g_sub;
-    vof_tuning tun{4, 4, 4, 4, 8, 1, 4};
+    vof::Geo g_pred, g_rhs, g_jac, g_jac2, g_corr, g_fx, g_fy, g_fz, g_fzx, g_ghost, g_red, g_sub;
+    vof_tuning tun{4, 4, 4, 4, 8, 1, 4, 2};
     int fsel = 0, psel = 0;
     int sweeps = 10;
     uint32_t flags = 0;
@@ -310,6 +311,8 @@ struct Ensemble : EnsBase {
             return vof::blocks_wcol(I.nx, I.ny, I.nz + 2, kz, 0); }));
         VOF_TRY(make_geo(g_fx, tun.kz_fct_x, [](const vof::Inst<R>& I, int kz) {
             return vof::blocks_wcol(I.nx, I.ny, I.nz, kz, 3); }));
+        VOF_TRY(make_geo(g_fzx, tun.kz_fct_x, [](const vof::Inst<R>& I, int kz) {
+            return vof::blocks_wcol_pad8(I.nx, I.ny, I.nz, kz, 3); }));
         VOF_TRY(make_geo(g_fy, tun.fct_chunk, [](const vof::Inst<R>& I, int ch) {
             return vof::blocks_march(I.nx, I.ny, I.nz, ch); }));
         VOF_TRY(make_geo(g_fz, tun.fct_chunk, [](const vof::Inst<R>& I, int ch) {
@@ -393,6 +396,7 @@ struct Ensemble : EnsBase {
         if (t->fct_chunk > 0) tun.fct_chunk = t->fct_chunk;
         if (t->jacobi_tblock > 0) tun.jacobi_tblock = t->jacobi_tblock;
         if (t->kz_fct_x > 0) tun.kz_fct_x = t->kz_fct_x;
+        if (t->fct_zx_fused > 0) tun.fct_zx_fused = t->fct_zx_fused;
         VOF_HIP(hipStreamSynchronize(st));
         drop_graphs();
         return build_geos();
@@ -450,6 +454,36 @@ struct Ensemble : EnsBase {
         if (flags & VOF_FLAG_CSF) vof::launch_csf(e, g_sub, fsel, st);
     }
 
+    // ---- the FCT sweeps of step istep, in the order of :1384-1396; the last
+    // one also does :1398-1401.  Where the z sweep is directly followed by the
+    // x sweep (orders y-z-x and z-x-y) the pair is one k_fct_zx launch when
+    // fct_zx_fused == 2 and the x chunk is one it is built for.  mark(q) is
+    // called after each launch with the stage-time slot q of that launch.
+    bool zx_fused(int r) const {
+        const int c = tun.kz_fct_x;
+        return r != 0 && tun.fct_zx_fused >= 2 && (c == 4 || c == 8 || c == 16);
+    }
+    int fct_launches(int64_t istep) const { return zx_fused(int(istep % 3)) ? 2 : 3; }
+    template <class Mark>
+    void enqueue_fct(int64_t istep, Mark&& mark) {
+        static const int order[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
+        const int r = int(istep % 3);
+        for (int d = 0; d < 3; ++d) {
+            const int dir = order[r][d];
+            if (dir == 2 && zx_fused(r)) {  // z, then x as sweep d + 1
+                (void)vof::launch_fct_zx(e, g_fzx, fsel, d + 1 == 2, st);
+                fsel ^= 1;
+                ++d;
+                mark(9);
+                continue;
+            }
+            vof::Geo& g = (dir == 0) ? g_fx : (dir == 1) ? g_fy : g_fz;
+            vof::launch_fct(e, g, dir, fsel, d == 2, st);
+            fsel ^= 1;
+            mark(5 + dir);
+        }
+    }
+
     void enqueue_step(int64_t istep) {
         enqueue_predict();
         vof::launch_rhs(e, g_rhs, fsel, st);                  // rhs of :418-422
@@ -459,13 +493,7 @@ struct Ensemble : EnsBase {
             vof::launch_sub_mean(e, g_sub, psel, means, st);
         }
         vof::launch_correct(e, g_corr, fsel, psel, st);       // :1377-1382
-        static const int order[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};  // :1384-1396
-        const int r = int(istep % 3);
-        for (int d = 0; d < 3; ++d) {  // the last sweep also does :1398-1401
-            vof::Geo& g = (order[r][d] == 0) ? g_fx : (order[r][d] == 1) ? g_fy : g_fz;
-            vof::launch_fct(e, g, order[r][d], fsel, d == 2, st);
-            fsel ^= 1;
-        }
+        enqueue_fct(istep, [](int) {});                       // :1384-1401
     }
 
     vof_status replay(int64_t istep) {
@@ -490,7 +518,7 @@ struct Ensemble : EnsBase {
         VOF_HIP(hipGraphLaunch(it->second, st));
         // advance the buffer parities exactly as enqueue_step does
         psel ^= (jacobi_launches() & 1);
-        fsel ^= 1;
+        fsel ^= (fct_launches(istep) & 1);
         return VOF_OK;
     }
 
@@ -748,7 +776,9 @@ struct Ensemble : EnsBase {
             if (stage[1] < bestt[1]) { bestt[1] = stage[1]; best.kz_rhs = c; }
             if (stage[2] < bestt[2]) { bestt[2] = stage[2]; best.kz_jacobi = c; best.jacobi_tblock = tun.jacobi_tblock; }
             if (stage[4] < bestt[4]) { bestt[4] = stage[4]; best.kz_correct = c; }
-            if (stage[5] < bestt[5]) { bestt[5] = stage[5]; best.kz_fct_x = c; }
+            // the x chunk also sets the fused z+x launches' chunk: x, z and z+x together
+            const double fx = stage[5] + stage[7] + stage[9];
+            if (fx < bestt[5]) { bestt[5] = fx; best.kz_fct_x = c; }
         }
         // Jacobi temporal-block depth with its own chunk lengths
         const int other_tb = (tun.jacobi_tblock >= 2) ? 1 : 2;
@@ -772,6 +802,17 @@ struct Ensemble : EnsBase {
             rc = trial(t, stage);
             if (rc == VOF_OK && stage[6] + stage[7] < bestm) { bestm = stage[6] + stage[7]; best.fct_chunk = c; }
         }
+        // z and x sweeps in one launch or two, with the chosen chunks
+        double fz[2] = {1e300, 1e300};
+        for (int k = 0; k < 2; ++k) {
+            if (rc != VOF_OK) break;
+            vof_tuning t = best;
+            t.fct_zx_fused = k + 1;
+            double stage[VOF_NSTAGE_TIMES];
+            rc = trial(t, stage);
+            if (rc == VOF_OK) fz[k] = stage[5] + stage[6] + stage[7] + stage[9];
+        }
+        if (rc == VOF_OK) best.fct_zx_fused = (fz[1] <= fz[0]) ? 2 : 1;
         // restore the state and apply the winner
         const hipError_t re = hipMemcpyAsync(arena, snap, arena_bytes, hipMemcpyDeviceToDevice, st);
         (void)hipStreamSynchronize(st);
@@ -803,7 +844,8 @@ struct Ensemble : EnsBase {
         if (stage_ms) {
             // Per-stage device time: an event after every stage of eager steps.
             // Slot q of ev brackets stage q-1; stages follow the order of the
-            // header (predictor, rhs, jacobi, p-mean, correction, fct x, y, z, ghosts).
+            // header (predictor, rhs, jacobi, p-mean, correction, fct x, y, z,
+            // ghosts, fused fct z+x); slots 6.. take the FCT launches of a step.
             constexpr int NS = VOF_NSTAGE_TIMES;
             for (int q = 0; q < NS; ++q) stage_ms[q] = 0.0;
             hipEvent_t ev[NS + 1];
@@ -823,17 +865,15 @@ struct Ensemble : EnsBase {
                 VOF_HIP(hipEventRecord(ev[4], st));
                 vof::launch_correct(e, g_corr, fsel, psel, st);
                 VOF_HIP(hipEventRecord(ev[5], st));
-                static const int order[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
-                const int r = int(is % 3);
-                hipEvent_t dir_end[3];
-                for (int d = 0; d < 3; ++d) {
-                    const int dir = order[r][d];
-                    vof::Geo& g = (dir == 0) ? g_fx : (dir == 1) ? g_fy : g_fz;
-                    vof::launch_fct(e, g, dir, fsel, d == 2, st);
-                    fsel ^= 1;
-                    dir_end[d] = ev[6 + d];
-                    VOF_HIP(hipEventRecord(dir_end[d], st));
-                }
+                int nl = 0, slot[3];
+                hipError_t me = hipSuccess;
+                enqueue_fct(is, [&](int q) {
+                    slot[nl] = q;
+                    const hipError_t x = hipEventRecord(ev[6 + nl], st);
+                    if (me == hipSuccess) me = x;
+                    ++nl;
+                });
+                VOF_HIP(me);
                 // F ghosts are written by the last sweep: the last slot stays ~0
                 VOF_HIP(hipEventRecord(ev[NS], st));
                 VOF_HIP(hipEventSynchronize(ev[NS]));
@@ -843,10 +883,10 @@ struct Ensemble : EnsBase {
                     stage_ms[q] += t;
                 }
                 hipEvent_t prev = ev[5];
-                for (int d = 0; d < 3; ++d) {
-                    VOF_HIP(hipEventElapsedTime(&t, prev, dir_end[d]));
-                    stage_ms[5 + order[r][d]] += t;
-                    prev = dir_end[d];
+                for (int d = 0; d < nl; ++d) {
+                    VOF_HIP(hipEventElapsedTime(&t, prev, ev[6 + d]));
+                    stage_ms[slot[d]] += t;
+                    prev = ev[6 + d];
                 }
                 VOF_HIP(hipEventElapsedTime(&t, prev, ev[NS]));
                 stage_ms[8] += t;

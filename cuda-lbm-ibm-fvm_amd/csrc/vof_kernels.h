@@ -46,6 +46,9 @@ template <class R> void launch_jacobi(const Ens<R>& e, const Geo& g, int psrc, h
 template <class R> void launch_jacobi2(const Ens<R>& e, const Geo& g, int psrc, hipStream_t st);
 template <class R> void launch_correct(const Ens<R>& e, const Geo& g, int fsel, int psel, hipStream_t st);
 template <class R> void launch_fct(const Ens<R>& e, const Geo& g, int dir, int src, int fin, hipStream_t st);
+// z sweep followed by the x sweep in one launch, chunks of g.kz in {4, 8, 16}
+// planes (false, and nothing launched, for any other chunk).
+template <class R> bool launch_fct_zx(const Ens<R>& e, const Geo& g, int src, int fin, hipStream_t st);
 template <class R> void launch_clamp_F(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
 template <class R> void launch_ghost_F(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
 template <class R> void launch_sub_mean(const Ens<R>& e, const Geo& g, int psel, const double* means, hipStream_t st);
@@ -64,6 +67,14 @@ inline int blocks_wcol(int nx, int ny, int nzrange, int kz, int h) {
     const int s = 64 - 2 * h;
     const int wpp = (plane + s - 1) / s;
     return ((wpp + 3) / 4) * ((nzrange + kz - 1) / kz);
+}
+// The same columns with each chunk's work-groups padded to a multiple of 8
+// (k_fct_zx: the chunks of one column share an XCD and its L2).
+inline int blocks_wcol_pad8(int nx, int ny, int nzrange, int kz, int h) {
+    const int plane = (nx + 2) * (ny + 2);
+    const int s = 64 - 2 * h;
+    const int wpp = (plane + s - 1) / s;
+    return ((((wpp + 3) / 4) + 7) & ~7) * ((nzrange + kz - 1) / kz);
 }
 // Two-sweep Jacobi tiles: 60 stored lanes x 14 stored rows per work-group
 // (1024 lanes), one work-group row per chunk of kz planes.

@@ -63,12 +63,15 @@ struct WMap {
     bool own;  // lane's cell is stored by this wave
 };
 
-template <class R, int H>
+// PAD8: the work-groups of one plane chunk are padded to a multiple of 8, so
+// that under the dispatcher's round robin the chunks of one column run on one
+// XCD (for kernels whose z windows overlap between chunks).
+template <class R, int H, bool PAD8 = false>
 __device__ __forceinline__ bool wmap(const Inst<R>& I, int lb, int kz, int klo, int khi, WMap& m) {
     constexpr int S = 64 - 2 * H;
     const int P = I.sx * (I.ny + 2);
     const int wpp = (P + S - 1) / S;
-    const int bpp = (wpp + 3) >> 2;
+    const int bpp = PAD8 ? ((((wpp + 3) >> 2) + 7) & ~7) : ((wpp + 3) >> 2);
     const int zb = lb / bpp;
     const int wv = (lb - zb * bpp) * 4 + ((int)threadIdx.x >> 6);
     if (wv >= wpp) return false;  // wave-uniform
@@ -1054,6 +1057,61 @@ __global__ __launch_bounds__(256) void k_fct_line(const EnsArg<R> ea,
     }
 }
 
+// z sweep directly followed by the x sweep, in one launch (the sweep orders
+// (y, z, x) and (z, x, y) of :1384-1396).  Lanes run along x in the
+// overlapped waves of the x sweep (halo 3, 58 stored lanes) and march along z
+// through a chunk of CH planes as in k_fct_line<R, 2, CH>; each z-sweep value
+// the march emits stays in a register, and the x sweep of that plane is formed
+// from it at once by lane shifts (every lane of the wave emits the same plane
+// in the same step).  The intermediate F never goes to memory.
+//
+// Lanes outside the interior (ghost columns and rows) carry F' = 0 into the x
+// sweep.  The two-launch form reads the intermediate buffer's ghost cells
+// there, whose values it does not depend on: at a wall face u = +0, so both
+// fluxes through it are signed zeros whose sign cancels in L - L' and H - L,
+// and ghost cells' own Ftd, ratios and limiters are zero.  The z part is never
+// the last sweep of a step, so only the x part clamps twice and writes ghosts.
+template <class R, int CH>
+__global__ __launch_bounds__(256) void k_fct_zx(const EnsArg<R> ea,
+                                                int src, int fin) {
+    VOF_PROLOGUE_RR(R)
+    WMap m;
+    if (!wmap<R, 3, true>(I, lb, CH, 1, I.nz, m)) return;
+    const int i = m.i, j = m.j, nd = I.nz, S = I.sy;
+    const bool line = (m.t == m.tc) && i >= 1 && i <= I.nx && j >= 1 && j <= I.ny;
+    const bool store = line && m.own;
+    const Buf<R> Fb(sel2(I.F, src), I.ntot), Fo(sel2(I.F, src ^ 1), I.ntot), Wb(I.w, I.ntot), Ub(I.u, I.ntot);
+    auto cl = [nd](int mm) { return mm < 0 ? 0 : (mm > nd + 1 ? nd + 1 : mm); };
+    const int m0 = m.k0, m1 = m.k1;
+    constexpr int NF = CH + 6, NV = CH + 5;
+    R fw[NF], vw[NV], uw[CH];  // F(m0-3+e), w(m0-2+e), u(m0+e)
+#pragma unroll
+    for (int e = 0; e < NF; ++e) fw[e] = Fb.ld(m.tc, cl(m0 - 3 + e) * S);
+    bool gas = true;
+#pragma unroll
+    for (int e = 0; e < NF; ++e) gas = gas && (!line || fw[e] == R(0));
+    if (__all(gas)) {  // F' = 0 on every lane and plane, so the x sweep stores +0
+        if (store) {
+#pragma unroll
+            for (int e = 0; e < CH; ++e)
+                if (m0 + e <= m1) store_F(I, Fo, i, j, m0 + e, R(0), fin != 0);
+        }
+        return;
+    }
+#pragma unroll
+    for (int e = 0; e < NV; ++e) vw[e] = Wb.ld(m.tc, cl(m0 - 2 + e) * S);
+#pragma unroll
+    for (int e = 0; e < CH; ++e) uw[e] = Ub.ld(m.tc, min(m0 + e, m1) * S);
+    FctWin<R> w;
+    fct_win_init(I, w, fw[0], fw[1], vw[0]);
+#pragma unroll
+    for (int s = 0; s < CH + 4; ++s) {  // cell c = m0 - 2 + s
+        const int c = m0 - 2 + s;
+        const R fz = fct_win_step<R, 2>(I, w, c, nd, fw[s + 2], vw[s + 1], s >= 4);
+        if (s >= 4 && c - 2 <= m1) fct_x_plane(I, Fo, line ? fz : R(0), uw[s - 4], line, store, i, j, c - 2, fin);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // F ghost layer on its own (after an IC upload or a single test stage): every
 // ghost equals the nearest interior cell, F(clamp(i), clamp(j), clamp(k)),
@@ -1384,6 +1442,17 @@ void launch_fct(const Ens<R>& e, const Geo& g, int dir, int src, int fin, hipStr
     else
         launch_march<R, 2>(e, g, src, fin, st);
 }
+// z sweep followed by the x sweep (k_fct_zx): reads F[src], writes F[src^1].
+template <class R>
+bool launch_fct_zx(const Ens<R>& e, const Geo& g, int src, int fin, hipStream_t st) {
+    const dim3 grid(g.nblk), blk(256);
+    switch (g.kz) {
+        case 4: hipLaunchKernelGGL((k_fct_zx<R, 4>), grid, blk, 0, st, ens_arg(e, g), src, fin); return true;
+        case 8: hipLaunchKernelGGL((k_fct_zx<R, 8>), grid, blk, 0, st, ens_arg(e, g), src, fin); return true;
+        case 16: hipLaunchKernelGGL((k_fct_zx<R, 16>), grid, blk, 0, st, ens_arg(e, g), src, fin); return true;
+        default: return false;
+    }
+}
 template <class R>
 void launch_clamp_F(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st) {
     hipLaunchKernelGGL(k_clamp_F<R>, dim3(g.nblk), dim3(256), 0, st, ens_arg(e, g), fsel, g.kz);
@@ -1417,6 +1486,7 @@ void launch_reduce(const Ens<R>& e, const Geo& g, int fsel, int psel, int mode, 
     template void launch_jacobi2<R>(const Ens<R>&, const Geo&, int, hipStream_t);                        \
     template void launch_correct<R>(const Ens<R>&, const Geo&, int, int, hipStream_t);                   \
     template void launch_fct<R>(const Ens<R>&, const Geo&, int, int, int, hipStream_t);                  \
+    template bool launch_fct_zx<R>(const Ens<R>&, const Geo&, int, int, hipStream_t);                    \
     template void launch_clamp_F<R>(const Ens<R>&, const Geo&, int, hipStream_t);                        \
     template void launch_ghost_F<R>(const Ens<R>&, const Geo&, int, hipStream_t);                        \
     template void launch_sub_mean<R>(const Ens<R>&, const Geo&, int, const double*, hipStream_t);        \

@@ -178,6 +178,34 @@ def test_launch_shapes_match_oracle(vofmod, orc, prec, shape):
 
 
 @pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("kz", [4, 8, 16])
+@pytest.mark.parametrize("fused", [1, 2])
+def test_fct_zx_fused_matches_oracle(vofmod, orc, prec, kz, fused):
+    """Sweep orders y-z-x and z-x-y with the z and x sweeps in one launch
+    (fused 2) or two (fused 1): whole steps bit for bit, F ghosts included, on
+    a grid with partial z chunks, a large CFL and random F in the ghost layer
+    (which the single launch never reads for its x part).  Steps 1..4 cover
+    the fused pair as the last sweeps of a step (order y-z-x, which also
+    clamps twice and writes the ghosts), as the first (z-x-y), and the
+    unfused order x-y-z; eager and graph replay."""
+    print(f"fct z+x prec={prec} kz_fct_x={kz} fused={fused}", flush=True)
+    rng = np.random.default_rng(SEED + 17 + kz)
+    for graph in (False, True):
+        s, o = make_pair(vofmod, orc, nx=37, ny=45, nz=27, precision=prec, dt=2e-3)
+        s.set_tuning(kz_fct_x=kz, fct_zx_fused=fused)
+        assert s.get_tuning()["fct_zx_fused"] == fused
+        st = random_state(s.shape, rng, s.dtype, vel=1.5)
+        load_state(s, o, st)
+        for step in range(1, 5):
+            s.step(step, 1, graph=graph)
+            o.step(step, 1)
+            for k in ("F", "U", "V", "W"):
+                assert_same(s.download(k), o.get(k), f"{k} after step {step} (graph {graph})")
+        s.close()
+        o.close()
+
+
+@pytest.mark.parametrize("prec", [4, 8])
 @pytest.mark.parametrize("kz", [3, 4, 8, 16])
 def test_jacobi2_tiles_and_chunks(vofmod, orc, kz, prec):
     """Two-sweep launches on a grid with several x tiles, row bands and z chunks
