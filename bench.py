@@ -41,12 +41,12 @@ STAGE_BYTES = {
     "fct_z": 12,
 }
 LAUNCHES_PER_STEP = {"predictor": 1, "rhs": 1, "jacobi": 10, "correction": 1, "fct_x": 1, "fct_y": 1, "fct_z": 1}
-# The z sweep and the x sweep after it in one launch (sweep orders y-z-x and
-# z-x-y, two steps of every three): credited with both sweeps' 24 B/cell, the
-# "effective" figure as for two Jacobi sweeps per launch; what it must move
-# (read F, w, u; write F) is 16.
-FCT_ZX_BYTES = 24
-FCT_ZX_COMPULSORY_BYTES = 16
+# Two FCT sweeps in one launch (z then x in the sweep orders y-z-x and z-x-y,
+# x then y in x-y-z): credited with both sweeps' 24 B/cell, the "effective"
+# figure as for two Jacobi sweeps per launch; what such a launch must move
+# (read F and the two velocities; write F) is 16.
+FCT_PAIR_BYTES = 24
+FCT_PAIR_COMPULSORY_BYTES = 16
 
 
 def parse():
@@ -446,14 +446,19 @@ def main():
     launches = dict(LAUNCHES_PER_STEP)
     sweeps_per_launch = 2 if tuning.get("jacobi_tblock", 1) == 2 else 1
     launches["jacobi"] = LAUNCHES_PER_STEP["jacobi"] // sweeps_per_launch
-    stage_bytes = dict(STAGE_BYTES)
+    stage_bytes = dict(STAGE_BYTES, fct_zx=FCT_PAIR_BYTES, fct_xy=FCT_PAIR_BYTES)
+    # FCT launches of the three sweep orders (:1384-1396), with fused pairs
     zx = tuning.get("fct_zx_fused", 1) == 2 and tuning.get("kz_fct_x") in (4, 8, 16)
-    if zx:  # per step: x and z alone in one step of three, fused in two
-        launches["fct_x"] = launches["fct_z"] = 1 / 3
-        launches["fct_zx"] = 2 / 3
-        stage_bytes["fct_zx"] = FCT_ZX_BYTES
+    xy = tuning.get("fct_xy_fused", 1) == 2 and tuning.get("fct_chunk") in (4, 8, 16)
+    orders = [["fct_xy", "fct_z"] if xy else ["fct_x", "fct_y", "fct_z"],
+              ["fct_y", "fct_zx"] if zx else ["fct_y", "fct_z", "fct_x"],
+              ["fct_zx", "fct_y"] if zx else ["fct_z", "fct_x", "fct_y"]]
+    for k in ("fct_x", "fct_y", "fct_z", "fct_zx", "fct_xy"):
+        launches[k] = sum(o.count(k) for o in orders) / 3
     kern = {}
     for k, b in stage_bytes.items():
+        if launches[k] == 0:
+            continue
         t_launch = per_step[k] / launches[k] * 1e-3
         if t_launch <= 0:
             continue
@@ -461,9 +466,10 @@ def main():
         kern[k] = {"ms_per_launch": t_launch * 1e3, "bytes_per_launch": byts, "gbs": byts / t_launch / 1e9,
                    "frac": byts / t_launch / 1e9 / HBM_PEAK_GBS}
     kern["jacobi"]["sweeps_per_launch"] = sweeps_per_launch
-    if "fct_zx" in kern:
-        kern["fct_zx"]["sweeps_per_launch"] = 2
-        kern["fct_zx"]["compulsory_bytes_per_launch"] = FCT_ZX_COMPULSORY_BYTES * (esz // 4) * cells
+    for k in ("fct_zx", "fct_xy"):
+        if k in kern:
+            kern[k]["sweeps_per_launch"] = 2
+            kern[k]["compulsory_bytes_per_launch"] = FCT_PAIR_COMPULSORY_BYTES * (esz // 4) * cells
     dom = max(kern, key=lambda k: per_step[k])
     d = kern[dom]
     step_bytes = sum(STAGE_BYTES[k] * LAUNCHES_PER_STEP[k] for k in STAGE_BYTES) * (esz // 4) * cells

@@ -78,14 +78,16 @@ int box_extent(int n, double L, double b) {
 }
 
 // Tuning cache: a JSON object, one entry per line,
-//   "gfx950|200x200x200|p4|n1|c8000000|s10": {"kz_predict": 4, ..., "fct_zx_fused": 2},
-constexpr int kNumTuneFields = 8;
-const char* const kTuneFields[kNumTuneFields] = {"kz_predict", "kz_rhs", "kz_jacobi", "kz_correct",
-                                                 "fct_chunk", "jacobi_tblock", "kz_fct_x", "fct_zx_fused"};
+//   "gfx950|200x200x200|p4|n1|c8000000|s10": {"kz_predict": 4, ..., "fct_xy_fused": 2},
+constexpr int kNumTuneFields = 9;
+const char* const kTuneFields[kNumTuneFields] = {"kz_predict", "kz_rhs",    "kz_jacobi",     "kz_correct",
+                                                 "fct_chunk",  "jacobi_tblock", "kz_fct_x", "fct_zx_fused",
+                                                 "fct_xy_fused"};
 
 int32_t* tune_slot(vof_tuning* t, int f) {
     int32_t* s[kNumTuneFields] = {&t->kz_predict, &t->kz_rhs, &t->kz_jacobi, &t->kz_correct,
-                                  &t->fct_chunk, &t->jacobi_tblock, &t->kz_fct_x, &t->fct_zx_fused};
+                                  &t->fct_chunk, &t->jacobi_tblock, &t->kz_fct_x, &t->fct_zx_fused,
+                                  &t->fct_xy_fused};
     return s[f];
 }
 
@@ -183,8 +185,8 @@ struct Ensemble : EnsBase {
     std::vector<vof_params> prm;
     std::vector<vof::Inst<R>> ih;
     vof::Ens<R> e;
-    vof::Geo g_pred, g_rhs, g_jac, g_jac2, g_corr, g_fx, g_fy, g_fz, g_fzx, g_ghost, g_red, g_sub;
-    vof_tuning tun{4, 4, 4, 4, 8, 1, 4, 2};
+    vof::Geo g_pred, g_rhs, g_jac, g_jac2, g_corr, g_fx, g_fy, g_fz, g_fzx, g_fxy, g_ghost, g_red, g_sub;
+    vof_tuning tun{4, 4, 4, 4, 8, 1, 4, 2, 2};
     int fsel = 0, psel = 0;
     int sweeps = 10;
     uint32_t flags = 0;
@@ -313,6 +315,8 @@ struct Ensemble : EnsBase {
             return vof::blocks_wcol(I.nx, I.ny, I.nz, kz, 3); }));
         VOF_TRY(make_geo(g_fzx, tun.kz_fct_x, [](const vof::Inst<R>& I, int kz) {
             return vof::blocks_wcol_pad8(I.nx, I.ny, I.nz, kz, 3); }));
+        VOF_TRY(make_geo(g_fxy, tun.fct_chunk, [](const vof::Inst<R>& I, int ch) {
+            return vof::blocks_xy(I.nx, I.ny, I.nz, ch); }));
         VOF_TRY(make_geo(g_fy, tun.fct_chunk, [](const vof::Inst<R>& I, int ch) {
             return vof::blocks_march(I.nx, I.ny, I.nz, ch); }));
         VOF_TRY(make_geo(g_fz, tun.fct_chunk, [](const vof::Inst<R>& I, int ch) {
@@ -397,6 +401,7 @@ struct Ensemble : EnsBase {
         if (t->jacobi_tblock > 0) tun.jacobi_tblock = t->jacobi_tblock;
         if (t->kz_fct_x > 0) tun.kz_fct_x = t->kz_fct_x;
         if (t->fct_zx_fused > 0) tun.fct_zx_fused = t->fct_zx_fused;
+        if (t->fct_xy_fused > 0) tun.fct_xy_fused = t->fct_xy_fused;
         VOF_HIP(hipStreamSynchronize(st));
         drop_graphs();
         return build_geos();
@@ -457,13 +462,22 @@ struct Ensemble : EnsBase {
     // ---- the FCT sweeps of step istep, in the order of :1384-1396; the last
     // one also does :1398-1401.  Where the z sweep is directly followed by the
     // x sweep (orders y-z-x and z-x-y) the pair is one k_fct_zx launch when
-    // fct_zx_fused == 2 and the x chunk is one it is built for.  mark(q) is
-    // called after each launch with the stage-time slot q of that launch.
+    // fct_zx_fused == 2 and the x chunk is one it is built for; likewise the x
+    // and y sweeps of the order x-y-z (k_fct_xy, fct_xy_fused, the march
+    // chunk).  mark(q) is called after each launch with the stage-time slot q
+    // of that launch.
     bool zx_fused(int r) const {
         const int c = tun.kz_fct_x;
         return r != 0 && tun.fct_zx_fused >= 2 && (c == 4 || c == 8 || c == 16);
     }
-    int fct_launches(int64_t istep) const { return zx_fused(int(istep % 3)) ? 2 : 3; }
+    bool xy_fused(int r) const {
+        const int c = tun.fct_chunk;
+        return r == 0 && tun.fct_xy_fused >= 2 && (c == 4 || c == 8 || c == 16);
+    }
+    int fct_launches(int64_t istep) const {
+        const int r = int(istep % 3);
+        return (zx_fused(r) || xy_fused(r)) ? 2 : 3;
+    }
     template <class Mark>
     void enqueue_fct(int64_t istep, Mark&& mark) {
         static const int order[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
@@ -475,6 +489,13 @@ struct Ensemble : EnsBase {
                 fsel ^= 1;
                 ++d;
                 mark(9);
+                continue;
+            }
+            if (dir == 0 && xy_fused(r)) {  // x, then y as sweep d + 1 (never the last)
+                (void)vof::launch_fct_xy(e, g_fxy, fsel, st);
+                fsel ^= 1;
+                ++d;
+                mark(10);
                 continue;
             }
             vof::Geo& g = (dir == 0) ? g_fx : (dir == 1) ? g_fy : g_fz;
@@ -777,7 +798,7 @@ struct Ensemble : EnsBase {
             if (stage[2] < bestt[2]) { bestt[2] = stage[2]; best.kz_jacobi = c; best.jacobi_tblock = tun.jacobi_tblock; }
             if (stage[4] < bestt[4]) { bestt[4] = stage[4]; best.kz_correct = c; }
             // the x chunk also sets the fused z+x launches' chunk: x, z and z+x together
-            const double fx = stage[5] + stage[7] + stage[9];
+            const double fx = stage[5] + stage[7] + stage[9];  // x, z, z+x
             if (fx < bestt[5]) { bestt[5] = fx; best.kz_fct_x = c; }
         }
         // Jacobi temporal-block depth with its own chunk lengths
@@ -800,19 +821,22 @@ struct Ensemble : EnsBase {
             t.fct_chunk = c;
             double stage[VOF_NSTAGE_TIMES];
             rc = trial(t, stage);
-            if (rc == VOF_OK && stage[6] + stage[7] < bestm) { bestm = stage[6] + stage[7]; best.fct_chunk = c; }
+            const double fm = stage[6] + stage[7] + stage[10];  // y, z, x+y
+            if (rc == VOF_OK && fm < bestm) { bestm = fm; best.fct_chunk = c; }
         }
-        // z and x sweeps in one launch or two, with the chosen chunks
-        double fz[2] = {1e300, 1e300};
-        for (int k = 0; k < 2; ++k) {
-            if (rc != VOF_OK) break;
-            vof_tuning t = best;
-            t.fct_zx_fused = k + 1;
-            double stage[VOF_NSTAGE_TIMES];
-            rc = trial(t, stage);
-            if (rc == VOF_OK) fz[k] = stage[5] + stage[6] + stage[7] + stage[9];
+        // sweep pairs in one launch or two, with the chosen chunks
+        for (int pair = 0; pair < 2; ++pair) {
+            double fz[2] = {1e300, 1e300};
+            for (int k = 0; k < 2; ++k) {
+                if (rc != VOF_OK) break;
+                vof_tuning t = best;
+                (pair ? t.fct_xy_fused : t.fct_zx_fused) = k + 1;
+                double stage[VOF_NSTAGE_TIMES];
+                rc = trial(t, stage);
+                if (rc == VOF_OK) fz[k] = stage[5] + stage[6] + stage[7] + stage[9] + stage[10];
+            }
+            if (rc == VOF_OK) (pair ? best.fct_xy_fused : best.fct_zx_fused) = (fz[1] <= fz[0]) ? 2 : 1;
         }
-        if (rc == VOF_OK) best.fct_zx_fused = (fz[1] <= fz[0]) ? 2 : 1;
         // restore the state and apply the winner
         const hipError_t re = hipMemcpyAsync(arena, snap, arena_bytes, hipMemcpyDeviceToDevice, st);
         (void)hipStreamSynchronize(st);
@@ -845,7 +869,8 @@ struct Ensemble : EnsBase {
             // Per-stage device time: an event after every stage of eager steps.
             // Slot q of ev brackets stage q-1; stages follow the order of the
             // header (predictor, rhs, jacobi, p-mean, correction, fct x, y, z,
-            // ghosts, fused fct z+x); slots 6.. take the FCT launches of a step.
+            // ghosts, fused fct z+x, fused fct x+y); slots 6.. take the FCT
+            // launches of a step.
             constexpr int NS = VOF_NSTAGE_TIMES;
             for (int q = 0; q < NS; ++q) stage_ms[q] = 0.0;
             hipEvent_t ev[NS + 1];

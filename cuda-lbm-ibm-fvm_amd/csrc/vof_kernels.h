@@ -49,6 +49,9 @@ template <class R> void launch_fct(const Ens<R>& e, const Geo& g, int dir, int s
 // z sweep followed by the x sweep in one launch, chunks of g.kz in {4, 8, 16}
 // planes (false, and nothing launched, for any other chunk).
 template <class R> bool launch_fct_zx(const Ens<R>& e, const Geo& g, int src, int fin, hipStream_t st);
+// x sweep followed by the y sweep in one launch, chunks of g.kz in {4, 8, 16}
+// rows (false, and nothing launched, for any other chunk).
+template <class R> bool launch_fct_xy(const Ens<R>& e, const Geo& g, int src, hipStream_t st);
 template <class R> void launch_clamp_F(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
 template <class R> void launch_ghost_F(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
 template <class R> void launch_sub_mean(const Ens<R>& e, const Geo& g, int psel, const double* means, hipStream_t st);
@@ -75,6 +78,14 @@ inline int blocks_wcol_pad8(int nx, int ny, int nzrange, int kz, int h) {
     const int s = 64 - 2 * h;
     const int wpp = (plane + s - 1) / s;
     return ((((wpp + 3) / 4) + 7) & ~7) * ((nzrange + kz - 1) / kz);
+}
+// k_fct_xy: overlapped waves (halo 3) over the flattened (x, z) plane of
+// interior planes, padded to a multiple of 8 work-groups, one such row of
+// work-groups per chunk of ch rows along y.
+inline int blocks_xy(int nx, int ny, int nz, int ch) {
+    const int lines = (nx + 2) * nz;
+    const int wpp = (lines + 57) / 58;
+    return ((((wpp + 3) / 4) + 7) & ~7) * ((ny + ch - 1) / ch);
 }
 // Two-sweep Jacobi tiles: 60 stored lanes x 14 stored rows per work-group
 // (1024 lanes), one work-group row per chunk of kz planes.

@@ -755,15 +755,12 @@ __device__ __forceinline__ void store_F(const Inst<R>& I, const Buf<R>& Fo, int 
 // the wave); each wave walks kz planes.  Row ends need no care: ghost lanes
 // carry the reference's zero conventions, which decouple consecutive rows.
 //
-// One plane k of a wave, from F and u of the plane (every lane active, so that
-// each shuffle reads a live lane).
+// x-sweep value of the lane's cell from F and u of its row (every lane
+// active, so that each shuffle reads a live lane); before the last sweep's
+// second clamp, and +0 where the lane is not an interior cell.
 template <class R>
-__device__ __forceinline__ void fct_x_plane(const Inst<R>& I, const Buf<R>& Fo, const R f0, const R u0, bool cell,
-                                            bool store, int i, int j, int k, int fin) {
-    if (__all(f0 == R(0))) {
-        if (store) store_F(I, Fo, i, j, k, R(0), fin != 0);
-        return;
-    }
+__device__ __forceinline__ R fct_x_value(const Inst<R>& I, const R f0, const R u0, bool cell) {
+    if (__all(f0 == R(0))) return R(0);
     const R fm = sh_up(f0);
     const R up = sh_dn(u0);
     const R vdt = u0 * I.dt;
@@ -795,10 +792,15 @@ __device__ __forceinline__ void fct_x_plane(const Inst<R>& I, const Buf<R>& Fo, 
     const R c1 = cell ? ((a1 >= R(0)) ? vmin(rp1, rm) : vmin(rp, rm1)) : R(0);
     const R ac1 = a1 * c1;
     const R ac0 = sh_up(ac1);
-    if (store) {
-        const R fn = clamp01(ftd - ((ac1 - ac0) / I.dy) * I.vol / dv);
-        store_F(I, Fo, i, j, k, fin ? clamp01(fn) : fn, fin != 0);
-    }
+    return cell ? clamp01(ftd - ((ac1 - ac0) / I.dy) * I.vol / dv) : R(0);
+}
+
+// One plane k of a wave: the x sweep of the plane and its store.
+template <class R>
+__device__ __forceinline__ void fct_x_plane(const Inst<R>& I, const Buf<R>& Fo, const R f0, const R u0, bool cell,
+                                            bool store, int i, int j, int k, int fin) {
+    const R fn = fct_x_value(I, f0, u0, cell);
+    if (store) store_F(I, Fo, i, j, k, fin ? clamp01(fn) : fn, fin != 0);
 }
 
 // Any chunk length: planes streamed with F and u fetched one plane ahead.
@@ -1109,6 +1111,79 @@ __global__ __launch_bounds__(256) void k_fct_zx(const EnsArg<R> ea,
         const int c = m0 - 2 + s;
         const R fz = fct_win_step<R, 2>(I, w, c, nd, fw[s + 2], vw[s + 1], s >= 4);
         if (s >= 4 && c - 2 <= m1) fct_x_plane(I, Fo, line ? fz : R(0), uw[s - 4], line, store, i, j, c - 2, fin);
+    }
+}
+
+// x sweep directly followed by the y sweep, in one launch (the sweep order
+// x-y-z of :1384-1396; neither is the last sweep of the step).  Lanes run
+// along x over the flattened (x, z) plane of interior planes in overlapped
+// waves (halo 3, 58 stored lanes) and march along y through a chunk of CH
+// rows as in k_fct_line<R, 1, CH>.  Each row that enters the march window is
+// first swept along x by lane shifts; the intermediate F never goes to
+// memory.  Rows outside the interior (ghost rows, clamped window rows) and
+// ghost columns enter the y march as +0: the two-launch form reads the
+// intermediate buffer's ghost cells there, whose values it does not depend on
+// (at a wall face v = +0, so both fluxes through it are signed zeros whose
+// sign cancels, and ghost cells' own Ftd, ratios and limiters are zero).
+template <class R, int CH>
+__global__ __launch_bounds__(256) void k_fct_xy(const EnsArg<R> ea,
+                                                int src) {
+    VOF_PROLOGUE_RR(R)
+    constexpr int H = 3, SW = 64 - 2 * H;
+    const int P = I.sx * I.nz;
+    const int wpp = (P + SW - 1) / SW;
+    const int bpp = (((wpp + 3) >> 2) + 7) & ~7;  // multiple of 8: a line's chunks share an XCD
+    const int chn = lb / bpp;
+    const int wv = (lb - chn * bpp) * 4 + ((int)threadIdx.x >> 6);
+    if (wv >= wpp) return;  // wave-uniform
+    const int lane = (int)threadIdx.x & 63;
+    const int t = wv * SW - H + lane;
+    const int tc = min(max(t, 0), P - 1);
+    const int k = 1 + tc / I.sx;
+    const int i = tc - (k - 1) * I.sx;
+    const bool col = (t == tc) && i >= 1 && i <= I.nx;  // an interior line
+    const bool store = col && lane >= H && lane < 64 - H;
+    const int nd = I.ny, S = I.sx, base = k * I.sy + i;
+    const Buf<R> Fb(sel2(I.F, src), I.ntot), Fo(sel2(I.F, src ^ 1), I.ntot), Ub(I.u, I.ntot), Vb(I.v, I.ntot);
+    auto cl = [nd](int mm) { return mm < 0 ? 0 : (mm > nd + 1 ? nd + 1 : mm); };
+    const int m0 = 1 + chn * CH;
+    const int m1 = min(nd, m0 + CH - 1);
+    constexpr int NF = CH + 6, NV = CH + 5;
+    R fw[NF], uw[NF], vw[NV];  // F and u of rows m0-3+e, v of rows m0-2+e
+#pragma unroll
+    for (int e = 0; e < NF; ++e) fw[e] = Fb.ld(base, cl(m0 - 3 + e) * S);
+    bool gas = true;
+#pragma unroll
+    for (int e = 0; e < NF; ++e) gas = gas && (fw[e] == R(0));
+    if (__all(gas)) {  // every x-swept row is +0, so the y sweep stores +0
+        if (store) {
+#pragma unroll
+            for (int e = 0; e < CH; ++e)
+                if (m0 + e <= m1) Fo.st(R(0), base, (m0 + e) * S);
+        }
+        return;
+    }
+#pragma unroll
+    for (int e = 0; e < NF; ++e) uw[e] = Ub.ld(base, cl(m0 - 3 + e) * S);
+#pragma unroll
+    for (int e = 0; e < NV; ++e) vw[e] = Vb.ld(base, cl(m0 - 2 + e) * S);
+    // x-swept F of window row e (wave-uniform row index)
+    auto xrow = [&](int e) __attribute__((always_inline)) -> R {
+        const int mm = m0 - 3 + e;
+        if (mm < 1 || mm > nd) return R(0);
+        return fct_x_value(I, fw[e], uw[e], col);
+    };
+    FctWin<R> w;
+    {
+        const R f0 = xrow(0), f1 = xrow(1);
+        fct_win_init(I, w, f0, f1, vw[0]);
+    }
+#pragma unroll
+    for (int s = 0; s < CH + 4; ++s) {  // cell c = m0 - 2 + s
+        const int c = m0 - 2 + s;
+        const R fcp = xrow(s + 2);
+        const R fn = fct_win_step<R, 1>(I, w, c, nd, fcp, vw[s + 1], s >= 4);
+        if (s >= 4 && c - 2 <= m1 && store) Fo.st(fn, base, (c - 2) * S);
     }
 }
 
@@ -1442,6 +1517,17 @@ void launch_fct(const Ens<R>& e, const Geo& g, int dir, int src, int fin, hipStr
     else
         launch_march<R, 2>(e, g, src, fin, st);
 }
+// x sweep followed by the y sweep (k_fct_xy): reads F[src], writes F[src^1].
+template <class R>
+bool launch_fct_xy(const Ens<R>& e, const Geo& g, int src, hipStream_t st) {
+    const dim3 grid(g.nblk), blk(256);
+    switch (g.kz) {
+        case 4: hipLaunchKernelGGL((k_fct_xy<R, 4>), grid, blk, 0, st, ens_arg(e, g), src); return true;
+        case 8: hipLaunchKernelGGL((k_fct_xy<R, 8>), grid, blk, 0, st, ens_arg(e, g), src); return true;
+        case 16: hipLaunchKernelGGL((k_fct_xy<R, 16>), grid, blk, 0, st, ens_arg(e, g), src); return true;
+        default: return false;
+    }
+}
 // z sweep followed by the x sweep (k_fct_zx): reads F[src], writes F[src^1].
 template <class R>
 bool launch_fct_zx(const Ens<R>& e, const Geo& g, int src, int fin, hipStream_t st) {
@@ -1487,6 +1573,7 @@ void launch_reduce(const Ens<R>& e, const Geo& g, int fsel, int psel, int mode, 
     template void launch_correct<R>(const Ens<R>&, const Geo&, int, int, hipStream_t);                   \
     template void launch_fct<R>(const Ens<R>&, const Geo&, int, int, int, hipStream_t);                  \
     template bool launch_fct_zx<R>(const Ens<R>&, const Geo&, int, int, hipStream_t);                    \
+    template bool launch_fct_xy<R>(const Ens<R>&, const Geo&, int, hipStream_t);                         \
     template void launch_clamp_F<R>(const Ens<R>&, const Geo&, int, hipStream_t);                        \
     template void launch_ghost_F<R>(const Ens<R>&, const Geo&, int, hipStream_t);                        \
     template void launch_sub_mean<R>(const Ens<R>&, const Geo&, int, const double*, hipStream_t);        \

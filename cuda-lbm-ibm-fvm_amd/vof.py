@@ -32,7 +32,7 @@ FLAG_WATCHDOG = 0x10
 
 # Order of the per-stage times returned by Solver.time_steps(stages=True).
 STAGE_TIME_NAMES = ("predictor", "rhs", "jacobi", "p_mean", "correction", "fct_x", "fct_y", "fct_z", "ghost_F",
-                    "fct_zx")
+                    "fct_zx", "fct_xy")
 
 # Stages for Solver.debug_stage (test hook).
 (STAGE_PREDICT, STAGE_RHS, STAGE_JACOBI, STAGE_CORRECT, STAGE_FCT_X, STAGE_FCT_Y, STAGE_FCT_Z, STAGE_GHOST_F,
@@ -64,7 +64,7 @@ class VofAdjScalars(ctypes.Structure):
 class VofTuning(ctypes.Structure):
     _fields_ = [("kz_predict", c_int32), ("kz_rhs", c_int32), ("kz_jacobi", c_int32), ("kz_correct", c_int32),
                 ("fct_chunk", c_int32), ("jacobi_tblock", c_int32), ("kz_fct_x", c_int32),
-                ("fct_zx_fused", c_int32)]
+                ("fct_zx_fused", c_int32), ("fct_xy_fused", c_int32)]
 
 
 def default_params(**kw) -> VofParams:

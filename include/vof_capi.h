@@ -184,9 +184,9 @@ vof_status vof_sync(vof_solver* s);
  * with an event between stages and stage_ms[VOF_NSTAGE_TIMES] receives the
  * device time per stage summed over the timed steps, in the order
  * predictor, rhs, jacobi (all sweeps), p-mean removal, correction+BC,
- * fct x, fct y, fct z, F ghosts, fused fct z+x (steps whose z sweep runs in
- * one launch with the x sweep that follows it: see fct_zx_fused). */
-#define VOF_NSTAGE_TIMES 10
+ * fct x, fct y, fct z, F ghosts, fused fct z+x and fused fct x+y (steps whose
+ * sweep pairs run in one launch: see fct_zx_fused and fct_xy_fused). */
+#define VOF_NSTAGE_TIMES 11
 vof_status vof_time_steps(vof_solver* s, int64_t first_istep, int warmup, int steps,
                           int use_graph, double* ms_total, double* stage_ms);
 
@@ -195,11 +195,13 @@ vof_status vof_time_steps(vof_solver* s, int64_t first_istep, int warmup, int st
  * the y/z FCT marches, jacobi_tblock = Jacobi sweeps fused per launch,
  * fct_zx_fused = FCT sweeps per launch where the z sweep is directly followed
  * by the x sweep (sweep orders y-z-x and z-x-y; 2 = one launch with kz_fct_x
- * planes per chunk, 1 = two launches; the taped steps of reverse mode always
- * use two).  Values <= 0 keep the current setting.  Changing them drops
- * captured graphs. */
+ * planes per chunk, 1 = two launches), fct_xy_fused the same for the x and y
+ * sweeps of the order x-y-z (2 = one launch with fct_chunk rows per chunk);
+ * the taped steps of reverse mode always launch the sweeps one by one.
+ * Values <= 0 keep the current setting.  Changing them drops captured graphs. */
 typedef struct vof_tuning {
     int32_t kz_predict, kz_rhs, kz_jacobi, kz_correct, fct_chunk, jacobi_tblock, kz_fct_x, fct_zx_fused;
+    int32_t fct_xy_fused;
 } vof_tuning;
 vof_status vof_set_tuning(vof_solver* s, const vof_tuning* t);
 vof_status vof_get_tuning(const vof_solver* s, vof_tuning* t);

@@ -180,20 +180,22 @@ def test_launch_shapes_match_oracle(vofmod, orc, prec, shape):
 @pytest.mark.parametrize("prec", [4, 8])
 @pytest.mark.parametrize("kz", [4, 8, 16])
 @pytest.mark.parametrize("fused", [1, 2])
-def test_fct_zx_fused_matches_oracle(vofmod, orc, prec, kz, fused):
-    """Sweep orders y-z-x and z-x-y with the z and x sweeps in one launch
-    (fused 2) or two (fused 1): whole steps bit for bit, F ghosts included, on
-    a grid with partial z chunks, a large CFL and random F in the ghost layer
-    (which the single launch never reads for its x part).  Steps 1..4 cover
-    the fused pair as the last sweeps of a step (order y-z-x, which also
-    clamps twice and writes the ghosts), as the first (z-x-y), and the
-    unfused order x-y-z; eager and graph replay."""
-    print(f"fct z+x prec={prec} kz_fct_x={kz} fused={fused}", flush=True)
+def test_fct_pairs_fused_match_oracle(vofmod, orc, prec, kz, fused):
+    """FCT sweep pairs in one launch (fused 2) or two (fused 1): z then x in
+    the orders y-z-x and z-x-y (chunks of kz planes), x then y in x-y-z
+    (chunks of kz rows).  Whole steps bit for bit, F ghosts included, on a grid
+    with partial chunks, a large CFL and random F in the ghost layer (which a
+    fused launch never reads for its second sweep).  Steps 1..4 cover z+x as
+    the last sweeps of a step (order y-z-x, which also clamps twice and writes
+    the ghosts), z+x as the first (z-x-y) and x+y as the first (x-y-z); eager
+    and graph replay."""
+    print(f"fct pairs prec={prec} chunk={kz} fused={fused}", flush=True)
     rng = np.random.default_rng(SEED + 17 + kz)
     for graph in (False, True):
         s, o = make_pair(vofmod, orc, nx=37, ny=45, nz=27, precision=prec, dt=2e-3)
-        s.set_tuning(kz_fct_x=kz, fct_zx_fused=fused)
-        assert s.get_tuning()["fct_zx_fused"] == fused
+        s.set_tuning(kz_fct_x=kz, fct_chunk=kz, fct_zx_fused=fused, fct_xy_fused=fused)
+        t = s.get_tuning()
+        assert (t["fct_zx_fused"], t["fct_xy_fused"]) == (fused, fused)
         st = random_state(s.shape, rng, s.dtype, vel=1.5)
         load_state(s, o, st)
         for step in range(1, 5):
