@@ -738,6 +738,53 @@ __device__ __forceinline__ void store_F_ghosts(const Inst<R>& I, const Buf<R>& F
                 if (a | b | c) Fo.st(val, kk[c] * sy + jj[b] * sx + ii[a], 0);
 }
 
+// Ghost copies in column iw of the value of interior cell (i, j, k), i = iw
+// or its interior neighbour: (iw, jj, kk) for jj in {j, j's ghost row} and kk
+// in {k, k's ghost plane}, except (iw, j, k) itself.  With iw = i these are the
+// cell's j and k ghosts; with iw = 0 or nx+1 the edges and corners of its i
+// ghost.  Together with the i ghost (iw, j, k) they are store_F_ghosts.
+template <class R>
+__device__ __forceinline__ void store_F_ghosts_jk(const Inst<R>& I, const Buf<R>& Fo, int iw, int j, int k, R val) {
+    const int sx = I.sx, sy = I.sy;
+    if (!(j == 1 || j == I.ny || k == 1 || k == I.nz)) return;
+    int jj[3], kk[3], nj = 0, nk = 0;
+    jj[nj++] = j; if (j == 1) jj[nj++] = 0; if (j == I.ny) jj[nj++] = I.ny + 1;
+    kk[nk++] = k; if (k == 1) kk[nk++] = 0; if (k == I.nz) kk[nk++] = I.nz + 1;
+    for (int b = 0; b < nj; ++b)
+        for (int c = 0; c < nk; ++c)
+            if (b | c) Fo.st(val, kk[c] * sy + jj[b] * sx + iw, 0);
+}
+
+// Last-sweep store of one row segment of a wave whose lanes run along x
+// (every lane active, so that each shuffle reads a live lane); vo, so: the
+// lane's offset and the scalar offset of the row.  Interior lanes (store)
+// write their value and its j/k ghosts.  A ghost-column lane (i = 0 or nx+1)
+// with nb set takes the value of its interior neighbour lane (lane+1 at i = 0,
+// lane-1 at i = nx+1), which must hold a final value, and writes it as the i
+// ghost in the same store instruction as the row, so the ghost shares the
+// row's cache line instead of costing a lone 4-byte write later, then that
+// value's edge and corner ghosts.  Where the ghost lane cannot (its neighbour
+// is in another wave or is a halo lane), the interior lane writes that column
+// itself: wlo at i = 1 (column 0), whi at i = nx (column nx+1).
+template <class R>
+__device__ __forceinline__ void store_F_row_fin(const Inst<R>& I, const Buf<R>& Fo, R val, bool store, bool nb,
+                                                bool wlo, bool whi, int i, int j, int k, int vo, int so) {
+    const R dn = sh_dn(val), up = sh_up(val);
+    const R v = nb ? ((i == 0) ? dn : up) : val;
+    if (store || nb) {
+        Fo.st(v, vo, so);
+        store_F_ghosts_jk(I, Fo, i, j, k, v);
+    }
+    if (store && wlo) {
+        Fo.st(val, vo - 1, so);
+        store_F_ghosts_jk(I, Fo, 0, j, k, val);
+    }
+    if (store && whi) {
+        Fo.st(val, vo + 1, so);
+        store_F_ghosts_jk(I, Fo, I.nx + 1, j, k, val);
+    }
+}
+
 // Store of cell (i,j,k), k wave-uniform; plus its ghost copies on the last sweep.
 template <class R>
 __device__ __forceinline__ void store_F(const Inst<R>& I, const Buf<R>& Fo, int i, int j, int k, R val,
@@ -963,6 +1010,20 @@ __device__ __forceinline__ MarchLine march_uniform(const Inst<R>& I, int lb) {
     return L;
 }
 
+// As march_lane, keeping the lanes of the ghost columns i = 0 and nx+1 (g set;
+// they march on their own column's values, which are never stored).
+template <class R, int DIR>
+__device__ __forceinline__ bool march_lane_g(const Inst<R>& I, int lb, MarchLine& L, bool& g) {
+    const int no = (DIR == 1) ? I.nz : I.ny;
+    const int t = (lb - L.chn * L.nbt) * 256 + (int)threadIdx.x;
+    if (t >= no * I.sx) return false;
+    L.o = 1 + t / I.sx;
+    L.i = t - (L.o - 1) * I.sx;
+    g = L.i < 1 || L.i > I.nx;
+    L.base = (DIR == 1) ? (L.o * I.sy + L.i) : (L.o * I.sx + L.i);
+    return true;
+}
+
 template <class R, int DIR>
 __device__ __forceinline__ bool march_lane(const Inst<R>& I, int lb, MarchLine& L) {
     const int no = (DIR == 1) ? I.nz : I.ny;
@@ -984,6 +1045,21 @@ __device__ __forceinline__ void march_put(const Inst<R>& I, const Buf<R>& Fo, co
     if (!gh) return;
     if (DIR == 1) store_F_ghosts(I, Fo, L.i, mm, L.o, val);
     else store_F_ghosts(I, Fo, L.i, L.o, mm, val);
+}
+
+// Last-sweep store of line element mm (wave-uniform) in k_fct_line, whose
+// ghost-column lanes stay active and whose lanes all hold final values (each
+// marches its own line): a ghost lane stores its neighbour's value where that
+// neighbour is in the same wave; where it is not (i = 0 on lane 63, i = nx+1
+// on lane 0), the interior lane writes that column itself.
+template <class R, int DIR>
+__device__ __forceinline__ void march_put_fin(const Inst<R>& I, const Buf<R>& Fo, const MarchLine& L, bool g,
+                                              int S, int mm, R val) {
+    const int lane = (int)threadIdx.x & 63;
+    const int j = (DIR == 1) ? mm : L.o, k = (DIR == 1) ? L.o : mm;
+    const bool nb = g && ((L.i == 0) ? (lane != 63) : (lane != 0));
+    store_F_row_fin(I, Fo, val, !g, nb, L.i == 1 && lane == 0, L.i == I.nx && lane == 63, L.i, j, k, L.base,
+                    mm * S);
 }
 
 // Any chunk length: the line is streamed through the window with F and V
@@ -1021,14 +1097,17 @@ __global__ __launch_bounds__(256) void k_fct_march(const EnsArg<R> ea,
 // Chunks of CH cells: the whole window of a chunk (F on CH+6 cells, V on CH+5)
 // is loaded into registers before the march, so every load of a wave is in
 // flight at once.  A wave whose F window is zero on every lane stores zeros
-// without loading V (all values of the chunk are +0, as in the march).
+// without loading V (all values of the chunk are +0, as in the march).  The
+// lanes of the ghost columns stay active so that the last sweep of a step can
+// store the i ghosts with the rows (march_put_fin).
 template <class R, int DIR, int CH>
 __global__ __launch_bounds__(256) void k_fct_line(const EnsArg<R> ea,
                                                   int src, int fin) {
     VOF_PROLOGUE_RR(R)
     MarchLine L = march_uniform<R, DIR>(I, lb);
     const int S = L.S, nd = L.nd, chn = L.chn;
-    if (!march_lane<R, DIR>(I, lb, L)) return;
+    bool g = false;  // ghost-column lane: marches, stores only i ghosts of the last sweep
+    if (!march_lane_g<R, DIR>(I, lb, L, g)) return;
     const Buf<R> Fb(sel2(I.F, src), I.ntot), Fo(sel2(I.F, src ^ 1), I.ntot), Vb((DIR == 1) ? I.v : I.w, I.ntot);
     auto cl = [nd](int mm) { return mm < 0 ? 0 : (mm > nd + 1 ? nd + 1 : mm); };
     const bool gh = fin != 0;
@@ -1040,11 +1119,14 @@ __global__ __launch_bounds__(256) void k_fct_line(const EnsArg<R> ea,
     for (int e = 0; e < NF; ++e) fw[e] = Fb.ld(L.base, cl(m0 - 3 + e) * S);
     bool gas = true;
 #pragma unroll
-    for (int e = 0; e < NF; ++e) gas = gas && (fw[e] == R(0));
+    for (int e = 0; e < NF; ++e) gas = gas && (g || fw[e] == R(0));
     if (__all(gas)) {
 #pragma unroll
-        for (int e = 0; e < CH; ++e)
-            if (m0 + e <= m1) march_put<R, DIR>(I, Fo, L, S, m0 + e, R(0), gh);
+        for (int e = 0; e < CH; ++e) {
+            if (m0 + e > m1) continue;
+            if (gh) march_put_fin<R, DIR>(I, Fo, L, g, S, m0 + e, R(0));
+            else if (!g) Fo.st(R(0), L.base, (m0 + e) * S);
+        }
         return;
     }
 #pragma unroll
@@ -1055,7 +1137,10 @@ __global__ __launch_bounds__(256) void k_fct_line(const EnsArg<R> ea,
     for (int s = 0; s < CH + 4; ++s) {  // cell c = m0 - 2 + s
         const int c = m0 - 2 + s;
         const R fn = fct_win_step<R, DIR>(I, w, c, nd, fw[s + 2], vw[s + 1], s >= 4);
-        if (s >= 4 && c - 2 <= m1) march_put<R, DIR>(I, Fo, L, S, c - 2, gh ? clamp01(fn) : fn, gh);
+        if (s >= 4 && c - 2 <= m1) {
+            if (gh) march_put_fin<R, DIR>(I, Fo, L, g, S, c - 2, clamp01(fn));
+            else if (!g) Fo.st(fn, L.base, (c - 2) * S);
+        }
     }
 }
 
@@ -1082,6 +1167,12 @@ __global__ __launch_bounds__(256) void k_fct_zx(const EnsArg<R> ea,
     const int i = m.i, j = m.j, nd = I.nz, S = I.sy;
     const bool line = (m.t == m.tc) && i >= 1 && i <= I.nx && j >= 1 && j <= I.ny;
     const bool store = line && m.own;
+    // last sweep: a stored ghost-column lane writes its neighbour's value as the i ghost when
+    // that neighbour is a stored lane too (lanes 3..60 hold final values; 0-2 and 61-63 are
+    // halo); otherwise the neighbour, stored by the adjacent wave on its lane 3 or 60, writes it
+    const bool gcol = m.own && (m.t == m.tc) && j >= 1 && j <= I.ny &&
+                      ((i == 0 && m.lane < 63 - 3) || (i == I.nx + 1 && m.lane > 3));
+    const bool wlo = i == 1 && m.lane == 3, whi = i == I.nx && m.lane == 63 - 3;
     const Buf<R> Fb(sel2(I.F, src), I.ntot), Fo(sel2(I.F, src ^ 1), I.ntot), Wb(I.w, I.ntot), Ub(I.u, I.ntot);
     auto cl = [nd](int mm) { return mm < 0 ? 0 : (mm > nd + 1 ? nd + 1 : mm); };
     const int m0 = m.k0, m1 = m.k1;
@@ -1093,10 +1184,11 @@ __global__ __launch_bounds__(256) void k_fct_zx(const EnsArg<R> ea,
 #pragma unroll
     for (int e = 0; e < NF; ++e) gas = gas && (!line || fw[e] == R(0));
     if (__all(gas)) {  // F' = 0 on every lane and plane, so the x sweep stores +0
-        if (store) {
 #pragma unroll
-            for (int e = 0; e < CH; ++e)
-                if (m0 + e <= m1) store_F(I, Fo, i, j, m0 + e, R(0), fin != 0);
+        for (int e = 0; e < CH; ++e) {
+            if (m0 + e > m1) continue;
+            if (fin) store_F_row_fin(I, Fo, R(0), store, gcol, wlo, whi, i, j, m0 + e, m.tc, (m0 + e) * S);
+            else if (store) Fo.st(R(0), m.tc, (m0 + e) * S);
         }
         return;
     }
@@ -1110,7 +1202,11 @@ __global__ __launch_bounds__(256) void k_fct_zx(const EnsArg<R> ea,
     for (int s = 0; s < CH + 4; ++s) {  // cell c = m0 - 2 + s
         const int c = m0 - 2 + s;
         const R fz = fct_win_step<R, 2>(I, w, c, nd, fw[s + 2], vw[s + 1], s >= 4);
-        if (s >= 4 && c - 2 <= m1) fct_x_plane(I, Fo, line ? fz : R(0), uw[s - 4], line, store, i, j, c - 2, fin);
+        if (s >= 4 && c - 2 <= m1) {
+            const R fx = fct_x_value(I, line ? fz : R(0), uw[s - 4], line);
+            if (fin) store_F_row_fin(I, Fo, clamp01(fx), store, gcol, wlo, whi, i, j, c - 2, m.tc, (c - 2) * S);
+            else if (store) Fo.st(fx, m.tc, (c - 2) * S);
+        }
     }
 }
 

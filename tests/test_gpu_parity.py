@@ -207,6 +207,46 @@ def test_fct_pairs_fused_match_oracle(vofmod, orc, prec, kz, fused):
         o.close()
 
 
+def _ghost_handoffs(nx, ny, nz):
+    """Where the last sweep's i ghosts change writer at a wave edge: the fused
+    z+x launch (waves of 64 lanes over j*(nx+2)+i storing lanes 3..60) and the
+    y and z line kernels (waves of 64 lanes over (o-1)*(nx+2)+i)."""
+    sx = nx + 2
+    return {
+        "zx: ghost i=0 on lane 60": any((j * sx) % 58 == 57 for j in range(1, ny + 1)),
+        "zx: ghost i=nx+1 on lane 3": any((j * sx + nx + 1) % 58 == 0 for j in range(1, ny + 1)),
+        "y line: ghost i=0 on lane 63": any(((k - 1) * sx) % 64 == 63 for k in range(1, nz + 1)),
+        "y line: ghost i=nx+1 on lane 0": any(((k - 1) * sx + nx + 1) % 64 == 0 for k in range(1, nz + 1)),
+        "z line: ghost i=0 on lane 63": any(((j - 1) * sx) % 64 == 63 for j in range(1, ny + 1)),
+        "z line: ghost i=nx+1 on lane 0": any(((j - 1) * sx + nx + 1) % 64 == 0 for j in range(1, ny + 1)),
+    }
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_last_sweep_ghosts_at_wave_edges(vofmod, orc, prec):
+    """The last FCT sweep of a step writes the i ghosts from the ghost-column
+    lanes (a lane shift of the neighbour's value, stored with the row), except
+    where the neighbour is in another wave or is a halo lane, where the
+    interior lane writes them.  On a 17x54x38 grid every such hand-off occurs
+    (checked here); three steps put each last-sweep kernel (z line, fused z+x,
+    y line) last once.  Every F value, ghosts included, bit for bit."""
+    nx, ny, nz = 17, 54, 38
+    cases = _ghost_handoffs(nx, ny, nz)
+    assert all(cases.values()), cases
+    print(f"last-sweep ghosts at wave edges prec={prec}", flush=True)
+    s, o = make_pair(vofmod, orc, nx=nx, ny=ny, nz=nz, precision=prec, dt=2e-3)
+    s.set_tuning(kz_fct_x=8, fct_chunk=8, fct_zx_fused=2, fct_xy_fused=2)
+    rng = np.random.default_rng(SEED + 23)
+    st = random_state(s.shape, rng, s.dtype, vel=1.5)
+    load_state(s, o, st)
+    for step in range(3, 6):  # sweep orders x-y-z, y-z-x, z-x-y
+        s.step(step, 1)
+        o.step(step, 1)
+        assert_same(s.download("F"), o.get("F"), f"F after step {step}")
+    s.close()
+    o.close()
+
+
 @pytest.mark.parametrize("prec", [4, 8])
 @pytest.mark.parametrize("kz", [3, 4, 8, 16])
 def test_jacobi2_tiles_and_chunks(vofmod, orc, kz, prec):
