@@ -712,7 +712,8 @@ __global__ __launch_bounds__(256) void k_correct_blk(const EnsArg<R> ea, int fse
 // a second clamp01 (not idempotent: clamp01 maps the tiny negatives it can
 // leave, -x, to +x), and apply_bc_F (:1122-1135), whose three passes leave every
 // ghost equal to the nearest interior cell: the lane of a boundary cell writes
-// its value to every ghost cell that clamps onto it.
+// its value to every ghost cell that clamps onto it (in the kernels that end a
+// plain step, the ghost-column lanes store the x ghosts: store_F_row_fin).
 
 template <class R, int DIR>
 __device__ __forceinline__ R fct_stage1_scale(const Inst<R>& I, R d) {
