@@ -247,6 +247,29 @@ def test_last_sweep_ghosts_at_wave_edges(vofmod, orc, prec):
     o.close()
 
 
+@pytest.mark.parametrize("grid", [(1, 7, 5), (3, 1, 6), (2, 2, 2), (5, 4, 1), (1, 1, 1), (63, 2, 3), (64, 3, 2)])
+def test_degenerate_grids(vofmod, orc, grid):
+    """Grids one to three cells thick in some direction (a lane that is both
+    the first and the last interior cell of its row, planes that are both the
+    first and the last) and rows of 63/64 cells (ghost columns at wave edges):
+    three whole steps, one per sweep order, with the default launch shapes
+    (fused FCT pairs, two-sweep Jacobi), every field bit for bit."""
+    nx, ny, nz = grid
+    print(f"degenerate grid {grid}", flush=True)
+    s, o = make_pair(vofmod, orc, nx=nx, ny=ny, nz=nz, precision=4, dt=2e-3)
+    rng = np.random.default_rng(SEED + 29)
+    st = random_state(s.shape, rng, s.dtype, vel=1.5)
+    load_state(s, o, st)
+    for step in range(1, 4):
+        s.step(step, 1, graph=(step == 2))
+        o.step(step, 1)
+        for k in ("F", "U", "V", "W"):
+            assert_same(s.download(k), o.get(k), f"{k} after step {step}")
+        assert_same(interior(s.download("P")), interior(o.get("P")), f"P after step {step}")
+    s.close()
+    o.close()
+
+
 @pytest.mark.parametrize("prec", [4, 8])
 @pytest.mark.parametrize("kz", [3, 4, 8, 16])
 def test_jacobi2_tiles_and_chunks(vofmod, orc, kz, prec):
