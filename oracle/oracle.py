@@ -16,7 +16,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# VOF_ORACLE_LIB names another build of the same source: the mutation tests
+# (tests/test_oracle_mutants.py) point it at copies with a planted mistake.
+LIB_PATH = os.environ.get("VOF_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 sys.path.insert(0, os.path.join(ROOT, "cuda-lbm-ibm-fvm_amd"))
 from vof import VofDiag, VofParams, default_params  # noqa: E402  (struct layouts only)
 

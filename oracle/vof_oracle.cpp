@@ -11,10 +11,15 @@
 // templated on the real type: float reproduces the reference, double is the
 // "all-FP64" run.  Citations are voftrial1.cu line numbers.
 //
-// Parity pins (tests/test_oracle.py): clamp01f KATs, IC cell counts, zero-velocity
-// and linear-field predictor identities, Jacobi shift commutation and direct-solve
-// fixed point, brute-force one-sweep matrix form, FCT boundedness / u=0 identity /
-// mirror symmetry / telescoping of sum(Ftd*dv), BC closed forms, VTK bytes.
+// Parity pins (tests/test_oracle.py, tests/test_oracle_pins.py): clamp01f KATs,
+// IC cell counts, zero-velocity, linear- and quadratic-field predictor identities
+// on non-cubic grids, axis-rotation equivariance of every y/z operator (predictor,
+// rhs, Jacobi, correction, FCT), Jacobi shift commutation, direct-solve fixed point
+// and one-sweep matrix form with a right-hand side, hand-derived FCT known answers
+// (downwind anti-diffusion, limiter, dilation, stage-1 clip), interface sharpness,
+// FCT boundedness / u=0 identity / mirror symmetry / telescoping of sum(Ftd*dv),
+// BC closed forms, diagnostics sums, VTK bytes.  tests/test_oracle_mutants.py
+// plants plausible mistakes in copies of this file and checks each fails a pin.
 #include "../include/vof_capi.h"
 
 #include <algorithm>
