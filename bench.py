@@ -444,7 +444,7 @@ def main():
     # performs two sweeps of 12 algorithmic B/cell each: the "effective" figure
     # of the roofline, beside the counter bytes (about half of it).
     launches = dict(LAUNCHES_PER_STEP)
-    sweeps_per_launch = 2 if tuning.get("jacobi_tblock", 1) == 2 else 1
+    sweeps_per_launch = 2 if tuning.get("jacobi_tblock", 1) >= 2 else 1
     launches["jacobi"] = LAUNCHES_PER_STEP["jacobi"] // sweeps_per_launch
     stage_bytes = dict(STAGE_BYTES, fct_zx=FCT_PAIR_BYTES, fct_xy=FCT_PAIR_BYTES)
     # FCT launches of the three sweep orders (:1384-1396), with fused pairs

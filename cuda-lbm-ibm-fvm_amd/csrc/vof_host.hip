@@ -189,6 +189,7 @@ struct Ensemble : EnsBase {
     vof_tuning tun{4, 4, 4, 4, 8, 1, 4, 2, 2};
     int fsel = 0, psel = 0;
     int sweeps = 10;
+    bool launch_failed = false;  // a launcher refused its launch shape (set while enqueueing)
     uint32_t flags = 0;
     hipStream_t st = nullptr;
     void* arena = nullptr;
@@ -393,6 +394,7 @@ struct Ensemble : EnsBase {
 
     vof_status set_tuning(const vof_tuning* t) override {
         if (!t) { set_err("null tuning"); return VOF_ERR_INVALID; }
+        const vof_tuning old = tun;
         if (t->kz_predict > 0) tun.kz_predict = t->kz_predict;
         if (t->kz_rhs > 0) tun.kz_rhs = t->kz_rhs;
         if (t->kz_jacobi > 0) tun.kz_jacobi = t->kz_jacobi;
@@ -404,7 +406,14 @@ struct Ensemble : EnsBase {
         if (t->fct_xy_fused > 0) tun.fct_xy_fused = t->fct_xy_fused;
         VOF_HIP(hipStreamSynchronize(st));
         drop_graphs();
-        return build_geos();
+        const vof_status rc = build_geos();
+        if (rc != VOF_OK) {  // keep the launch tables and the tuning they were built for together
+            const std::string why = g_err;
+            tun = old;
+            (void)build_geos();
+            set_err("%s", why.c_str());
+        }
+        return rc;
     }
     void get_tuning(vof_tuning* t) const override { *t = tun; }
 
@@ -485,14 +494,14 @@ struct Ensemble : EnsBase {
         for (int d = 0; d < 3; ++d) {
             const int dir = order[r][d];
             if (dir == 2 && zx_fused(r)) {  // z, then x as sweep d + 1
-                (void)vof::launch_fct_zx(e, g_fzx, fsel, d + 1 == 2, st);
+                if (!vof::launch_fct_zx(e, g_fzx, fsel, d + 1 == 2, st)) launch_failed = true;
                 fsel ^= 1;
                 ++d;
                 mark(9);
                 continue;
             }
             if (dir == 0 && xy_fused(r)) {  // x, then y as sweep d + 1 (never the last)
-                (void)vof::launch_fct_xy(e, g_fxy, fsel, st);
+                if (!vof::launch_fct_xy(e, g_fxy, fsel, st)) launch_failed = true;
                 fsel ^= 1;
                 ++d;
                 mark(10);
@@ -517,6 +526,16 @@ struct Ensemble : EnsBase {
         enqueue_fct(istep, [](int) {});                       // :1384-1401
     }
 
+    // A launcher that refused its launch shape (a fused FCT pair whose chunk it
+    // has no kernel for) sets launch_failed while the step is enqueued; the
+    // sweep pair did not run, so the step is reported failed.
+    vof_status launch_status() {
+        if (!launch_failed) return VOF_OK;
+        launch_failed = false;
+        set_err("no kernel for the fused FCT launch shape (kz_fct_x %d, fct_chunk %d)", tun.kz_fct_x, tun.fct_chunk);
+        return VOF_ERR_INVALID;
+    }
+
     vof_status replay(int64_t istep) {
         const int key = int(istep % 3) * 4 + fsel * 2 + psel;
         auto it = graphs.find(key);
@@ -526,11 +545,15 @@ struct Ensemble : EnsBase {
             VOF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             enqueue_step(istep);
             const hipError_t ce = hipStreamEndCapture(st, &g);
+            fsel = f0; psel = p0;  // capture recorded the step without running it
             if (ce != hipSuccess) {
                 set_err("HIP failure at %s:%d : %s", __FILE__, __LINE__, hipGetErrorString(ce));
                 return VOF_ERR_HIP;
             }
-            fsel = f0; psel = p0;  // capture recorded the step without running it
+            if (launch_failed) {
+                (void)hipGraphDestroy(g);
+                return launch_status();
+            }
             hipGraphExec_t ex = nullptr;
             VOF_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
             graph_defs.push_back(g);
@@ -550,6 +573,7 @@ struct Ensemble : EnsBase {
                 VOF_TRY(replay(first + s));
             } else {
                 enqueue_step(first + s);
+                VOF_TRY(launch_status());
             }
         }
         VOF_HIP(hipGetLastError());
@@ -682,6 +706,10 @@ struct Ensemble : EnsBase {
             set_err("state file: version %u, %u-byte values, grid %ux%ux%u, %u fields; this solver needs version %u, "
                     "%u-byte values, grid %ux%ux%u, %u fields", h[0], h[1], h[2], h[3], h[4], h[5], hdr[0], hdr[1],
                     hdr[2], hdr[3], hdr[4], hdr[5]);
+            return VOF_ERR_INVALID;
+        }
+        if (is < 0) {
+            set_err("state file: step index %lld is negative", (long long)is);
             return VOF_ERR_INVALID;
         }
         // the whole file must be there before the state is touched; then one
@@ -899,6 +927,7 @@ struct Ensemble : EnsBase {
                     ++nl;
                 });
                 VOF_HIP(me);
+                VOF_TRY(launch_status());
                 // F ghosts are written by the last sweep: the last slot stays ~0
                 VOF_HIP(hipEventRecord(ev[NS], st));
                 VOF_HIP(hipEventSynchronize(ev[NS]));
@@ -1303,6 +1332,14 @@ EnsBase* make_ensemble(const vof_params* p, int n, vof_status* st) {
         *st = validate(p[i]);
         if (*st != VOF_OK) return nullptr;
         if (p[i].precision != p[0].precision) { set_err("all instances must share one precision"); *st = VOF_ERR_INVALID; return nullptr; }
+        // one launch per stage serves every instance: the stage sequence
+        // (flags, sweep count, sweeps per launch) is the ensemble's, not per instance
+        if (p[i].flags != p[0].flags || p[i].jacobi_sweeps != p[0].jacobi_sweeps ||
+            p[i].jacobi_tblock != p[0].jacobi_tblock) {
+            set_err("instance %d: flags, jacobi_sweeps and jacobi_tblock must equal instance 0's", i);
+            *st = VOF_ERR_INVALID;
+            return nullptr;
+        }
     }
     EnsBase* e = nullptr;
     try {
@@ -1430,6 +1467,9 @@ void vof_destroy(vof_solver* s) {
 }
 
 #define VOF_NEED(s) do { if (!(s) || !(s)->e) { set_err("null handle"); return VOF_ERR_INVALID; } } while (0)
+// Step indices are 1-based (the reference's ++istep, voftrial1.cu:1362) and
+// pick the sweep rotation istep % 3: a negative one is refused.
+#define VOF_NEED_STEP(i) do { if ((i) < 0) { set_err("step index %lld is negative", (long long)(i)); return VOF_ERR_INVALID; } } while (0)
 
 vof_status vof_init_ic(vof_solver* s, int ic) {
     VOF_NEED(s);
@@ -1446,10 +1486,12 @@ vof_status vof_init_box(vof_solver* s, double bx, double by, double bz) {
 
 vof_status vof_step(vof_solver* s, int64_t first_istep, int nsteps) {
     VOF_NEED(s);
+    VOF_NEED_STEP(first_istep);
     return s->e->step(first_istep, nsteps, false);
 }
 vof_status vof_step_graph(vof_solver* s, int64_t first_istep, int nsteps) {
     VOF_NEED(s);
+    VOF_NEED_STEP(first_istep);
     return s->e->step(first_istep, nsteps, true);
 }
 vof_status vof_download(vof_solver* s, vof_field f, void* host, size_t count) {
@@ -1471,6 +1513,7 @@ vof_status vof_sync(vof_solver* s) {
 vof_status vof_time_steps(vof_solver* s, int64_t first_istep, int warmup, int steps, int use_graph,
                           double* ms_total, double* stage_ms) {
     VOF_NEED(s);
+    VOF_NEED_STEP(first_istep);
     if (!ms_total || warmup < 0 || steps < 1) { set_err("time_steps: bad argument"); return VOF_ERR_INVALID; }
     return s->e->time_steps(first_istep, warmup, steps, use_graph != 0, ms_total, stage_ms);
 }
@@ -1486,11 +1529,13 @@ vof_status vof_get_tuning(const vof_solver* s, vof_tuning* t) {
 vof_status vof_autotune(vof_solver* s, int64_t first_istep, int steps_per_trial, const char* cache_path,
                         vof_tuning* out) {
     VOF_NEED(s);
+    VOF_NEED_STEP(first_istep);
     return s->e->autotune(first_istep, steps_per_trial, cache_path, out);
 }
 vof_status vof_batch_autotune(vof_batch* b, int64_t first_istep, int steps_per_trial, const char* cache_path,
                               vof_tuning* out) {
     VOF_NEED(b);
+    VOF_NEED_STEP(first_istep);
     return b->e->autotune(first_istep, steps_per_trial, cache_path, out);
 }
 vof_status vof_debug_stage(vof_solver* s, int stage) {
@@ -1580,6 +1625,7 @@ vof_status vof_write_vtk_async(vof_solver* s, const char* path) {
 
 vof_status vof_save_state(vof_solver* s, const char* path, int64_t istep) {
     VOF_NEED(s);
+    VOF_NEED_STEP(istep);
     if (!path) { set_err("null path"); return VOF_ERR_INVALID; }
     if (s->e->ninst() != 1) { set_err("state files hold one instance"); return VOF_ERR_INVALID; }
     VOF_HIP(hipStreamSynchronize(s->e->stream()));
@@ -1717,6 +1763,7 @@ vof_status vof_batch_init_ic(vof_batch* b, int ic) {
 }
 vof_status vof_batch_step(vof_batch* b, int64_t first_istep, int nsteps, int use_graph) {
     VOF_NEED(b);
+    VOF_NEED_STEP(first_istep);
     return b->e->step(first_istep, nsteps, use_graph != 0);
 }
 vof_status vof_batch_download(vof_batch* b, int inst, vof_field f, void* host, size_t count) {
@@ -1738,6 +1785,7 @@ vof_status vof_batch_sync(vof_batch* b) {
 vof_status vof_batch_time_steps(vof_batch* b, int64_t first_istep, int warmup, int steps, int use_graph,
                                 double* ms_total) {
     VOF_NEED(b);
+    VOF_NEED_STEP(first_istep);
     if (!ms_total || warmup < 0 || steps < 1) { set_err("time_steps: bad argument"); return VOF_ERR_INVALID; }
     return b->e->time_steps(first_istep, warmup, steps, use_graph != 0, ms_total, nullptr);
 }
@@ -1751,6 +1799,7 @@ int vof_batch_size(const vof_batch* b) { return (b && b->e) ? b->e->ninst() : 0;
 // ---------------------------------------------------------------- reverse mode
 vof_status vof_adj_forward(vof_solver* s, int64_t first_istep, int nsteps, int checkpoint_every, double* J) {
     VOF_NEED(s);
+    VOF_NEED_STEP(first_istep);
     return s->e->adj_forward(first_istep, nsteps, checkpoint_every, J);
 }
 vof_status vof_adj_seed(vof_solver* s, vof_field f, const double* host, size_t count) {

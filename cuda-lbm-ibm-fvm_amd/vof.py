@@ -129,6 +129,7 @@ def load_library(path: str = LIB_PATH):
         "vof_save_state": (c_int, [S, c_char_p, c_int64]),
         "vof_load_state": (c_int, [S, c_char_p, POINTER(c_int64)]),
         "vof_write_vtk_host": (c_int, [c_char_p, c_void_p, c_int, c_int, c_int, c_int]),
+        "vof_debug_vtk_thread_limit": (None, [c_int]),
         "vof_copy_bandwidth": (c_int, [c_int64, c_int, POINTER(c_double)]),
         "vof_sync": (c_int, [S]),
         "vof_time_steps": (c_int, [S, c_int64, c_int, c_int, c_int, POINTER(c_double), POINTER(c_double)]),

@@ -126,7 +126,9 @@ vof_status vof_init_box(vof_solver* s, double bx, double by, double bz);
 
 /* Advance nsteps time steps.  first_istep is the 1-based index of the first
  * step (the reference's ++istep, :1362); it selects the FCT sweep rotation
- * istep%3 (:1384-1396).  Replaces the loop body :1362-1401. */
+ * istep%3 (:1384-1396).  Replaces the loop body :1362-1401.  A negative
+ * first_istep or nsteps returns VOF_ERR_INVALID (as in every entry point that
+ * takes a step index). */
 vof_status vof_step(vof_solver* s, int64_t first_istep, int nsteps);
 
 /* Same as vof_step, but replays one captured hipGraph per step (captured on
@@ -170,6 +172,12 @@ vof_status vof_load_state(vof_solver* s, const char* path, int64_t* istep);
  * device access).  precision 4: data is float*, 8: double*. */
 vof_status vof_write_vtk_host(const char* path, const void* data, int precision, int nxTot, int nyTot,
                               int nzTot);
+
+/* Test hook of the VTK writer (no reference counterpart): n >= 0 lets at
+ * most n formatting threads start per batch of slabs, as if the system could
+ * start no more; the writer then formats the remaining slabs on the calling
+ * thread and the file is the same bytes.  n < 0 restores the default. */
+void vof_debug_vtk_thread_limit(int n);
 
 /* Attainable device bandwidth: a plain 16-byte-per-lane copy of `bytes`
  * bytes repeated `reps` times; *gbs = (read + written bytes) / time. */
@@ -233,7 +241,9 @@ const char* vof_last_error(void);
  * Batched ensemble: many instances of uneven grid size and parameters advance
  * together, one launch per stage for the whole ensemble (no reference
  * counterpart; the reference runs one fixed 200^3 case).
- * All instances share the precision and Jacobi sweep count of params[0].
+ * All instances share the precision, flags, Jacobi sweep count and Jacobi
+ * temporal-block depth of params[0]; vof_batch_create returns VOF_ERR_INVALID
+ * when any instance's precision, flags, jacobi_sweeps or jacobi_tblock differ.
  */
 typedef struct vof_batch vof_batch;
 vof_status vof_batch_create(const vof_params* params, int ninst, vof_batch** out);

@@ -92,6 +92,36 @@ def test_vtk_header_known_answer(vofmod, tmp_path):
     assert os.path.getsize(a) == len("\n".join(lines[:10]) + "\n") + 2 * 202 ** 3
 
 
+@pytest.mark.parametrize("limit", [0, 1, 3])
+def test_vtk_writer_survives_thread_start_failure(vofmod, tmp_path, limit):
+    """When formatting threads cannot be started (here: the writer's test hook
+    allows `limit` per batch of slabs), the remaining slabs are formatted on
+    the calling thread and the file is the same bytes."""
+    rng = np.random.default_rng(1)
+    F = rng.random((40, 200, 300)).astype(np.float32)  # 2.4 M values: three 2^20-value slabs
+    ref, got = tmp_path / "ref.vtk", tmp_path / "got.vtk"
+    lib = vofmod.load_library()
+    vofmod.write_vtk_host(str(ref), F)
+    lib.vof_debug_vtk_thread_limit(limit)
+    try:
+        vofmod.write_vtk_host(str(got), F)
+    finally:
+        lib.vof_debug_vtk_thread_limit(-1)
+    assert got.read_bytes() == ref.read_bytes()
+
+
+@pytest.mark.parametrize("field,value", [("flags", 0x2), ("jacobi_sweeps", 12), ("jacobi_tblock", 1)])
+def test_batch_refuses_instances_that_differ_in_the_stage_sequence(vofmod, field, value):
+    """One launch per stage serves every instance, so flags, sweep count and
+    sweeps per launch must be the same for all; refused before device use."""
+    p0 = vofmod.default_params(n=16, jacobi_tblock=2)
+    p1 = vofmod.default_params(n=24, jacobi_tblock=2)
+    setattr(p1, field, value)
+    with pytest.raises(vofmod.VofError) as e:
+        vofmod.Batch([p0, p1])
+    assert e.value.status == 1 and "instance 1" in str(e.value)
+
+
 REF_USAGE = ("Usage: ./3dvof_cuda [-ic 1|2|3] [--nstep N] [--max_steps N]\n"
              "  -ic         Initial condition selector. Only 1 is implemented, exactly as in the source.\n"
              "  --nstep     Dump interval.\n"

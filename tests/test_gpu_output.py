@@ -100,3 +100,29 @@ def test_cli_restart_and_async_output(vofmod, tmp_path):
     assert sorted(os.listdir(tmp_path / "a" / "output")) == [f"step-{k:05d}.vtk" for k in (5, 10, 15, 20)]
     assert (tmp_path / "b" / "output" / "step-00010.vtk").read_bytes() == \
         (tmp_path / "c" / "output" / "step-00010.vtk").read_bytes()
+
+
+def test_negative_step_indices_are_refused(vofmod, tmp_path):
+    """Step indices select the sweep rotation istep % 3 and are 1-based; a
+    negative one is refused by every entry point that takes one, and a
+    restart file holding one is refused on load with the state untouched."""
+    print("negative step indices, 12^3", flush=True)
+    s = vofmod.Solver(vofmod.default_params(n=12, precision=4))
+    s.init_ic(1)
+    for call in (lambda: s.step(-1, 1), lambda: s.step(-4, 1, graph=True), lambda: s.time_steps(-2, 0, 1),
+                 lambda: s.adj_forward(-1, 2, 1), lambda: s.save_state(str(tmp_path / "x.bin"), -3)):
+        with pytest.raises(vofmod.VofError) as e:
+            call()
+        assert e.value.status == 1 and "negative" in str(e.value)
+    path = tmp_path / "state.bin"
+    s.save_state(str(path), 5)
+    raw = bytearray(path.read_bytes())
+    raw[32:40] = (-7).to_bytes(8, "little", signed=True)  # the istep field after magic + 6 uint32
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(bytes(raw))
+    F0 = s.download("F")
+    with pytest.raises(vofmod.VofError, match="negative"):
+        s.load_state(str(bad))
+    assert np.array_equal(s.download("F"), F0)
+    assert s.load_state(str(path)) == 5
+    s.close()
