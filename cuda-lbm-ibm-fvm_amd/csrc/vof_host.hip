@@ -1543,6 +1543,55 @@ vof_status vof_debug_stage(vof_solver* s, int stage) {
     return s->e->debug_stage(stage);
 }
 
+extern "C++" {
+namespace {
+template <class R>
+vof_status diag_noalloc(const vof_params& p, const void* F, const void* u, const void* v, const void* w,
+                        const void* pres, const void* rhs, vof_diag* out, hipStream_t st) {
+    vof::Inst<R> I{};
+    Ensemble<R>::derive(p, I);
+    I.F[0] = I.F[1] = static_cast<R*>(const_cast<void*>(F));
+    I.u = static_cast<R*>(const_cast<void*>(u));
+    I.v = static_cast<R*>(const_cast<void*>(v));
+    I.w = static_cast<R*>(const_cast<void*>(w));
+    I.p[0] = I.p[1] = static_cast<R*>(const_cast<void*>(pres));
+    I.rhs = static_cast<R*>(const_cast<void*>(rhs));
+    // the chunk length vof_diagnostics uses, longer only where its launch
+    // would need more partials than the static array holds
+    int kz = 8;
+    while (vof::noalloc_blocks(p.nx, p.ny, p.nz, kz) > vof::kNoAllocMaxBlocks && kz < p.nz + 2) kz *= 2;
+    const int nblk = vof::noalloc_blocks(p.nx, p.ny, p.nz, kz);
+    if (nblk > vof::kNoAllocMaxBlocks) {
+        set_err("diagnostics_noalloc: %dx%d stored cells per plane need %d partials, more than the %d the static "
+                "scratch holds", p.nx + 2, p.ny + 2, nblk, vof::kNoAllocMaxBlocks);
+        return VOF_ERR_INVALID;
+    }
+    double* part = nullptr;
+    hipError_t e = vof::noalloc_scratch(&part);
+    if (e != hipSuccess) {
+        set_err("diagnostics_noalloc: scratch address lookup failed: %s", hipGetErrorString(e));
+        return VOF_ERR_HIP;
+    }
+    e = vof::launch_diag_noalloc<R>(I, kz, part, out, st);
+    if (e != hipSuccess) {
+        set_err("HIP failure at %s:%d : %s", __FILE__, __LINE__, hipGetErrorString(e));
+        return VOF_ERR_HIP;
+    }
+    return VOF_OK;
+}
+}  // namespace
+}  // extern "C++"
+
+vof_status vof_diagnostics_noalloc(const vof_params* p, const void* F, const void* u, const void* v, const void* w,
+                                   const void* pres, const void* rhs, vof_diag* out, void* stream) {
+    if (!p || !F || !u || !v || !w || !pres || !rhs || !out) { set_err("diagnostics_noalloc: null argument"); return VOF_ERR_INVALID; }
+    const vof_status st = validate(*p);
+    if (st != VOF_OK) return st;
+    hipStream_t hs = static_cast<hipStream_t>(stream);
+    return (p->precision == 8) ? diag_noalloc<double>(*p, F, u, v, w, pres, rhs, out, hs)
+                               : diag_noalloc<float>(*p, F, u, v, w, pres, rhs, out, hs);
+}
+
 vof_status vof_write_vtk(vof_solver* s, const char* path) {
     VOF_NEED(s);
     if (!path) { set_err("null path"); return VOF_ERR_INVALID; }

@@ -1,5 +1,6 @@
 // vof_kernels.h — host-side launchers of the gfx950 kernels (vof_kernels.hip).
 #pragma once
+#include "../../include/vof_capi.h"
 #include "vof_device.h"
 
 namespace vof {
@@ -38,6 +39,9 @@ inline EnsArg<R> ens_arg(const Ens<R>& e, const Geo& g) {
 }
 
 constexpr int kNumReduce = 7;
+// Largest k_reduce1 launch the allocation-free diagnostics serve: its
+// partials fill a static device array of this many work-groups x kNumReduce.
+constexpr int kNoAllocMaxBlocks = 65536;
 
 void launch_copy(const void* a, void* b, long n16, hipStream_t st);
 template <class R> void launch_predict(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
@@ -105,5 +109,13 @@ inline int blocks_ghost(int nx, int ny, int nz) {
     const long n = 2L * sx * (ny + 2) + 2L * sx * nz + 2L * ny * nz;
     return int((n + 255) / 256);
 }
+
+// Allocation-free diagnostics: per-device address of the static partials
+// array (looked up on first use), the work-groups a chunk length gives, and
+// the two-kernel launch writing a vof_diag record to device memory.
+hipError_t noalloc_scratch(double** out);
+int noalloc_blocks(int nx, int ny, int nz, int kz);
+template <class R>
+hipError_t launch_diag_noalloc(const Inst<R>& I, int kz, double* part, vof_diag* out, hipStream_t st);
 
 }  // namespace vof

@@ -24,7 +24,9 @@
 //    L2), except in the FCT sweeps, whose gas skip makes the work depend on
 //    the data and which use the dispatcher's round robin.
 #include <algorithm>
+#include <mutex>
 
+#include "../../include/vof_capi.h"
 #include "vof_device.h"
 #include "vof_kernels.h"
 
@@ -1482,13 +1484,9 @@ __global__ __launch_bounds__(256) void k_reduce1(const EnsArg<R> ea,
     }
 }
 
-// Pass 2: one work-group per instance.  out[inst*NRED + q].  If means is not
-// null, also stores the p mean (sum p / n^3) for k_sub_mean.
-__global__ __launch_bounds__(256) void k_reduce2(const int* __restrict__ boff, const double* __restrict__ part,
-                                                 const long long* __restrict__ ncell, double* __restrict__ out,
-                                                 double* __restrict__ means) {
-    const int inst = blockIdx.x;
-    const int b0 = boff[inst], b1 = boff[inst + 1];
+// Pass 2 body: the 256 threads of one work-group sum the partials of
+// work-groups [b0, b1) in a fixed order; thread q < NRED returns quantity q.
+__device__ __forceinline__ double reduce_parts(const double* __restrict__ part, int b0, int b1) {
     __shared__ double sh[4][NRED];
     double s[NRED] = {0.0, 0.0, 0.0, 0.0, 1e300, -1e300, 0.0};
     for (int b = b0 + (int)threadIdx.x; b < b1; b += 256) {
@@ -1505,13 +1503,95 @@ __global__ __launch_bounds__(256) void k_reduce2(const int* __restrict__ boff, c
         if (lane == 0) sh[wid][q] = r;
     }
     __syncthreads();
+    double r = 0.0;
     if (threadIdx.x < NRED) {
         const int q = threadIdx.x;
-        double r = sh[0][q];
+        r = sh[0][q];
         for (int w2 = 1; w2 < 4; ++w2) r = (q == 4) ? fmin(r, sh[w2][q]) : (q == 5) ? fmax(r, sh[w2][q]) : r + sh[w2][q];
+    }
+    return r;
+}
+
+// Pass 2: one work-group per instance.  out[inst*NRED + q].  If means is not
+// null, also stores the p mean (sum p / n^3) for k_sub_mean.
+__global__ __launch_bounds__(256) void k_reduce2(const int* __restrict__ boff, const double* __restrict__ part,
+                                                 const long long* __restrict__ ncell, double* __restrict__ out,
+                                                 double* __restrict__ means) {
+    const int inst = blockIdx.x;
+    const double r = reduce_parts(part, boff[inst], boff[inst + 1]);
+    if (threadIdx.x < NRED) {
+        const int q = threadIdx.x;
         out[(size_t)inst * NRED + q] = r;
         if (q == 1 && means) means[inst] = r / double(ncell[inst]);
     }
+}
+
+// Pass 2 of the allocation-free diagnostics: the partials of nblk
+// work-groups summed as k_reduce2 sums them, then turned into the caller's
+// vof_diag record in device memory (the same arithmetic the host applies to
+// k_reduce2's sums in vof_diagnostics).
+__global__ __launch_bounds__(256) void k_reduce_diag(const double* __restrict__ part, int nblk, double vol,
+                                                     double ncl, vof_diag* __restrict__ out) {
+    const double r = reduce_parts(part, 0, nblk);
+    switch (threadIdx.x) {
+        case 0: out->liquid_volume = vol * r; break;
+        case 1: out->p_mean = r / ncl; break;
+        case 2: out->residual_l2 = sqrt(r); break;
+        case 3: out->rhs_l2 = sqrt(r); break;
+        case 4: out->f_min = r; break;
+        case 5: out->f_max = r; break;
+        case 6: out->nonfinite = (int64_t)r; break;
+        default: break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free diagnostics (for callers that cannot allocate device
+// memory: code under stream capture, serving loops).  The per-work-group
+// partials of k_reduce1 live in `scratch`, a file-scope device array of
+// kNoAllocMaxBlocks x NRED doubles (3.7 MB) that each GPU holds until the
+// process ends.  Host code never uses the array's name as an address: the
+// first call on a device, with that device current, looks its address up
+// with hipGetSymbolAddress and keeps it per device.  Every call on one GPU
+// shares the array; k_reduce1 writes every partial that k_reduce_diag reads,
+// so nothing needs clearing, but calls that overlap on one GPU (different
+// streams, or threads) race on it.
+static __device__ __align__(256) double scratch[kNoAllocMaxBlocks * NRED];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch(double** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch[dev];
+    return hipSuccess;
+}
+
+// Work-groups of k_reduce1 for a chunk of kz planes over the stored planes
+// 0..nz+1 (as vof_diagnostics uses with kz = 8).
+int noalloc_blocks(int nx, int ny, int nz, int kz) { return blocks_wcol(nx, ny, nz + 2, kz, 0); }
+
+template <class R>
+hipError_t launch_diag_noalloc(const Inst<R>& I, int kz, double* part, vof_diag* out, hipStream_t st) {
+    const int nblk = noalloc_blocks(I.nx, I.ny, I.nz, kz);
+    const EnsArg<R> ea{nullptr, nullptr, 1, I};
+    hipLaunchKernelGGL(k_reduce1<R>, dim3(nblk), dim3(256), 0, st, ea, 0, 0, kz, 0, part);
+    hipLaunchKernelGGL(k_reduce_diag, dim3(1), dim3(256), 0, st, (const double*)part, nblk, double(I.vol),
+                       double(I.nx) * I.ny * I.nz, out);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -1676,7 +1756,8 @@ void launch_reduce(const Ens<R>& e, const Geo& g, int fsel, int psel, int mode, 
     template void launch_sub_mean<R>(const Ens<R>&, const Geo&, int, const double*, hipStream_t);        \
     template void launch_csf<R>(const Ens<R>&, const Geo&, int, hipStream_t);                            \
     template void launch_reduce<R>(const Ens<R>&, const Geo&, int, int, int, double*, const long long*, \
-                                   double*, double*, hipStream_t);
+                                   double*, double*, hipStream_t);                                     \
+    template hipError_t launch_diag_noalloc<R>(const Inst<R>&, int, double*, vof_diag*, hipStream_t);
 
 VOF_INSTANTIATE(float)
 VOF_INSTANTIATE(double)

@@ -123,6 +123,8 @@ def load_library(path: str = LIB_PATH):
         "vof_download": (c_int, [S, c_int, c_void_p, c_size_t]),
         "vof_upload": (c_int, [S, c_int, c_void_p, c_size_t]),
         "vof_diagnostics": (c_int, [S, POINTER(VofDiag)]),
+        "vof_diagnostics_noalloc": (c_int, [POINTER(VofParams), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                            c_void_p, c_void_p, c_void_p]),
         "vof_write_vtk": (c_int, [S, c_char_p]),
         "vof_write_vtk_async": (c_int, [S, c_char_p]),
         "vof_output_wait": (c_int, [S]),

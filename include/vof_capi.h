@@ -144,6 +144,27 @@ vof_status vof_upload(vof_solver* s, vof_field f, const void* host, size_t count
 /* FP64 diagnostics of the current state (synchronises). */
 vof_status vof_diagnostics(vof_solver* s, vof_diag* d);
 
+/* Diagnostics that allocate nothing (no counterpart in the reference, whose
+ * step has no reductions).  For callers that cannot allocate device memory:
+ * code under HIP stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  The same FP64 record as vof_diagnostics, computed from the
+ * caller's device fields in the reference layout ((nx+2)(ny+2)(nz+2) values
+ * of the field precision each: F, u, v, w, p and the Poisson right-hand side
+ * rhs), written to `out`, a vof_diag in device memory, by two kernels
+ * enqueued on `stream` (a hipStream_t; NULL = the null stream); the call does
+ * not wait for them.  No handle is needed and nothing is allocated: the
+ * per-work-group partial sums live in a static device array of 65536 x 7
+ * doubles (3.7 MB) that each GPU holds until the process ends, shared by
+ * every call on that GPU, so calls that overlap on one GPU race.  Grids
+ * whose reduction needs more than 65536 work-groups even with one chunk of
+ * all planes (a stored plane of more than 16,777,216 cells) are refused
+ * with VOF_ERR_INVALID.  Results equal
+ * vof_diagnostics' bit for bit on grids of up to 65536 work-groups with
+ * 8-plane chunks (e.g. 384^3); larger grids use longer chunks (same sums,
+ * other rounding order). */
+vof_status vof_diagnostics_noalloc(const vof_params* p, const void* F, const void* u, const void* v, const void* w,
+                                   const void* pres, const void* rhs, vof_diag* out, void* stream);
+
 /* Legacy ASCII VTK of F, byte-identical to write_legacy_vtk (:907-941). */
 vof_status vof_write_vtk(vof_solver* s, const char* path);
 
