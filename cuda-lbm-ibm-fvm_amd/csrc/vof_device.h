@@ -26,6 +26,8 @@ struct Inst {
     R rho_l, rho_g, nu_l, nu_g;
     R gx, gy, gz;
     R sigma;
+    R slip;   // sign of a tangential velocity ghost: +1 copies the interior (free slip,
+              // the reference's walls), -1 negates it (no-slip, VOF_FLAG_NO_SLIP)
     R* F[2];  // volume fraction, ping-pong across directional sweeps
     R* u;
     R* v;
@@ -107,10 +109,13 @@ __device__ __forceinline__ void Buf<double>::st(double v, int vo, int so) const 
 }
 
 // ---------------------------------------------------------------------------
-// Whole-wave lane shifts by one lane (DPP wave_shr:1 / wave_shl:1, one VALU
-// op each; the value entering at the wave's edge is 0).
-__device__ __forceinline__ int dpp_up_i(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false); }
-__device__ __forceinline__ int dpp_dn_i(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false); }
+// Whole-wave lane shifts by one lane (DPP wave_shr:1 / wave_shl:1; the value
+// entering at the wave's edge is 0).  The zero comes from bound_ctrl (a lane
+// whose source lane does not exist reads 0), not from a zeroed old value, so
+// the shift needs no initialising move and can be folded into the DPP
+// operand of the instruction that consumes it.
+__device__ __forceinline__ int dpp_up_i(int x) { return __builtin_amdgcn_mov_dpp(x, 0x138, 0xf, 0xf, true); }
+__device__ __forceinline__ int dpp_dn_i(int x) { return __builtin_amdgcn_mov_dpp(x, 0x130, 0xf, 0xf, true); }
 
 __device__ __forceinline__ float lane_up(float x) { return __int_as_float(dpp_up_i(__float_as_int(x))); }  // lane-1
 __device__ __forceinline__ float lane_dn(float x) { return __int_as_float(dpp_dn_i(__float_as_int(x))); }  // lane+1
@@ -119,6 +124,53 @@ __device__ __forceinline__ double lane_up(double x) {
 }
 __device__ __forceinline__ double lane_dn(double x) {
     return __hiloint2double(dpp_dn_i(__double2hiint(x)), dpp_dn_i(__double2loint(x)));
+}
+
+// ---------------------------------------------------------------------------
+// Division by a divisor whose reciprocal y = RN(1/b) is formed once (by an
+// IEEE division) and reused: Markstein's correction.  q0 = RN(a y) is within
+// two ulps of a/b; one correction with the residual r0 = a - b q0 (formed by
+// an FMA) gives a faithful q1; the residual of a faithful quotient is exact,
+// and a second correction then returns RN(a/b), the IEEE quotient (Markstein,
+// IBM J. Res. Dev. 34 (1990); Muller et al., Handbook of Floating-Point
+// Arithmetic, Thm. "Markstein").  The proof needs every intermediate in the
+// normal range: with |b| in [2^-40, 2^40] (ok_b, checked once per divisor)
+// that holds for |a| in [2^-60, 2^80]; any other a (0, tiny, huge, inf, NaN)
+// takes the IEEE division.  So the result is the IEEE quotient's bits for
+// every a and b; the common case costs one multiply and four FMAs instead of
+// the ~10-instruction scaled division with its quarter-rate reciprocal.
+__device__ __forceinline__ bool div_b_ok(float b) {
+    const unsigned u = __float_as_uint(b) & 0x7fffffffu;
+    return u - 0x2b800000u <= 0x53800000u - 0x2b800000u;  // 2^-40 <= |b| <= 2^40
+}
+__device__ __forceinline__ bool div_b_ok(double b) {
+    const unsigned u = unsigned(__double2hiint(b)) & 0x7fffffffu;
+    return u - 0x3d700000u <= 0x42700000u - 0x3d700000u;  // 2^-40 <= |b| < 2^40 (1 + 2^-20)
+}
+__device__ __forceinline__ bool div_a_ok(float a) {
+    const unsigned u = __float_as_uint(a) & 0x7fffffffu;
+    return u - 0x21800000u <= 0x67800000u - 0x21800000u;  // 2^-60 <= |a| <= 2^80
+}
+__device__ __forceinline__ bool div_a_ok(double a) {
+    const unsigned u = unsigned(__double2hiint(a)) & 0x7fffffffu;
+    return u - 0x3c300000u <= 0x44f00000u - 0x3c300000u;  // 2^-60 <= |a| < 2^80 (1 + 2^-20)
+}
+__device__ __forceinline__ float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+template <class R>
+__device__ __forceinline__ R div_rn(R a, R b, R y, bool ok_b) {
+    R q;
+    if (ok_b && div_a_ok(a)) {
+        const R q0 = a * y;
+        const R r0 = fma_r(-b, q0, a);
+        const R q1 = fma_r(r0, y, q0);
+        const R r1 = fma_r(-b, q1, a);
+        q = fma_r(r1, y, q1);
+    } else {
+        q = a / b;
+    }
+    return q;
 }
 
 template <class R> __device__ __forceinline__ R vmax(R a, R b);

@@ -185,7 +185,7 @@ struct Ensemble : EnsBase {
     std::vector<vof_params> prm;
     std::vector<vof::Inst<R>> ih;
     vof::Ens<R> e;
-    vof::Geo g_pred, g_rhs, g_jac, g_jac2, g_corr, g_fx, g_fy, g_fz, g_fzx, g_fxy, g_ghost, g_red, g_sub;
+    vof::Geo g_pred, g_rhs, g_jac, g_jac2, g_corr, g_fx, g_fy, g_fz, g_fzx, g_fxy, g_ghost, g_red, g_sub, g_cell;
     vof_tuning tun{4, 4, 4, 4, 8, 1, 4, 2, 2};
     int fsel = 0, psel = 0;
     int sweeps = 10;
@@ -286,6 +286,7 @@ struct Ensemble : EnsBase {
         I.nu_l = R(p.nu_l); I.nu_g = R(p.nu_g);
         I.gx = R(p.gx); I.gy = R(p.gy); I.gz = R(p.gz);
         I.sigma = R(p.sigma);
+        I.slip = (p.flags & VOF_FLAG_NO_SLIP) ? R(-1) : R(1);
     }
 
     vof_status make_geo(vof::Geo& g, int kz, int (*count)(const vof::Inst<R>&, int)) {
@@ -324,6 +325,8 @@ struct Ensemble : EnsBase {
             return vof::blocks_march(I.nx, I.nz, I.ny, ch); }));
         VOF_TRY(make_geo(g_ghost, 1, [](const vof::Inst<R>& I, int) {
             return vof::blocks_ghost(I.nx, I.ny, I.nz); }));
+        VOF_TRY(make_geo(g_cell, 1, [](const vof::Inst<R>& I, int) {
+            return vof::blocks_cells(I.nx, I.ny, I.nz); }));
         const size_t need = size_t(g_red.nblk) * vof::kNumReduce;
         if (need > part_cap) {
             if (part) (void)hipFree(part);
@@ -464,7 +467,10 @@ struct Ensemble : EnsBase {
 
     // ---- one step, the reference loop body (voftrial1.cu:1362-1401).
     void enqueue_predict() {
-        vof::launch_predict(e, g_pred, fsel, st);             // :1364-1368
+        if (flags & VOF_FLAG_RHO_PREDICTOR)
+            vof::launch_predict_rho(e, g_cell, fsel, st);      // density-weighted viscous term
+        else
+            vof::launch_predict(e, g_pred, fsel, st);          // :1364-1368
         if (flags & VOF_FLAG_CSF) vof::launch_csf(e, g_sub, fsel, st);
     }
 
@@ -597,7 +603,10 @@ struct Ensemble : EnsBase {
 
     vof_status debug_stage(int stage) override {
         switch (stage) {
-            case 0: vof::launch_predict(e, g_pred, fsel, st); break;
+            case 0:
+                if (flags & VOF_FLAG_RHO_PREDICTOR) vof::launch_predict_rho(e, g_cell, fsel, st);
+                else vof::launch_predict(e, g_pred, fsel, st);
+                break;
             case 1: vof::launch_rhs(e, g_rhs, fsel, st); break;
             case 2: vof::launch_jacobi(e, g_jac, psel, st); psel ^= 1; break;
             case 8: vof::launch_jacobi2(e, g_jac2, psel, st); psel ^= 1; break;
@@ -1320,8 +1329,10 @@ vof_status validate(const vof_params& p) {
     }
     if (p.jacobi_sweeps < 0) { set_err("jacobi_sweeps must be >= 0"); return VOF_ERR_INVALID; }
     if (!(p.dt > 0.0) || !(p.Lx > 0.0) || !(p.Ly > 0.0) || !(p.Lz > 0.0)) { set_err("dt and L must be positive"); return VOF_ERR_INVALID; }
-    if (p.flags & (VOF_FLAG_RHO_PREDICTOR | VOF_FLAG_NO_SLIP)) {
-        set_err("flags 0x%x are not available in this build", p.flags & (VOF_FLAG_RHO_PREDICTOR | VOF_FLAG_NO_SLIP));
+    const uint32_t known = VOF_FLAG_REMOVE_P_MEAN | VOF_FLAG_CSF | VOF_FLAG_RHO_PREDICTOR | VOF_FLAG_NO_SLIP |
+                           VOF_FLAG_WATCHDOG;
+    if (p.flags & ~known) {
+        set_err("flags 0x%x are not defined", p.flags & ~known);
         return VOF_ERR_UNSUPPORTED;
     }
     return VOF_OK;

@@ -45,6 +45,7 @@ constexpr int kNoAllocMaxBlocks = 65536;
 
 void launch_copy(const void* a, void* b, long n16, hipStream_t st);
 template <class R> void launch_predict(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
+template <class R> void launch_predict_rho(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
 template <class R> void launch_rhs(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
 template <class R> void launch_jacobi(const Ens<R>& e, const Geo& g, int psrc, hipStream_t st);
 template <class R> void launch_jacobi2(const Ens<R>& e, const Geo& g, int psrc, hipStream_t st);
@@ -93,6 +94,8 @@ inline int blocks_xy(int nx, int ny, int nz, int ch) {
 }
 // Two-sweep Jacobi tiles: 60 stored lanes x 14 stored rows per work-group
 // (1024 lanes), one work-group row per chunk of kz planes.
+// k_predict_rho: one lane per interior cell, 256 per work-group.
+inline int blocks_cells(int nx, int ny, int nz) { return int((int64_t(nx) * ny * nz + 255) / 256); }
 inline int blocks_jacobi2(int nx, int ny, int nz, int kz) {
     return ((nx + 59) / 60) * ((ny + 13) / 14) * ((nz + kz - 1) / kz);
 }

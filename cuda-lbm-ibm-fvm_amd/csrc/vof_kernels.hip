@@ -25,6 +25,7 @@
 //    the data and which use the dispatcher's round robin.
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 
 #include "../../include/vof_capi.h"
 #include "vof_device.h"
@@ -195,6 +196,98 @@ __global__ __launch_bounds__(256) void k_predict(const EnsArg<R> ea,
 }
 
 // ---------------------------------------------------------------------------
+// Predictor with the density-weighted viscous term (VOF_FLAG_RHO_PREDICTOR;
+// extension: the reference's predictor takes rho and never reads it, :233,
+// :290, :347, and uses the nu of one cell, :277-280).  The viscous
+// acceleration of a face between cells a (low) and b (high) along its axis d
+// is div(mu grad phi) / rho_f in flux form, mu = rho nu per cell:
+//   visc = ((T_x + T_y) + T_z) / (0.5 (rho_a + rho_b)),
+//   T_t  = (m+ (phi(+e_t) - phi) - m- (phi - phi(-e_t))) * h_t^-1 * h_t^-1,
+// with m+ = mu_b, m- = mu_a along d, and across it the edge averages
+// m+ = 0.25 ((mu_a + mu_b) + (mu_a(+e_t) + mu_b(+e_t))),
+// m- = 0.25 ((mu_a(-e_t) + mu_b(-e_t)) + (mu_a + mu_b)).  Advection and
+// gravity are the reference's (:253-284 and the v, w forms), in the same
+// association as the CPU oracle.  One lane per interior cell forms the faces
+// it owns (u* on face i >= 2, v* on j >= 2, w* on k >= 2); operands are read
+// straight from memory (this form is not on the measured path).
+template <class R>
+__device__ __forceinline__ R mu_of(const Inst<R>& I, R F) {
+    return rho_of(I, F) * nu_of(I, F);
+}
+
+template <class R, int D>
+__device__ __forceinline__ R visc_rho(const Inst<R>& I, const R* __restrict__ F, const R* __restrict__ phi, int c) {
+    const int off[3] = {1, I.sx, I.sy};
+    const R hi[3] = {I.dxi, I.dyi, I.dzi};
+    const int ca = c - off[D], cb = c;
+    const R Fa = F[ca], Fb = F[cb];
+    const R ma = mu_of(I, Fa), mb = mu_of(I, Fb);
+    R T[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        const int o = off[t];
+        R mp, mm;
+        if (t == D) {
+            mp = mb;
+            mm = ma;
+        } else {
+            mp = R(0.25) * ((ma + mb) + (mu_of(I, F[ca + o]) + mu_of(I, F[cb + o])));
+            mm = R(0.25) * ((mu_of(I, F[ca - o]) + mu_of(I, F[cb - o])) + (ma + mb));
+        }
+        T[t] = (mp * (phi[c + o] - phi[c]) - mm * (phi[c] - phi[c - o])) * hi[t] * hi[t];
+    }
+    const R rf = R(0.5) * (rho_of(I, Fa) + rho_of(I, Fb));
+    return ((T[0] + T[1]) + T[2]) / rf;
+}
+
+// Advected, viscous and gravity update of face c of component D (u, v, w).
+template <class R, int D>
+__device__ __forceinline__ R predict_face_rho(const Inst<R>& I, const R* __restrict__ F, const R* __restrict__ u,
+                                              const R* __restrict__ v, const R* __restrict__ w, int c) {
+    const int sx = I.sx, sy = I.sy;
+    const R* phi = (D == 0) ? u : (D == 1) ? v : w;
+    const R pc = phi[c];
+    R ax, ay, az, g;
+    if (D == 0) {  // :253-264
+        ax = pc;
+        ay = R(0.25) * (v[c - 1] + v[c - 1 + sx] + v[c] + v[c + sx]);
+        az = R(0.25) * (w[c - 1] + w[c - 1 + sy] + w[c] + w[c + sy]);
+        g = I.gx;
+    } else if (D == 1) {  // :310-321
+        ax = R(0.25) * (u[c - sx] + u[c] + u[c + 1 - sx] + u[c + 1]);
+        ay = pc;
+        az = R(0.25) * (w[c - sx + sy] + w[c - sx] + w[c] + w[c + sy]);
+        g = I.gy;
+    } else {  // :367-378
+        ax = R(0.25) * (u[c + 1 - sy] + u[c - sy] + u[c + 1] + u[c]);
+        ay = R(0.25) * (v[c + sx - sy] + v[c - sy] + v[c] + v[c + sx]);
+        az = pc;
+        g = I.gz;
+    }
+    const R ddx = (ax > R(0)) ? (pc - phi[c - 1]) * I.dxi : (phi[c + 1] - pc) * I.dxi;
+    const R ddy = (ay > R(0)) ? (pc - phi[c - sx]) * I.dyi : (phi[c + sx] - pc) * I.dyi;
+    const R ddz = (az > R(0)) ? (pc - phi[c - sy]) * I.dzi : (phi[c + sy] - pc) * I.dzi;
+    const R visc = visc_rho<R, D>(I, F, phi, c);
+    const R adv = -ax * ddx - ay * ddy - az * ddz;
+    return pc + I.dt * (visc + adv + g);
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_predict_rho(const EnsArg<R> ea, int fsel) {
+    VOF_PROLOGUE(R)
+    const int q = lb * 256 + (int)threadIdx.x;
+    const int nxy = I.nx * I.ny;
+    if (q >= nxy * I.nz) return;
+    const int k = 1 + q / nxy, r = q - (k - 1) * nxy;
+    const int j = 1 + r / I.nx, i = 1 + r - (j - 1) * I.nx;
+    const int c = k * I.sy + j * I.sx + i;
+    const R* __restrict__ F = sel2(I.F, fsel);
+    if (i >= 2) I.us[c] = predict_face_rho<R, 0>(I, F, I.u, I.v, I.w, c);
+    if (j >= 2) I.vs[c] = predict_face_rho<R, 1>(I, F, I.u, I.v, I.w, c);
+    if (k >= 2) I.ws[c] = predict_face_rho<R, 2>(I, F, I.u, I.v, I.w, c);
+}
+
+// ---------------------------------------------------------------------------
 // Poisson right-hand side rho/dt * div(u*) (the rhs term of
 // solve_p_jacobi_kernel, :418-422), formed once per step instead of in every
 // sweep; it depends only on u* and rho, so the sweeps see the same bits.
@@ -264,6 +357,44 @@ __global__ __launch_bounds__(256) void k_rhs_blk(const EnsArg<R> ea, int fsel) {
 }
 
 // ---------------------------------------------------------------------------
+// Jacobi point (solve_p_jacobi_kernel, :424-438): the numerator in the
+// reference's order, divided by ap = -(ae+aw+an+as+af+ab).  The divisor of a
+// lane takes only four values, one per kind of plane (interior, k = 1,
+// k = nz, nz = 1), so its reciprocal is formed once and every point divides
+// by Markstein's correction (div_rn: the IEEE quotient's bits, no per-point
+// reciprocal).  Boundary coefficients are zero times the (zero) ghost p,
+// kept in the expression so the rounding sequence is the reference's.
+template <class R>
+__device__ __forceinline__ R jac_num(R b, R ae, R aw, R an, R as, R af, R ab, R pe, R pw, R pn, R ps, R pf,
+                                     R pb) {
+    return b - ae * pe - aw * pw - an * pn - as * ps - af * pf - ab * pb;
+}
+
+template <class R>
+struct JacAp {
+    R in, lo, hi, one;  // ap per kind of plane, summed in the order of :430
+    R rin;              // RN(1/in)
+    bool kin;           // rin usable (div_b_ok)
+    __device__ __forceinline__ JacAp(R ae, R aw, R an, R as, R azz, int nz) {
+        in = -(ae + aw + an + as + azz + azz);
+        lo = -(ae + aw + an + as + ((nz != 1) ? azz : R(0)) + R(0));
+        hi = -(ae + aw + an + as + R(0) + ((nz != 1) ? azz : R(0)));
+        one = -(ae + aw + an + as + R(0) + R(0));
+        rin = R(1) / in;
+        kin = div_b_ok(in);
+    }
+    // num / ap on an interior plane (1 < s < nz): reused reciprocal
+    __device__ __forceinline__ R div_in(R num) const { return div_rn(num, in, rin, kin); }
+    // num / ap of plane s (s wave-uniform); the first and last planes (two
+    // of every column's nz) divide the IEEE way.  Bit-identical either way.
+    __device__ __forceinline__ R div(R num, int s, int nz) const {
+        if (s != 1 && s != nz) return div_in(num);
+        const R a_one = one, a_lo = lo, a_hi = hi;  // selected as values, not member addresses
+        return num / (nz == 1 ? a_one : (s == 1 ? a_lo : a_hi));
+    }
+};
+
+// ---------------------------------------------------------------------------
 // One Jacobi sweep (solve_p_jacobi_kernel, :401-439) with the precomputed rhs.
 // Boundary neighbours carry a zero coefficient times the (zero) ghost p, kept
 // in the expression so the rounding sequence is the reference's.
@@ -282,13 +413,8 @@ __global__ __launch_bounds__(256) void k_jacobi(const EnsArg<R> ea,
     const R aw = (i != 1) ? I.axx : R(0);
     const R an = (j != I.ny) ? I.ayy : R(0);
     const R as = (j != 1) ? I.ayy : R(0);
-    // ap = -(ae+aw+an+as+af+ab) for the three kinds of plane (k = 1, interior, k = nz),
-    // summed in the reference's order (:430)
     const R azz = I.azz;
-    const R ap_in = -(ae + aw + an + as + azz + azz);
-    const R ap_lo = -(ae + aw + an + as + ((I.nz != 1) ? azz : R(0)) + R(0));
-    const R ap_hi = -(ae + aw + an + as + R(0) + ((I.nz != 1) ? azz : R(0)));
-    const R ap_one = -(ae + aw + an + as + R(0) + R(0));
+    const JacAp<R> apk(ae, aw, an, as, azz, I.nz);
     int o = m.k0 * sy;
     const int olim = I.nz * sy;  // last plane offset whose +1 neighbour is stored
     R p_km = p.ld(tc, o - sy), p_k = p.ld(tc, o);
@@ -305,8 +431,7 @@ __global__ __launch_bounds__(256) void k_jacobi(const EnsArg<R> ea,
         if (out) {
             const R af = (k != I.nz) ? azz : R(0);
             const R ab = (k != 1) ? azz : R(0);
-            const R ap = (k != 1 && k != I.nz) ? ap_in : (I.nz == 1 ? ap_one : (k == 1 ? ap_lo : ap_hi));
-            pt.st((b - ae * pip - aw * pim - an * pjp - as * pjm - af * p_kp - ab * p_km) / ap, tc, o);
+            pt.st(apk.div(jac_num(b, ae, aw, an, as, af, ab, pip, pim, pjp, pjm, p_kp, p_km), k, I.nz), tc, o);
         }
         p_km = p_k;
         p_k = p_kp;
@@ -332,11 +457,9 @@ __global__ __launch_bounds__(256) void k_jacobi_blk(const EnsArg<R> ea, int psrc
     const R an = (j != I.ny) ? I.ayy : R(0);
     const R as = (j != 1) ? I.ayy : R(0);
     const R azz = I.azz;
-    const R ap_in = -(ae + aw + an + as + azz + azz);
-    const R ap_lo = -(ae + aw + an + as + ((I.nz != 1) ? azz : R(0)) + R(0));
-    const R ap_hi = -(ae + aw + an + as + R(0) + ((I.nz != 1) ? azz : R(0)));
-    const R ap_one = -(ae + aw + an + as + R(0) + R(0));
     const int k0 = m.k0, k1 = m.k1, nz = I.nz;
+    const bool interior = (k0 >= 2 && k0 + KZ - 1 <= nz - 1);  // full chunk of planes 2 .. nz-1
+    const JacAp<R> apk(ae, aw, an, as, azz, nz);
     R pc[KZ + 2], pjm[KZ], pjp[KZ], b[KZ];  // pc[e]: plane k0-1+e; the others: plane k0+e
 #pragma unroll
     for (int e = 0; e < KZ + 2; ++e) pc[e] = p.ld(tc, min(k0 - 1 + e, nz + 1) * sy);
@@ -347,20 +470,24 @@ __global__ __launch_bounds__(256) void k_jacobi_blk(const EnsArg<R> ea, int psrc
         pjp[e] = p.ld(tjp, o);
         b[e] = rhs.ld(tc, o);
     }
+    auto march = [&](auto in_tag) __attribute__((always_inline)) {
+        constexpr bool IN = decltype(in_tag)::value;
 #pragma unroll
-    for (int e = 0; e < KZ; ++e) {
-        const int k = k0 + e;
-        if (k > k1) break;
-        const R p_k = pc[e + 1];
-        const R pim = sh_up(p_k), pip = sh_dn(p_k);
-        if (out) {
-            const R af = (k != nz) ? azz : R(0);
-            const R ab = (k != 1) ? azz : R(0);
-            const R ap = (k != 1 && k != nz) ? ap_in : (nz == 1 ? ap_one : (k == 1 ? ap_lo : ap_hi));
-            pt.st((b[e] - ae * pip - aw * pim - an * pjp[e] - as * pjm[e] - af * pc[e + 2] - ab * pc[e]) / ap, tc,
-                  k * sy);
+        for (int e = 0; e < KZ; ++e) {
+            const int k = k0 + e;
+            if (!IN && k > k1) break;
+            const R p_k = pc[e + 1];
+            const R pim = sh_up(p_k), pip = sh_dn(p_k);
+            if (out) {
+                const R af = (IN || k != nz) ? azz : R(0);
+                const R ab = (IN || k != 1) ? azz : R(0);
+                const R num = jac_num(b[e], ae, aw, an, as, af, ab, pip, pim, pjp[e], pjm[e], pc[e + 2], pc[e]);
+                pt.st(IN ? apk.div_in(num) : apk.div(num, k, nz), tc, k * sy);
+            }
         }
-    }
+    };
+    if (interior) march(std::true_type{});
+    else march(std::false_type{});
 }
 
 // ---------------------------------------------------------------------------
@@ -376,31 +503,12 @@ __global__ __launch_bounds__(256) void k_jacobi_blk(const EnsArg<R> ea, int psrc
 // order, so two launches of this kernel equal four single sweeps bit for bit.
 constexpr int kJ2Rows = 16, kJ2OutRows = 14, kJ2OutLanes = 60;
 
-// One point of a sweep; ap = -(ae+aw+an+as+af+ab) in that order, formed once
-// per lane for each kind of plane (jac_ap).
-template <class R>
-__device__ __forceinline__ R jac_point(R b, R ae, R aw, R an, R as, R af, R ab, R pe, R pw, R pn, R ps, R pf,
-                                       R pb, R ap) {
-    return (b - ae * pe - aw * pw - an * pn - as * ps - af * pf - ab * pb) / ap;
-}
-
-// ap of a lane on the four kinds of plane (interior, k = 1, k = nz, nz = 1),
-// summed in the reference's order (:430) with the zero coefficients in place.
-template <class R>
-struct JacAp {
-    R in, lo, hi, one;
-    __device__ __forceinline__ JacAp(R ae, R aw, R an, R as, R azz, int nz) {
-        in = -(ae + aw + an + as + azz + azz);
-        lo = -(ae + aw + an + as + ((nz != 1) ? azz : R(0)) + R(0));
-        hi = -(ae + aw + an + as + R(0) + ((nz != 1) ? azz : R(0)));
-        one = -(ae + aw + an + as + R(0) + R(0));
-    }
-    __device__ __forceinline__ R at(int s, int nz) const {
-        return (s != 1 && s != nz) ? in : (nz == 1 ? one : (s == 1 ? lo : hi));
-    }
-};
-
-template <class R>
+// Streamed form for any chunk length: the operands of plane s + D (the p
+// column one plane further, p on the two neighbouring rows and rhs) are
+// requested while plane s is formed, so D planes of loads are in flight
+// behind the march; the prefetch ring is indexed by the plane modulo D with
+// the march unrolled D-fold, so every slot is a fixed register.
+template <class R, int D>
 __global__ __launch_bounds__(1024) void k_jacobi2(const EnsArg<R> ea,
                                                   int psrc, int kz) {
     VOF_PROLOGUE(R)
@@ -428,37 +536,50 @@ __global__ __launch_bounds__(1024) void k_jacobi2(const EnsArg<R> ea,
     const int olast = (nz + 1) * sy;
     auto po = [&](int s) { return min(max(s, 0) * sy, olast); };  // clamped plane offset
     // march state: sweep-0 column at s-1, s; sweep-1 column at s-2, s-1; rhs at s-1
-    int s = k0 - 1;
-    R q0m = p.ld(tc, po(s - 1)), q0 = p.ld(tc, po(s));
+    const int s_beg = k0 - 1, s_end = k1 + 1;
+    R q0m = p.ld(tc, po(s_beg - 1)), q0 = p.ld(tc, po(s_beg));
     R q1mm = R(0), q1m = R(0), bm = R(0);
-    R n_q0p = p.ld(tc, po(s + 1)), n_jm = p.ld(tjm, po(s)), n_jp = p.ld(tjp, po(s)), n_b = rhs.ld(tc, po(s));
-    for (; s <= k1 + 1; ++s) {
-        const R q0p = n_q0p, q0jm = n_jm, q0jp = n_jp, b = n_b;
-        n_q0p = p.ld(tc, po(s + 2));
-        n_jm = p.ld(tjm, po(s + 1));
-        n_jp = p.ld(tjp, po(s + 1));
-        n_b = rhs.ld(tc, po(s + 1));
-        // sweep 1 at plane s
-        const R q0w = sh_up(q0), q0e = sh_dn(q0);
-        R q1 = R(0);
-        if (in1 && s >= 1 && s <= nz) {
-            const R af = (s != nz) ? azz : R(0), ab = (s != 1) ? azz : R(0);
-            q1 = jac_point(b, ae, aw, an, as, af, ab, q0e, q0w, q0jp, q0jm, q0p, q0m, apk.at(s, nz));
+    R fq[D], fjm[D], fjp[D], fb[D];  // slot d: plane s_beg + d (column: one plane further)
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        fq[d] = p.ld(tc, po(s_beg + d + 1));
+        fjm[d] = p.ld(tjm, po(s_beg + d));
+        fjp[d] = p.ld(tjp, po(s_beg + d));
+        fb[d] = rhs.ld(tc, po(s_beg + d));
+    }
+    for (int s0 = s_beg; s0 <= s_end; s0 += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int s = s0 + d;
+            if (s > s_end) break;  // uniform over the work-group
+            const R q0p = fq[d], q0jm = fjm[d], q0jp = fjp[d], b = fb[d];
+            fq[d] = p.ld(tc, po(s + D + 1));
+            fjm[d] = p.ld(tjm, po(s + D));
+            fjp[d] = p.ld(tjp, po(s + D));
+            fb[d] = rhs.ld(tc, po(s + D));
+            // sweep 1 at plane s
+            const R q0w = sh_up(q0), q0e = sh_dn(q0);
+            R q1 = R(0);
+            if (in1 && s >= 1 && s <= nz) {
+                const R af = (s != nz) ? azz : R(0), ab = (s != 1) ? azz : R(0);
+                q1 = apk.div(jac_num(b, ae, aw, an, as, af, ab, q0e, q0w, q0jp, q0jm, q0p, q0m), s, nz);
+            }
+            const int slot = s % 3, pslot = (s + 2) % 3;
+            ring[slot][w][lane] = q1;
+            __syncthreads();
+            // sweep 2 at plane s-1
+            const int s2 = s - 1;
+            const R q1w = sh_up(q1m), q1e = sh_dn(q1m);
+            if (st2 && s2 >= k0) {
+                const R q1s = ring[pslot][w - 1][lane], q1n = ring[pslot][w + 1][lane];
+                const R af = (s2 != nz) ? azz : R(0), ab = (s2 != 1) ? azz : R(0);
+                pt.st(apk.div(jac_num(bm, ae, aw, an, as, af, ab, q1e, q1w, q1n, q1s, q1, q1mm), s2, nz), tc,
+                      s2 * sy);
+            }
+            q0m = q0; q0 = q0p;
+            q1mm = q1m; q1m = q1;
+            bm = b;
         }
-        const int slot = s % 3, pslot = (s + 2) % 3;
-        ring[slot][w][lane] = q1;
-        __syncthreads();
-        // sweep 2 at plane s-1
-        const int s2 = s - 1;
-        const R q1w = sh_up(q1m), q1e = sh_dn(q1m);
-        if (st2 && s2 >= k0) {
-            const R q1s = ring[pslot][w - 1][lane], q1n = ring[pslot][w + 1][lane];
-            const R af = (s2 != nz) ? azz : R(0), ab = (s2 != 1) ? azz : R(0);
-            pt.st(jac_point(bm, ae, aw, an, as, af, ab, q1e, q1w, q1n, q1s, q1, q1mm, apk.at(s2, nz)), tc, s2 * sy);
-        }
-        q0m = q0; q0 = q0p;
-        q1mm = q1m; q1m = q1;
-        bm = b;
     }
 }
 
@@ -491,6 +612,8 @@ __global__ __launch_bounds__(1024) void k_jacobi2_blk(const EnsArg<R> ea, int ps
     const R ae = (i != nx) ? I.axx : R(0), aw = (i != 1) ? I.axx : R(0);
     const R an = (j != ny) ? I.ayy : R(0), as = (j != 1) ? I.ayy : R(0);
     const R azz = I.azz;
+    // interior chunk: every plane the march forms, k0-1 .. k1+1, lies in 2 .. nz-1
+    const bool interior = (k0 >= 3 && k0 + KZ - 1 <= nz - 2);
     const JacAp<R> apk(ae, aw, an, as, azz, nz);
     const int olast = (nz + 1) * sy;
     auto po = [&](int s) { return min(max(s, 0) * sy, olast); };  // clamped plane offset
@@ -505,33 +628,41 @@ __global__ __launch_bounds__(1024) void k_jacobi2_blk(const EnsArg<R> ea, int ps
         pjp[e] = p.ld(tjp, o);
         bb[e] = rhs.ld(tc, o);
     }
-    // sweep 1 at plane s, sweep 2 at plane s-1, for s = k0-1 .. k1+1
-    R q1mm = R(0), q1m = R(0);
+    // sweep 1 at plane s, sweep 2 at plane s-1, for s = k0-1 .. k1+1; an
+    // interior chunk (IN) has every coefficient and divisor of its interior
+    // planes fixed, the others select them per plane
+    auto march = [&](auto in_tag) __attribute__((always_inline)) {
+        constexpr bool IN = decltype(in_tag)::value;
+        R q1mm = R(0), q1m = R(0);
 #pragma unroll
-    for (int e = 0; e < KZ + 2; ++e) {
-        const int s = k0 - 1 + e;
-        if (s > k1 + 1) break;  // uniform over the work-group
-        const R q0 = pc[e + 1];
-        const R q0w = sh_up(q0), q0e = sh_dn(q0);
-        R q1 = R(0);
-        if (in1 && s >= 1 && s <= nz) {
-            const R af = (s != nz) ? azz : R(0), ab = (s != 1) ? azz : R(0);
-            q1 = jac_point(bb[e], ae, aw, an, as, af, ab, q0e, q0w, pjp[e], pjm[e], pc[e + 2], pc[e], apk.at(s, nz));
+        for (int e = 0; e < KZ + 2; ++e) {
+            const int s = k0 - 1 + e;
+            if (!IN && s > k1 + 1) break;  // uniform over the work-group
+            const R q0 = pc[e + 1];
+            const R q0w = sh_up(q0), q0e = sh_dn(q0);
+            R q1 = R(0);
+            if (in1 && (IN || (s >= 1 && s <= nz))) {
+                const R af = (IN || s != nz) ? azz : R(0), ab = (IN || s != 1) ? azz : R(0);
+                const R num = jac_num(bb[e], ae, aw, an, as, af, ab, q0e, q0w, pjp[e], pjm[e], pc[e + 2], pc[e]);
+                q1 = IN ? apk.div_in(num) : apk.div(num, s, nz);
+            }
+            const int slot = s % 3, pslot = (s + 2) % 3;
+            ring[slot][w][lane] = q1;
+            __syncthreads();
+            const int s2 = s - 1;
+            const R q1w = sh_up(q1m), q1e = sh_dn(q1m);
+            if (st2 && (IN ? e >= 2 : (e >= 1 && s2 >= k0))) {  // plane k0-1 is the previous chunk's
+                const R q1s = ring[pslot][w - 1][lane], q1n = ring[pslot][w + 1][lane];
+                const R af = (IN || s2 != nz) ? azz : R(0), ab = (IN || s2 != 1) ? azz : R(0);
+                const R num = jac_num(bb[e - 1], ae, aw, an, as, af, ab, q1e, q1w, q1n, q1s, q1, q1mm);
+                pt.st(IN ? apk.div_in(num) : apk.div(num, s2, nz), tc, s2 * sy);
+            }
+            q1mm = q1m;
+            q1m = q1;
         }
-        const int slot = s % 3, pslot = (s + 2) % 3;
-        ring[slot][w][lane] = q1;
-        __syncthreads();
-        const int s2 = s - 1;
-        const R q1w = sh_up(q1m), q1e = sh_dn(q1m);
-        if (e >= 1 && st2 && s2 >= k0) {
-            const R q1s = ring[pslot][w - 1][lane], q1n = ring[pslot][w + 1][lane];
-            const R af = (s2 != nz) ? azz : R(0), ab = (s2 != 1) ? azz : R(0);
-            pt.st(jac_point(bb[e - 1], ae, aw, an, as, af, ab, q1e, q1w, q1n, q1s, q1, q1mm, apk.at(s2, nz)), tc,
-                  s2 * sy);
-        }
-        q1mm = q1m;
-        q1m = q1;
-    }
+    };
+    if (interior) march(std::true_type{});
+    else march(std::false_type{});
 }
 
 // ---------------------------------------------------------------------------
@@ -588,29 +719,36 @@ __device__ __forceinline__ void correct_plane(const Inst<R>& I, const WMap& m, c
     const R wc = corr(I, wsk, rk, r_km, pk, p_km, I.dzi);
     const R v_r = sh_dn(vc), v_l = sh_up(vc), w_r = sh_dn(wc), w_l = sh_up(wc);
     if (m.own) {
+        // A tangential ghost is the nearest interior value times I.slip (+1 free
+        // slip, -1 no-slip) per ghost axis it lies on, in the passes that copy
+        // it: ti on the x-ghost columns, tj on the y-ghost rows, and I.slip
+        // again on the z-ghost planes.  The wall faces are zeroed by their own
+        // pass, so only the later passes' signs reach them (u's by k only, v's
+        // by i and k, w's none).  With +1 every product is the plain copy.
+        const R ti = iin ? R(1) : I.slip, tj = jin ? R(1) : I.slip;
         // u: walls at i = 1, nx+1; plane i = 0 keeps/copies its own values
         R uval = R(0);
         bool uw = true;
-        if (i >= 2 && i <= nx) uval = uc;
-        else if (i == 0) { uw = !jin; if (uw) uval = u.ld(ts, o); }
+        if (i >= 2 && i <= nx) uval = tj * uc;
+        else if (i == 0) { uw = !jin; if (uw) uval = tj * u.ld(ts, o); }
         if (uw) u.st(uval, t, o);
         // v: walls at j = 1, ny+1; row j = 0 keeps/copies; x ghosts copy the neighbour
-        R vval = R(0);
+        R vval = ti * R(0);
         bool vw = true;
-        if (j >= 2 && j <= ny) vval = iin ? vc : (i == 0 ? v_r : v_l);
-        else if (j == 0) { vw = !iin; if (vw) vval = v.ld(ci, o); }
+        if (j >= 2 && j <= ny) vval = ti * (iin ? vc : (i == 0 ? v_r : v_l));
+        else if (j == 0) { vw = !iin; if (vw) vval = ti * v.ld(ci, o); }
         if (vw) v.st(vval, t, o);
         // w: wall at k = 1
-        const R wval = (k == 1) ? R(0) : (iin ? wc : (i == 0 ? w_r : w_l));
+        const R wval = (k == 1) ? R(0) : ti * (tj * (iin ? wc : (i == 0 ? w_r : w_l)));
         w.st(wval, t, o);
         if (k == 1) {  // ghost plane 0: u, v copy plane 1; w copies its own plane-0 interior
-            u.st((i == 0) ? u.ld(ts, sy) : uval, t, 0);
-            v.st((j == 0) ? v.ld(ci, sy) : vval, t, 0);
-            if (!(iin && jin)) w.st(w.ld(cj * sx + ci, 0), t, 0);
+            u.st(I.slip * ((i == 0) ? tj * u.ld(ts, sy) : uval), t, 0);
+            v.st(I.slip * ((j == 0) ? ti * v.ld(ci, sy) : vval), t, 0);
+            if (!(iin && jin)) w.st(ti * (tj * w.ld(cj * sx + ci, 0)), t, 0);
         }
         if (k == nz) {  // ghost plane nz+1: u, v copy plane nz; w wall
-            u.st((i == 0) ? u.ld(ts, o) : uval, t, o + sy);
-            v.st((j == 0) ? v.ld(ci, o) : vval, t, o + sy);
+            u.st(I.slip * ((i == 0) ? tj * u.ld(ts, o) : uval), t, o + sy);
+            v.st(I.slip * ((j == 0) ? ti * v.ld(ci, o) : vval), t, o + sy);
             w.st(R(0), t, o + sy);
         }
     }
@@ -1623,6 +1761,10 @@ void launch_predict(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st) {
     }
 }
 template <class R>
+void launch_predict_rho(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st) {
+    hipLaunchKernelGGL(k_predict_rho<R>, dim3(g.nblk), dim3(256), 0, st, ens_arg(e, g), fsel);
+}
+template <class R>
 void launch_rhs(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st) {
     const dim3 grid(g.nblk), blk(256);
     switch (g.kz) {  // chunks of 4 or 8 planes: operands preloaded
@@ -1655,7 +1797,10 @@ void launch_jacobi2(const Ens<R>& e, const Geo& g, int psrc, hipStream_t st) {
             default: break;
         }
     }
-    hipLaunchKernelGGL(k_jacobi2<R>, grid, blk, 0, st, ens_arg(e, g), psrc, g.kz);
+    // One plane of prefetch: a deeper ring (2 planes in FP64, 4 in FP32, tried
+    // at 200^3) costs the registers of the second work-group per CU and ran
+    // 20-35% slower.
+    hipLaunchKernelGGL((k_jacobi2<R, 1>), grid, blk, 0, st, ens_arg(e, g), psrc, g.kz);
 }
 template <class R>
 void launch_correct(const Ens<R>& e, const Geo& g, int fsel, int psel, hipStream_t st) {
@@ -1744,6 +1889,7 @@ void launch_reduce(const Ens<R>& e, const Geo& g, int fsel, int psel, int mode, 
 
 #define VOF_INSTANTIATE(R)                                                                               \
     template void launch_predict<R>(const Ens<R>&, const Geo&, int, hipStream_t);                        \
+    template void launch_predict_rho<R>(const Ens<R>&, const Geo&, int, hipStream_t);                    \
     template void launch_rhs<R>(const Ens<R>&, const Geo&, int, hipStream_t);                            \
     template void launch_jacobi<R>(const Ens<R>&, const Geo&, int, hipStream_t);                         \
     template void launch_jacobi2<R>(const Ens<R>&, const Geo&, int, hipStream_t);                        \

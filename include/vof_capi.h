@@ -61,13 +61,19 @@ typedef enum vof_field {
  * face gains dt * sigma * kappa_f * dF/dn / rho_f, kappa = -div(grad F/|grad F|)
  * from vertex normals.  sigma = 0 gives the parity step. */
 #define VOF_FLAG_CSF           0x2u
-/* Reserved, not built (vof_create returns VOF_ERR_UNSUPPORTED): a density-
- * weighted viscous term in the predictor and no-slip tangential walls.  The
- * reference's predictor takes rho but never reads it (voftrial1.cu:233, :290,
- * :347) and its tangential wall ghosts copy the interior (free slip,
- * :959-1051); density enters the step through the Poisson right-hand side and
- * the correction, and the surface force above through its face density. */
+/* Density-weighted viscous term in the predictor (the reference's predictor
+ * takes rho but never reads it, voftrial1.cu:233, :290, :347, and scales the
+ * Laplacian by the nu of one cell, :277-280): every interior face gains
+ * dt * div(mu grad u)/rho_f in flux form, mu = rho nu per cell, edge
+ * viscosities the mean of the four cells around the edge, rho_f the mean of
+ * the face's two cells.  It conserves momentum across density jumps (the edge
+ * fluxes telescope) and keeps a two-layer shear steady; with one fluid it is
+ * the reference's term up to rounding.  Not supported in reverse mode. */
 #define VOF_FLAG_RHO_PREDICTOR 0x4u
+/* No-slip walls: every tangential velocity ghost holds the negated interior
+ * value (the reference copies it: free slip, :959-1051), so the velocity
+ * vanishes on the wall halfway between ghost and interior face; normal wall
+ * faces stay 0 as in the reference.  Not supported in reverse mode. */
 #define VOF_FLAG_NO_SLIP       0x8u
 #define VOF_FLAG_WATCHDOG      0x10u/* after every vof_step call: F in [0,1] (up to clamp01's half-ulp
                                          leaks) and u,v,w,p,F finite, else VOF_ERR_NUMERICS */
@@ -303,7 +309,8 @@ int vof_batch_size(const vof_batch* b);
  * keeps it, a copy of the nearest interior cell: the adjoint of F is reported
  * on interior cells (ghost entries 0), that of p likewise (ghost pressures
  * only meet zero coefficients).
- * Supported: flags 0, VOF_FLAG_CSF, VOF_FLAG_WATCHDOG. */
+ * Supported: flags 0, VOF_FLAG_CSF, VOF_FLAG_WATCHDOG; any other flag returns
+ * VOF_ERR_UNSUPPORTED. */
 typedef struct vof_adj_scalars {
     double d_rho_l, d_rho_g, d_nu_l, d_nu_g; /* dJ/d parameter */
     double d_gx, d_gy, d_gz;

@@ -156,10 +156,47 @@ struct Orc : OrcBase {
         }
     }
 
+    // ---- Density-weighted viscous term (VOF_FLAG_RHO_PREDICTOR; extension,
+    //      no reference counterpart: the reference's predictor takes rho and
+    //      never reads it, :233, :290, :347, and uses nu of one cell,
+    //      :277-280).  Face phi at index c between cell a = c - off[d] (low)
+    //      and cell b = c along its own axis d:
+    //        visc = ((T_x + T_y) + T_z) / (0.5 (rho_a + rho_b)),
+    //        T_t  = (m+ (phi(c+e_t) - phi) - m- (phi - phi(c-e_t))) * h_t^-1 * h_t^-1,
+    //      with mu = rho nu per cell; along d, m+ = mu_b and m- = mu_a; across,
+    //      m+- = 0.25 ((mu_a + mu_b) + (mu_a(+-e_t) + mu_b(+-e_t))) ordered
+    //      low edge first: m- = 0.25 ((mu_a(-e_t) + mu_b(-e_t)) + (mu_a + mu_b)).
+    //      This is div(mu grad phi) / rho in flux form: the fluxes through the
+    //      edges between faces telescope, so sum(rho_f visc) over faces only
+    //      sees the walls.  The association is the contract the HIP kernel repeats.
+    inline R mu_at(size_t q) const { return a[O_RHO][q] * a[O_NU][q]; }
+    R visc_rho(const R* phi, size_t c, int d) const {
+        const size_t off[3] = {1, size_t(NX), size_t(NX) * size_t(NY)};
+        const R hi[3] = {dxi, dyi, dzi};
+        const size_t ca = c - off[d], cb = c;
+        const R ma = mu_at(ca), mb = mu_at(cb);
+        R T[3];
+        for (int t = 0; t < 3; ++t) {
+            const size_t o = off[t];
+            R mp, mm;
+            if (t == d) {
+                mp = mb;
+                mm = ma;
+            } else {
+                mp = R(0.25) * ((ma + mb) + (mu_at(ca + o) + mu_at(cb + o)));
+                mm = R(0.25) * ((mu_at(ca - o) + mu_at(cb - o)) + (ma + mb));
+            }
+            T[t] = (mp * (phi[c + o] - phi[c]) - mm * (phi[c] - phi[c - o])) * hi[t] * hi[t];
+        }
+        const R rf = R(0.5) * (a[O_RHO][ca] + a[O_RHO][cb]);
+        return ((T[0] + T[1]) + T[2]) / rf;
+    }
+
     // ---- advect_u_kernel (:230-285): faces i = imin+1..imax.
     void advect_u() {
         const R *u = fld(O_U), *v = fld(O_V), *w = fld(O_W), *nu = fld(O_NU);
         R* us = fld(O_US);
+        const bool rho_visc = (flags & VOF_FLAG_RHO_PREDICTOR) != 0;
 #pragma omp parallel for collapse(2)
         for (int k = kmin; k <= kmax; ++k)
             for (int j = jmin; j <= jmax; ++j)
@@ -177,9 +214,10 @@ struct Orc : OrcBase {
                     const R ddy = (vh > R(0)) ? (uc - uym) * dyi : (uyp - uc) * dyi;
                     const R ddz = (wh > R(0)) ? (uc - uzm) * dzi : (uzp - uc) * dzi;
                     const R n0 = nu[c];
-                    const R visc = n0 * (uxm - R(2) * uc + uxp) * dxi * dxi +
-                                   n0 * (uym - R(2) * uc + uyp) * dyi * dyi +
-                                   n0 * (uzm - R(2) * uc + uzp) * dzi * dzi;
+                    const R visc = rho_visc ? visc_rho(u, c, 0)
+                                            : n0 * (uxm - R(2) * uc + uxp) * dxi * dxi +
+                                                  n0 * (uym - R(2) * uc + uyp) * dyi * dyi +
+                                                  n0 * (uzm - R(2) * uc + uzp) * dzi * dzi;
                     const R adv = -uc * ddx - vh * ddy - wh * ddz;
                     us[c] = uc + dt * (visc + adv + gx);
                 }
@@ -189,6 +227,7 @@ struct Orc : OrcBase {
     void advect_v() {
         const R *u = fld(O_U), *v = fld(O_V), *w = fld(O_W), *nu = fld(O_NU);
         R* vs = fld(O_VS);
+        const bool rho_visc = (flags & VOF_FLAG_RHO_PREDICTOR) != 0;
 #pragma omp parallel for collapse(2)
         for (int k = kmin; k <= kmax; ++k)
             for (int j = jmin + 1; j <= jmax; ++j)
@@ -206,9 +245,10 @@ struct Orc : OrcBase {
                     const R ddy = (vc > R(0)) ? (vc - vym) * dyi : (vyp - vc) * dyi;
                     const R ddz = (wh > R(0)) ? (vc - vzm) * dzi : (vzp - vc) * dzi;
                     const R n0 = nu[c];
-                    const R visc = n0 * (vxm - R(2) * vc + vxp) * dxi * dxi +
-                                   n0 * (vym - R(2) * vc + vyp) * dyi * dyi +
-                                   n0 * (vzm - R(2) * vc + vzp) * dzi * dzi;
+                    const R visc = rho_visc ? visc_rho(v, c, 1)
+                                            : n0 * (vxm - R(2) * vc + vxp) * dxi * dxi +
+                                                  n0 * (vym - R(2) * vc + vyp) * dyi * dyi +
+                                                  n0 * (vzm - R(2) * vc + vzp) * dzi * dzi;
                     const R adv = -uh * ddx - vc * ddy - wh * ddz;
                     vs[c] = vc + dt * (visc + adv + gy);
                 }
@@ -218,6 +258,7 @@ struct Orc : OrcBase {
     void advect_w() {
         const R *u = fld(O_U), *v = fld(O_V), *w = fld(O_W), *nu = fld(O_NU);
         R* ws = fld(O_WS);
+        const bool rho_visc = (flags & VOF_FLAG_RHO_PREDICTOR) != 0;
 #pragma omp parallel for collapse(2)
         for (int k = kmin + 1; k <= kmax; ++k)
             for (int j = jmin; j <= jmax; ++j)
@@ -235,9 +276,10 @@ struct Orc : OrcBase {
                     const R ddy = (vh > R(0)) ? (wc - wym) * dyi : (wyp - wc) * dyi;
                     const R ddz = (wc > R(0)) ? (wc - wzm) * dzi : (wzp - wc) * dzi;
                     const R n0 = nu[c];
-                    const R visc = n0 * (wxm - R(2) * wc + wxp) * dxi * dxi +
-                                   n0 * (wym - R(2) * wc + wyp) * dyi * dyi +
-                                   n0 * (wzm - R(2) * wc + wzp) * dzi * dzi;
+                    const R visc = rho_visc ? visc_rho(w, c, 2)
+                                            : n0 * (wxm - R(2) * wc + wxp) * dxi * dxi +
+                                                  n0 * (wym - R(2) * wc + wyp) * dyi * dyi +
+                                                  n0 * (wzm - R(2) * wc + wzp) * dzi * dzi;
                     const R adv = -uh * ddx - vh * ddy - wc * ddz;
                     ws[c] = wc + dt * (visc + adv + gz);
                 }
@@ -328,17 +370,22 @@ struct Orc : OrcBase {
     }
 
     // ---- apply_bc_velocity (:1107-1120): set_bc_vel_j, then _i, then _k
-    //      (:959-1051), each over full (n+2)^2 planes.
+    //      (:959-1051), each over full (n+2)^2 planes.  The reference's
+    //      tangential ghosts copy the interior (free slip, t = 1).  With
+    //      VOF_FLAG_NO_SLIP (extension) they are the negated interior value
+    //      (t = -1): the wall lies halfway between the ghost and the first
+    //      interior face, so the linear interpolant vanishes on it.
     void bc_vel() {
         R *u = fld(O_U), *v = fld(O_V), *w = fld(O_W);
+        const R t = (flags & VOF_FLAG_NO_SLIP) ? R(-1) : R(1);
         // j pass (:959-987)
 #pragma omp parallel for
         for (int k = 0; k < NZ; ++k)
             for (int i = 0; i < NX; ++i) {
                 const size_t gb = id(i, jmin - 1, k), ib = id(i, jmin, k);
                 const size_t gt = id(i, jmax + 1, k), it = id(i, jmax, k);
-                u[gb] = u[ib]; v[ib] = R(0); w[gb] = w[ib];
-                u[gt] = u[it]; v[gt] = R(0); w[gt] = w[it];
+                u[gb] = t * u[ib]; v[ib] = R(0); w[gb] = t * w[ib];
+                u[gt] = t * u[it]; v[gt] = R(0); w[gt] = t * w[it];
             }
         // i pass (:989-1019)
 #pragma omp parallel for
@@ -346,8 +393,8 @@ struct Orc : OrcBase {
             for (int j = 0; j < NY; ++j) {
                 const size_t gl = id(imin - 1, j, k), il = id(imin, j, k);
                 const size_t gr = id(imax + 1, j, k), ir = id(imax, j, k);
-                u[il] = R(0); v[gl] = v[il]; w[gl] = w[il];
-                u[gr] = R(0); v[gr] = v[ir]; w[gr] = w[ir];
+                u[il] = R(0); v[gl] = t * v[il]; w[gl] = t * w[il];
+                u[gr] = R(0); v[gr] = t * v[ir]; w[gr] = t * w[ir];
             }
         // k pass (:1021-1051)
 #pragma omp parallel for
@@ -355,8 +402,8 @@ struct Orc : OrcBase {
             for (int i = 0; i < NX; ++i) {
                 const size_t gb = id(i, j, kmin - 1), ib = id(i, j, kmin);
                 const size_t gf = id(i, j, kmax + 1), kf = id(i, j, kmax);
-                u[gb] = u[ib]; v[gb] = v[ib]; w[ib] = R(0);
-                u[gf] = u[kf]; v[gf] = v[kf]; w[gf] = R(0);
+                u[gb] = t * u[ib]; v[gb] = t * v[ib]; w[ib] = R(0);
+                u[gf] = t * u[kf]; v[gf] = t * v[kf]; w[gf] = R(0);
             }
     }
 

@@ -47,7 +47,7 @@ MUTANTS = [
      "const R ddy = (vc > R(0)) ? (vc - vym) * dyi : (vyp - vc) * dyi;",
      "const R ddy = (vc < R(0)) ? (vc - vym) * dyi : (vyp - vc) * dyi;"),
     ("advect_v: z viscous term dropped",
-     "n0 * (vym - R(2) * vc + vyp) * dyi * dyi +\n                                   n0 * (vzm - R(2) * vc + vzp) * dzi * dzi;",
+     "n0 * (vym - R(2) * vc + vyp) * dyi * dyi +\n                                                  n0 * (vzm - R(2) * vc + vzp) * dzi * dzi;",
      "n0 * (vym - R(2) * vc + vyp) * dyi * dyi;"),
     ("advect_v: gravity component", "vs[c] = vc + dt * (visc + adv + gy);", "vs[c] = vc + dt * (visc + adv + gx);"),
     ("advect_v: x advection by the wrong velocity",
@@ -86,10 +86,26 @@ MUTANTS = [
      "                    w[c] = ws[c] - dt / r * (p[c] - p[m]) * dzi;"),
     ("update_w: gradient sign", "w[c] = ws[c] - dt / r * (p[c] - p[m]) * dzi;", "w[c] = ws[c] + dt / r * (p[c] - p[m]) * dzi;"),
     # velocity BC (:959-1051, :1107-1120)
-    ("bc_vel: v wall not zeroed at the top", "u[gt] = u[it]; v[gt] = R(0); w[gt] = w[it];",
-     "u[gt] = u[it]; v[gt] = v[it]; w[gt] = w[it];"),
-    ("bc_vel: tangential ghost negated (no-slip) on the k pass",
-     "u[gb] = u[ib]; v[gb] = v[ib]; w[ib] = R(0);", "u[gb] = -u[ib]; v[gb] = v[ib]; w[ib] = R(0);"),
+    ("bc_vel: v wall not zeroed at the top", "u[gt] = t * u[it]; v[gt] = R(0); w[gt] = t * w[it];",
+     "u[gt] = t * u[it]; v[gt] = t * v[it]; w[gt] = t * w[it];"),
+    ("bc_vel: u ghost negated on the k pass in free slip",
+     "u[gb] = t * u[ib]; v[gb] = t * v[ib]; w[ib] = R(0);", "u[gb] = -t * u[ib]; v[gb] = t * v[ib]; w[ib] = R(0);"),
+    ("bc_vel: no-slip sign dropped on the j pass",
+     "u[gt] = t * u[it]; v[gt] = R(0); w[gt] = t * w[it];", "u[gt] = u[it]; v[gt] = R(0); w[gt] = t * w[it];"),
+    ("bc_vel: no-slip ghosts zeroed instead of negated",
+     "const R t = (flags & VOF_FLAG_NO_SLIP) ? R(-1) : R(1);", "const R t = (flags & VOF_FLAG_NO_SLIP) ? R(0) : R(1);"),
+    # density-weighted viscous term (extension)
+    ("rho visc: face density of one cell", "const R rf = R(0.5) * (a[O_RHO][ca] + a[O_RHO][cb]);",
+     "const R rf = a[O_RHO][cb];"),
+    ("rho visc: upper edge averaged on the lower cells",
+     "mp = R(0.25) * ((ma + mb) + (mu_at(ca + o) + mu_at(cb + o)));",
+     "mp = R(0.25) * ((ma + mb) + (mu_at(ca - o) + mu_at(cb - o)));"),
+    ("rho visc: lower edge viscosity of the face's cells only",
+     "mm = R(0.25) * ((mu_at(ca - o) + mu_at(cb - o)) + (ma + mb));", "mm = R(0.5) * (ma + mb);"),
+    ("rho visc: mu is nu (density dropped)", "inline R mu_at(size_t q) const { return a[O_RHO][q] * a[O_NU][q]; }",
+     "inline R mu_at(size_t q) const { return a[O_NU][q]; }"),
+    ("rho visc: own-axis viscosities swapped",
+     "                mp = mb;\n                mm = ma;", "                mp = ma;\n                mm = mb;"),
     # FCT (:509-905)
     ("fct: high-order flux central instead of downwind",
      "static inline R flux_high(R vel, R dt_, R fm, R fp) { return (vel <= R(0)) ? vel * dt_ * fm : vel * dt_ * fp; }",

@@ -476,3 +476,226 @@ def test_diagnostics_sums_min_max_and_residual():
     assert math.isclose(d["residual_l2"], float(np.sqrt(np.sum(r * r))), rel_tol=1e-10)
     assert math.isclose(d["rhs_l2"], float(np.sqrt(np.sum(b * b))), rel_tol=1e-12)
     o.close()
+
+
+# ----------------------------------------------------------------- no-slip walls (extension)
+FLAG_RHO_PREDICTOR, FLAG_NO_SLIP = 0x4, 0x8
+
+
+def test_no_slip_velocity_bc_closed_form():
+    """VOF_FLAG_NO_SLIP: the j, i, k passes of apply_bc_velocity (:959-1051)
+    with every tangential copy negated.  In closed form (s = -1 on a ghost
+    row/plane/column of that axis, c* the index clamped to the interior):
+      u(i,j,k) = s_k * 0 on the wall faces i = 1, nx+1 (zeroed by the i pass,
+                 after the j pass), else s_j s_k u(i, cj, ck);
+      v(i,j,k) = s_i s_k * 0 on j = 1, ny+1, else s_i s_k v(ci, j, ck);
+      w(i,j,k) = +0 on k = 1, nz+1 (the k pass is last), else s_i s_j w(ci, cj, k).
+    Checked bit for bit, signs of zeros included."""
+    nx, ny, nz = 5, 4, 6
+    p = default_params(nx=nx, ny=ny, nz=nz, precision=8, flags=FLAG_NO_SLIP)
+    o = Oracle(p)
+    rng = np.random.default_rng(40)
+    pre = {k: rng.normal(size=o.shape) for k in ("U", "V", "W")}
+    for k, a in pre.items():
+        o.set(k, a)
+    o.stage("bc_vel")
+    sgn = lambda n: np.where((np.arange(n + 2) >= 1) & (np.arange(n + 2) <= n), 1.0, -1.0)
+    si, sj, sk = sgn(nx), sgn(ny), sgn(nz)
+    ci, cj, ck = (np.clip(np.arange(n + 2), 1, n) for n in (nx, ny, nz))
+    S = lambda a, b: a[:, None, None] if b == "k" else (a[None, :, None] if b == "j" else a[None, None, :])
+    u = pre["U"][ck][:, cj] * S(sj, "j") * S(sk, "k")
+    u[:, :, [1, nx + 1]] = (np.zeros((nz + 2, ny + 2, 2)) * S(sk, "k"))
+    v = pre["V"][ck][:, :, ci] * S(si, "i") * S(sk, "k")
+    v[:, [1, ny + 1], :] = np.zeros((nz + 2, 2, nx + 2)) * S(si, "i") * S(sk, "k")
+    w = pre["W"][:, cj][:, :, ci] * S(si, "i") * S(sj, "j")
+    w[[1, nz + 1], :, :] = 0.0
+    for name, want in (("U", u), ("V", v), ("W", w)):
+        got = o.get(name)
+        assert np.array_equal(got, want), name
+        assert np.array_equal(np.signbit(got), np.signbit(want)), name + " signed zeros"
+    o.close()
+
+
+@pytest.mark.parametrize("comp", [0, 1, 2])
+def test_no_slip_shear_mode_decays_at_the_discrete_eigenvalue(comp):
+    """phi = sin(pi y_a / L_a) sin(pi y_b / L_b) across the two axes a, b
+    transverse to the face component phi (cell centres y = (m - 1/2) h), uniform
+    along its own axis: with the no-slip ghosts (phi(0) = -phi(1),
+    phi(n+1) = -phi(n)) this is an eigenvector of the discrete Laplacian with
+    eigenvalue sum_t (2 cos(pi h_t / L_t) - 2) / h_t^2, so one predictor step
+    (F = 0, no gravity, the other components zero) gives
+    phi* = phi (1 + dt nu_g lambda) on faces away from the normal walls.  The
+    reference's free-slip copies (phi(0) = phi(1)) give another value next to
+    every wall."""
+    n = (8, 6, 7)
+    L = (0.08, 0.06, 0.1)
+    p = default_params(nx=n[0], ny=n[1], nz=n[2], Lx=L[0], Ly=L[1], Lz=L[2], g=(0, 0, 0), precision=8,
+                       flags=FLAG_NO_SLIP, dt=1e-3)
+    o = Oracle(p)
+    shape = o.shape
+    axes = [t for t in range(3) if t != comp]
+    idx = np.meshgrid(np.arange(n[2] + 2), np.arange(n[1] + 2), np.arange(n[0] + 2), indexing="ij")
+    m_of = {0: idx[2], 1: idx[1], 2: idx[0]}
+    h = [L[t] / n[t] for t in range(3)]
+    phi = np.ones(shape)
+    for t in axes:
+        phi = phi * np.sin(np.pi * (m_of[t] - 0.5) * h[t] / L[t])
+    names = ("U", "V", "W")
+    st = {k: np.zeros(shape) for k in names}
+    st[names[comp]] = phi
+    o.set("F", np.zeros(shape))
+    for k in names:
+        o.set(k, st[k])
+    o.stage("bc_vel")  # normal walls to 0, no-slip ghosts
+    for s_ in ("nu_rho", "advect_u", "advect_v", "advect_w"):
+        o.stage(s_)
+    got = o.get(("USTAR", "VSTAR", "WSTAR")[comp])
+    lam = sum((2 * np.cos(np.pi * h[t] / L[t]) - 2) / h[t] ** 2 for t in axes)
+    want = phi * (1 + p.dt * p.nu_g * lam)
+    sel = [slice(1, -1)] * 3
+    sel[2 - comp] = slice(3, n[comp])  # own-axis faces 3 .. n-1 (the walls' neighbours see u = 0)
+    sel = tuple(sel)
+    np.testing.assert_allclose(got[sel], want[sel], rtol=1e-13, atol=1e-16)
+    o.close()
+
+
+# ----------------------------------------------------------------- density-weighted viscosity (extension)
+def _predict_flags(p, st):
+    o = Oracle(p)
+    for k in ("F", "U", "V", "W"):
+        o.set(k, st[k])
+    for s_ in ("nu_rho", "advect_u", "advect_v", "advect_w"):
+        o.stage(s_)
+    out = {k: o.get(k) for k in ("USTAR", "VSTAR", "WSTAR", "RHO")}
+    o.close()
+    return out
+
+
+def test_rho_predictor_reduces_to_the_reference_form_for_one_fluid():
+    """With F uniform, mu = rho nu is one constant, every edge average of it
+    is exact (sums of equal values times 1/4) and rho_f = rho, so the flux form
+    is nu times the reference's Laplacian up to the rounding of the different
+    association: the two predictors agree to 1e-13 (random velocities)."""
+    n = 7
+    st = random_state(n, 41)
+    st["F"] = np.full((n + 2,) * 3, 0.375)
+    # viscosities large enough that the viscous increment is comparable to the
+    # velocity itself (nu dt / h^2 ~ 0.25)
+    kw = dict(n=n, precision=8, dt=1e-3, g=(1.0, -2.0, 0.5), nu_l=0.05, nu_g=0.04)
+    a = _predict_flags(default_params(**kw), st)
+    b = _predict_flags(default_params(flags=FLAG_RHO_PREDICTOR, **kw), st)
+    for k, v in zip(("USTAR", "VSTAR", "WSTAR"), ("U", "V", "W")):
+        inc_a, inc_b = a[k] - st[v], b[k] - st[v]
+        m = a[k] != 0  # faces the predictor writes
+        assert close(inc_b[m], inc_a[m], rel=1e-12), k
+        assert np.max(np.abs(inc_a[m])) > 0.1 * np.max(np.abs(st[v]))
+
+
+@pytest.mark.parametrize("comp", [0, 1, 2])
+def test_rho_predictor_conserves_momentum_across_a_density_field(comp):
+    """phi varying only across its own axis (so advection vanishes: its own
+    difference is zero and the other velocities are zero), random F in (0,1),
+    free-slip ghosts (zero-gradient copies): the viscous fluxes through the
+    edges between faces telescope, so in every slice normal to the face's axis
+    sum(rho_f (phi* - phi)/dt) = 0 up to round-off, whatever the density
+    field.  A face density of one cell, an edge viscosity averaged over the
+    wrong cells or an unbalanced m+/m- breaks the telescoping."""
+    n = (6, 5, 7)
+    p = default_params(nx=n[0], ny=n[1], nz=n[2], precision=8, dt=1.0, g=(0, 0, 0), flags=FLAG_RHO_PREDICTOR)
+    rng = np.random.default_rng(42 + comp)
+    shape = (n[2] + 2, n[1] + 2, n[0] + 2)
+    F = rng.uniform(0.05, 0.95, shape)
+    own = 2 - comp  # numpy axis of the component's own direction
+    prof = rng.normal(size=[s if ax != own else 1 for ax, s in enumerate(shape)])
+    prof = np.pad(prof[tuple(slice(1, -1) if ax != own else slice(None) for ax in range(3))],
+                  [(1, 1) if ax != own else (0, 0) for ax in range(3)], mode="edge")  # zero-gradient ghosts
+    phi = np.broadcast_to(prof, shape).copy()
+    names = ("U", "V", "W")
+    st = {"F": F, "U": np.zeros(shape), "V": np.zeros(shape), "W": np.zeros(shape)}
+    st[names[comp]] = phi
+    out = _predict_flags(p, st)
+    star = out[("USTAR", "VSTAR", "WSTAR")[comp]]
+    rho = out["RHO"]
+    lo = [slice(1, -1)] * 3
+    lo[own] = slice(1, -2)
+    hi = [slice(1, -1)] * 3
+    hi[own] = slice(2, -1)
+    rf = 0.5 * (rho[tuple(lo)] + rho[tuple(hi)])  # faces 2..n along the own axis
+    acc = rf * (star[tuple(hi)] - phi[tuple(hi)]) / p.dt
+    other = tuple(ax for ax in range(3) if ax != own)
+    tot = acc.sum(axis=other)
+    scale = np.abs(acc).sum(axis=other)
+    assert np.all(np.abs(tot) <= 1e-12 * scale), (tot, scale)
+    assert np.all(scale > 0)
+
+
+def test_rho_predictor_two_layer_shear_is_steady():
+    """Two fluids in layers (liquid below the cell boundary z = 3 h, gas above),
+    u = u(z) with u(k+1) - u(k) = tau / mu_edge(k + 1/2), mu_edge the edge
+    average of the flux form (here (mu(k) + mu(k+1))/2): the shear stress is
+    continuous across the density jump, so the viscous term vanishes and
+    u* = u on every face whose two z-edges are interior.  The reference's
+    single-cell nu (flag off) makes the interface faces accelerate."""
+    n = 8
+    m = 3
+    p_on = default_params(n=n, precision=8, dt=1e-3, g=(0, 0, 0), flags=FLAG_RHO_PREDICTOR)
+    shape = (n + 2,) * 3
+    F = np.zeros(shape)
+    F[: m + 1] = 1.0  # planes 0..m liquid (plane 0 is the ghost)
+    rho = 50.0 * (1 - F[:, 0, 0]) + 1000.0 * F[:, 0, 0]
+    nu = 1.5e-5 * (1 - F[:, 0, 0]) + 1e-6 * F[:, 0, 0]
+    mu = rho * nu
+    tau = 2e-4
+    uz = np.zeros(n + 2)
+    for k in range(1, n + 1):
+        uz[k + 1 if k + 1 <= n + 1 else k] = uz[k] + tau / (0.5 * (mu[k] + mu[k + 1])) if k + 1 <= n + 1 else uz[k]
+    uz[0] = uz[1]
+    U = np.broadcast_to(uz[:, None, None], shape).copy()
+    st = {"F": F, "U": U, "V": np.zeros(shape), "W": np.zeros(shape)}
+    on = _predict_flags(p_on, st)["USTAR"]
+    off = _predict_flags(default_params(n=n, precision=8, dt=1e-3, g=(0, 0, 0)), st)["USTAR"]
+    sel = (slice(2, n), slice(1, -1), slice(2, n + 1))  # faces k = 2..n-1, all j, i = 2..nx
+    np.testing.assert_allclose(on[sel], U[sel], rtol=1e-12, atol=0)
+    assert np.max(np.abs(off[sel] - U[sel])) > 1e-6 * np.max(np.abs(U))
+
+
+@pytest.mark.parametrize("comp", [0, 1, 2])
+def test_rho_predictor_own_axis_stress_uses_each_cells_viscosity(comp):
+    """Along a face component's own axis the stress on the cell between faces
+    m and m+1 is carried by that cell's viscosity.  phi = 0 on every even face
+    and distinct values A, B on the odd faces around it, F (so mu and rho)
+    distinct per cell along the axis, everything uniform across: on an even
+    face advection vanishes (phi = 0 there, the other components zero), so
+    phi* = dt (mu_m B + mu_(m-1) A) / h^2 / (0.5 (rho_(m-1) + rho_m)).
+    A swap of the two cells' viscosities, or one cell's density for the face,
+    changes the value."""
+    n = (8, 9, 10)
+    L = (0.08, 0.09, 0.12)
+    p = default_params(nx=n[0], ny=n[1], nz=n[2], Lx=L[0], Ly=L[1], Lz=L[2], precision=8, dt=1e-3, g=(0, 0, 0),
+                       flags=FLAG_RHO_PREDICTOR)
+    own = 2 - comp
+    N = n[comp]
+    shape = (n[2] + 2, n[1] + 2, n[0] + 2)
+    rng = np.random.default_rng(43 + comp)
+    line_F = rng.uniform(0.05, 0.95, N + 2)
+    line_phi = np.zeros(N + 2)
+    line_phi[1::2] = rng.uniform(0.5, 2.0, len(line_phi[1::2])) * rng.choice([-1, 1], len(line_phi[1::2]))
+    bshape = [1, 1, 1]
+    bshape[own] = -1
+    F = np.broadcast_to(line_F.reshape(bshape), shape).copy()
+    phi = np.broadcast_to(line_phi.reshape(bshape), shape).copy()
+    names = ("U", "V", "W")
+    st = {"F": F, "U": np.zeros(shape), "V": np.zeros(shape), "W": np.zeros(shape)}
+    st[names[comp]] = phi
+    out = _predict_flags(p, st)
+    star = out[("USTAR", "VSTAR", "WSTAR")[comp]]
+    rho = 50.0 * (1 - line_F) + 1000.0 * line_F
+    mu = rho * (1e-6 * line_F + 1.5e-5 * (1 - line_F))
+    hi = N / L[comp]
+    pick = [2, 2, 2]
+    for m in range(2, N + 1, 2):  # even faces
+        A, B = line_phi[m - 1], line_phi[m + 1]
+        want = p.dt * ((mu[m] * B + mu[m - 1] * A) * hi * hi / (0.5 * (rho[m - 1] + rho[m])))
+        pick[own] = m
+        got = star[tuple(pick)]
+        assert got == pytest.approx(want, rel=1e-12, abs=0), (m, got, want)
