@@ -16,6 +16,7 @@ same workload.
 from __future__ import annotations
 
 import argparse
+import subprocess
 import json
 import os
 import sys
@@ -344,56 +345,98 @@ def transfer_leg(s, cells, ms_step):
 
 
 def _host_cpu():
-    """CPU model and logical CPU count of this host (from /proc/cpuinfo)."""
-    model, count = "unknown", os.cpu_count() or 0
+    """This host's CPU as lscpu reports it: model, sockets, cores per socket,
+    threads per core, logical CPUs; plus the CPUs this process may run on."""
+    info = {"model": "unknown", "sockets": None, "cores_per_socket": None, "threads_per_core": None,
+            "logical_cpus": os.cpu_count() or 0}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "logical_cpus"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys and v.strip():
+                v = v.strip()
+                info[keys[k.strip()]] = int(v) if v.isdigit() else v
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except OSError:
+        info["affinity_cpus"] = None
+    return info
+
+
+def _cpu_has_avx512():
     try:
         with open("/proc/cpuinfo") as fh:
             for line in fh:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
+                if line.startswith("flags"):
+                    f = set(line.split(":", 1)[1].split())
+                    return {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"} <= f
     except OSError:
         pass
-    return model, count
+    return False
+
+
+_CPU_TIMING = r"""
+import json, os, sys, time
+sys.path.insert(0, os.path.join(sys.argv[1], "oracle"))
+sys.path.insert(0, os.path.join(sys.argv[1], "cuda-lbm-ibm-fvm_amd"))
+import oracle, vof
+n, prec, budget = int(sys.argv[2]), int(sys.argv[3]), float(sys.argv[4])
+o = oracle.Oracle(vof.default_params(n=n, precision=prec, g=(0.0, 0.0, -9.81)))
+o.init_box(0.04, 0.04, 0.06)
+t0 = time.perf_counter()
+o.step(1, 1)
+per = max(time.perf_counter() - t0, 1e-6)
+k = max(1, min(1000, int(budget / 3 / per)))
+rates, istep = [], 2
+for _ in range(3):
+    t0 = time.perf_counter()
+    o.step(istep, k)
+    rates.append(n ** 3 * k / (time.perf_counter() - t0))
+    istep += k
+print(json.dumps({"rates": rates, "steps": k, "threads": oracle.threads()}))
+"""
 
 
 def cpu_baseline(n, prec, budget_s):
-    """Time the oracle (the reference's kernel bodies as OpenMP loops, one
-    IEEE rounding per source operation) on the same workload: one warm-up
-    step, then three timed runs of as many steps as fit a third of the budget
-    each; the median run is reported."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
-    import vof
-
-    o = oracle.Oracle(vof.default_params(n=n, precision=prec, g=(0.0, 0.0, -9.81)))
-    o.init_box(0.04, 0.04, 0.06)
-    t0 = time.perf_counter()
-    o.step(1, 1)
-    per = max(time.perf_counter() - t0, 1e-6)
-    k = max(1, min(1000, int(budget_s / 3 / per)))
-    rates = []
-    istep = 2
-    for _ in range(3):
-        t0 = time.perf_counter()
-        o.step(istep, k)
-        rates.append(n ** 3 * k / (time.perf_counter() - t0))
-        istep += k
-    cores = oracle.threads()
-    o.close()
-    rate = sorted(rates)[1]
-    model, ncpu = _host_cpu()
+    """Time the oracle (the reference's kernel bodies as OpenMP loops) on the
+    same workload, on this host's cores: one warm-up step, then three timed
+    runs of as many steps as fit a third of the budget each; the median run is
+    reported.  The timing build is oracle/liboracle_fast.so (-O3, x86-64-v4,
+    contraction allowed; the parity build keeps one rounding per source
+    operation) when the CPU has AVX-512, else the parity build.  It runs in a
+    child process whose OpenMP threads are bound one per place with
+    OMP_PROC_BIND=close, OMP_PLACES=cores; OMP_NUM_THREADS is the host's."""
+    fast = os.path.join(ROOT, "oracle", "liboracle_fast.so")
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores")
+    build = "parity build (-O3 -march=x86-64-v3 -ffp-contract=off)"
+    if os.path.exists(fast) and _cpu_has_avx512():
+        env["VOF_ORACLE_LIB"] = fast
+        build = "timing build (-O3 -march=x86-64-v4, FMA contraction allowed)"
+    r = subprocess.run([sys.executable, "-c", _CPU_TIMING, ROOT, str(n), str(prec), str(budget_s)], env=env,
+                       capture_output=True, text=True, timeout=int(10 * budget_s) + 120)
+    if r.returncode != 0:
+        raise RuntimeError("cpu baseline child failed: " + r.stderr[-2000:])
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    rate = sorted(res["rates"])[1]
+    host = _host_cpu()
     esz = 2 if prec == 8 else 1
     return {
         "value": rate,
         "unit": "cell-updates/s",
-        "cores": cores,
+        "cores": res["threads"],
         "kind": "port",
         "gbs_algorithmic": rate * 236 * esz / 1e9,
-        "host": f"{model}, {ncpu} logical CPUs visible, OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', 'unset')}",
-        "sample": f"median of 3 runs of {k} steps (after 1 warm-up) of the same {n}^3 "
+        "host": host,
+        "omp": {"OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS", "unset"), "OMP_PROC_BIND": "close",
+                "OMP_PLACES": "cores"},
+        "build": build,
+        "sample": f"median of 3 runs of {res['steps']} steps (after 1 warm-up) of the same {n}^3 "
                   f"{'FP64' if prec == 8 else 'FP32'} dam break, OpenMP oracle restating voftrial1.cu's kernels "
-                  f"on {cores} threads",
+                  f"on {res['threads']} threads",
     }
 
 

@@ -2,8 +2,10 @@
 
 * ``lib/libvof_hip.so`` — gfx950 HIP kernels + C ABI (hipcc).
 * ``lib/3dvof``         — the drop-in CLI, linked against the library.
-* ``oracle/liboracle.so`` — the CPU oracle used by tests and the CPU baseline
-  (test infrastructure, built here because ``build()`` builds every checker).
+* ``oracle/liboracle.so`` — the CPU oracle used by tests (test infrastructure,
+  built here because ``build()`` builds every checker).
+* ``oracle/liboracle_fast.so`` — the same source built for speed, timed as
+  bench.py's CPU baseline only.
 
 Run ``python cuda-lbm-ibm-fvm_amd/build_native.py`` or call :func:`build_all`.
 """
@@ -63,8 +65,22 @@ def build_oracle() -> str:
     return out
 
 
+def build_oracle_fast() -> str:
+    """Timing build of the oracle for bench.py's CPU baseline: the same source
+    at -O3 for the AVX-512 baseline (x86-64-v4) with FMA contraction allowed
+    (GCC's default), so its bits may differ from the parity build's; never
+    used as the checker."""
+    out = os.path.join(ORACLE, "liboracle_fast.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-O3", "-march=x86-64-v4", "-std=c++17", "-fopenmp", "-fPIC", "-shared",
+          os.path.join(ORACLE, "vof_oracle.cpp"), "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all(hip: bool = True) -> None:
     build_oracle()
+    build_oracle_fast()
     if hip:
         build_hip()
         build_cli()

@@ -68,6 +68,58 @@ def test_fields_beyond_2gib_fp64_one_step_bitwise(vofmod, orc):
     o.close()
 
 
+def test_dam_break_200_fp32_1000_steps_bitwise(vofmod, orc):
+    """The 200^3 FP32 dam break over its full 1000 steps (every sweep rotation
+    333 times), graph-replayed with the autotuned launch shapes, against the
+    oracle run of the same steps: every stored entry of F, u, v, w, p equal."""
+    print("200^3 FP32 dam break, 1000 steps, GPU vs oracle", flush=True)
+    s, o = _pair(vofmod, orc, dict(n=200, precision=4, g=(0.0, 0.0, -9.81)), lambda x: x.init_box(0.04, 0.04, 0.06))
+    s.autotune(1, 3, None)
+    s.step(1, 1000, graph=True)
+    for chunk in range(10):  # the oracle's 1000 steps, ~5 s each 100 on 16 threads
+        o.step(1 + 100 * chunk, 100)
+        print(f"  oracle step {100 * (chunk + 1)}", flush=True)
+    for f in ("F", "U", "V", "W", "P"):
+        assert_same(s.download(f), o.get(f), f)
+    s.close()
+    o.close()
+
+
+def test_ensemble_64_instances_500_steps(vofmod, orc):
+    """The 64-instance ensemble (uneven grids N in {32,...,96} and physics from
+    seed + i), autotuned launch shapes, one hipGraph per step, 500 steps in
+    one launch per stage.  One instance of every grid size is compared bit for
+    bit with its own oracle run; every instance stays finite with F in the
+    range clamp01 allows and its liquid volume within 1e-4 of the start."""
+    from vof_cases import ensemble_config
+
+    print("64-instance ensemble, 500 steps", flush=True)
+    prms, boxes = ensemble_config(vofmod)
+    bt = vofmod.Batch(prms)
+    bt.init_box(*zip(*boxes))
+    v0 = [bt.diagnostics(i)["liquid_volume"] for i in range(len(prms))]
+    bt.autotune(1, 2, None)
+    bt.step(1, 500, graph=True)
+    lo, hi = -(2.0 ** -25), 1.0 + 2.0 ** -23
+    for i in range(len(prms)):
+        d = bt.diagnostics(i)
+        assert d["nonfinite"] == 0 and lo <= d["f_min"] and d["f_max"] <= hi, (i, d)
+        assert abs(d["liquid_volume"] / v0[i] - 1.0) < 1e-4, (i, d["liquid_volume"] / v0[i] - 1.0)
+    first_of = {}
+    for i, p in enumerate(prms):
+        first_of.setdefault(p.nx, i)
+    assert sorted(first_of) == [32, 48, 64, 80, 96]
+    for n, i in sorted(first_of.items()):
+        print(f"  instance {i} ({n}^3) against the oracle", flush=True)
+        o = orc.Oracle(prms[i])
+        o.init_box(*boxes[i])
+        o.step(1, 500)
+        for f in ("F", "U", "V", "W", "P"):
+            assert_same(bt.download(i, f), o.get(f), f"instance {i} {f}")
+        o.close()
+    bt.close()
+
+
 def _fp32_vs_fp64(vofmod, n, steps, init):
     from vof_cases import field_errors
 
@@ -95,7 +147,13 @@ def test_dam_break_200_fp32_against_fp64_1000_steps(vofmod):
     assert errs["F"] < 5e-5
     assert max(errs["U"], errs["V"], errs["W"]) < 3e-4
     assert errs["P"] < 1.5e-3
-    assert abs(d32) < 1e-5 and abs(d64) < 1e-5
+    # The conservation target 1e-6 over the configuration's 1000 steps holds in
+    # FP64 (measured -2.1e-7).  The FP32 run, bit for bit the reference's FP32
+    # arithmetic, drifts -1.09e-6: its clamp01 rounds every F in (0, 1) to the
+    # 2^-23 grid of 1 + F and flushes values below 2^-24 to zero, which the
+    # FP64 run does 2^29 times more finely; the bound leaves it 1.8x room.
+    assert abs(d64) < 1e-6
+    assert abs(d32) < 2e-6
 
 
 def test_ripple_384_fp32_against_fp64_500_steps(vofmod):
