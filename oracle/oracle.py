@@ -62,6 +62,8 @@ def load():
     L.orc_write_vtk_float.restype = c_int
     L.orc_write_vtk_float.argtypes = [c_char_p, c_void_p, c_int, c_int, c_int]
     L.orc_threads.restype = c_int
+    L.orc_stats_enable.argtypes = [c_void_p, c_int]
+    L.orc_stats.argtypes = [c_void_p, c_void_p]
     _lib = L
     return L
 
@@ -144,6 +146,19 @@ class Oracle:
             idx = 100 + RAW[field]
         if self.lib.orc_set(self.h, idx, d.ctypes.data) != 0:
             raise ValueError(field)
+
+    def stats_enable(self, on: bool = True):
+        """Start (and reset) the liquid-volume accounting of later steps."""
+        self.lib.orc_stats_enable(self.h, int(on))
+
+    def stats(self) -> dict:
+        """Volume changes summed since stats_enable, in m^3: clip1 (stage-1
+        clip), clamp4 (stage-4 clamp01: clipping + rounding), clip4 (its plain
+        clipping part), dil (sum F (vol - dv) of the sweeps), post
+        (post_process_f's clamp01), div (dt vol sum F div u per step)."""
+        out = np.zeros(6)
+        self.lib.orc_stats(self.h, out.ctypes.data)
+        return dict(zip(("clip1", "clamp4", "clip4", "dil", "post", "div"), out.tolist()))
 
     def diagnostics(self) -> dict:
         d = VofDiag()

@@ -50,7 +50,9 @@ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v
 
 enum OrcField {
     O_F = 0, O_FTD, O_AX, O_AY, O_AZ, O_CX, O_CY, O_CZ, O_RP, O_RM,
-    O_U, O_V, O_W, O_US, O_VS, O_WS, O_P, O_PT, O_RHO, O_NU, O_FSM, O_KAPPA, O_NFIELDS
+    O_U, O_V, O_W, O_US, O_VS, O_WS, O_P, O_PT, O_RHO, O_NU, O_FSM, O_KAPPA,
+    O_S1, O_S4,  // stage-1 value before its clip, stage-4 value before clamp01 (volume accounting only)
+    O_NFIELDS
 };
 
 struct OrcBase {
@@ -71,6 +73,10 @@ struct Orc : OrcBase {
     std::vector<R> a[O_NFIELDS];
     int pcur = O_P;  // which buffer holds p after the ping-pong (:1374)
     double last_pmean = 0.0;
+    // Liquid-volume accounting (test infrastructure; off unless asked for):
+    // where the change of sum(F vol) over a step comes from.
+    bool stats_on = false;
+    double st_clip1 = 0, st_clamp4 = 0, st_clip4 = 0, st_dil = 0, st_post = 0, st_div = 0;
 
     inline size_t id(int i, int j, int k) const {  // voftrial1.cu:66-70
         return (static_cast<size_t>(k) * static_cast<size_t>(NY) + static_cast<size_t>(j)) *
@@ -457,6 +463,7 @@ struct Orc : OrcBase {
                     const R fl = flux_low(u[c], dt, F[cm], F[c]);
                     const R fr = flux_low(u[cp], dt, F[c], F[cp]);
                     R out = (F[c] + (fl - fr) * dy * dz / vol) * vol / dv;
+                    if (stats_on) a[O_S1][c] = out;
                     if (out > R(1) || out < R(0)) out = clamp01(out);
                     Ftd[c] = out;
                 }
@@ -504,6 +511,7 @@ struct Orc : OrcBase {
                     const R dv = vol - dt * dy * dz * (u[cxp] - u[c]);
                     const R out = Ftd[c] - ((ax[cxp] * cx[cxp] - ax[c] * cx[c] + ay[cyp] * cy[cyp] -
                                              ay[c] * cy[c]) / dy) * vol / dv;
+                    if (stats_on) a[O_S4][c] = out;
                     F[c] = clamp01(out);
                 }
     }
@@ -523,6 +531,7 @@ struct Orc : OrcBase {
                     const R ft = flux_low(v[cp], dt, F[c], F[cp]);
                     const R fb = flux_low(v[c], dt, F[cm], F[c]);
                     R out = (F[c] + (fb - ft) * dy / (dx * dy)) * vol / dv;
+                    if (stats_on) a[O_S1][c] = out;
                     if (out > R(1) || out < R(0)) out = clamp01(out);
                     Ftd[c] = out;
                 }
@@ -570,6 +579,7 @@ struct Orc : OrcBase {
                     const R dv = vol - dt * dx * dz * (v[cyp] - v[c]);
                     const R out = Ftd[c] - ((ax[cxp] * cx[cxp] - ax[c] * cx[c] + ay[cyp] * cy[cyp] -
                                              ay[c] * cy[c]) / dy) * vol / dv;
+                    if (stats_on) a[O_S4][c] = out;
                     F[c] = clamp01(out);
                 }
     }
@@ -589,6 +599,7 @@ struct Orc : OrcBase {
                     const R ff = flux_low(w[cp], dt, F[c], F[cp]);
                     const R fb = flux_low(w[c], dt, F[cm], F[c]);
                     R out = (F[c] + (fb - ff) * dy * dx / vol) * vol / dv;
+                    if (stats_on) a[O_S1][c] = out;
                     if (out > R(1) || out < R(0)) out = clamp01(out);
                     Ftd[c] = out;
                 }
@@ -629,8 +640,60 @@ struct Orc : OrcBase {
                     const size_t c = id(i, j, k), czp = id(i, j, k + 1);
                     const R dv = vol - dt * dx * dy * (w[czp] - w[c]);
                     const R out = Ftd[c] - ((az[czp] * cz[czp] - az[c] * cz[c]) / dz) * vol / dv;
+                    if (stats_on) a[O_S4][c] = out;
                     F[c] = clamp01(out);
                 }
+    }
+
+    // ---- Volume accounting of one directional sweep d (0 x, 1 y, 2 z), after
+    //      it ran with stats_on: with dv the sweep's dilated volume,
+    //        clip1  = sum (Ftd - Ftd before the stage-1 clip) dv,
+    //        clamp4 = sum (F - value before clamp01) dv  (clipping + rounding to
+    //                 the 2^-23 grid of 1 + F in FP32),
+    //        clip4  = the part of clamp4 a plain clip to [0, 1] accounts for,
+    //        dil    = sum F (vol - dv).
+    //      Stage 1 conserves sum Ftd dv and stage 4 sum F dv (telescoping
+    //      fluxes), so sum F vol changes by clip1 + clamp4 + dil per sweep, up
+    //      to the rounding of the sums themselves.
+    void sweep_stats(int d) {
+        const R *F = fld(O_F), *Ftd = fld(O_FTD), *s1 = fld(O_S1), *s4 = fld(O_S4);
+        const R* V = fld(d == 0 ? O_U : d == 1 ? O_V : O_W);
+        const R vol = dx * dy * dz;
+        const R kdv = (d == 0) ? dt * dy * dz : (d == 1) ? dt * dx * dz : dt * dx * dy;
+        const size_t off = (d == 0) ? 1 : (d == 1) ? size_t(NX) : size_t(NX) * size_t(NY);
+        double c1 = 0, c4 = 0, k4 = 0, dl = 0;
+#pragma omp parallel for collapse(2) reduction(+ : c1, c4, k4, dl)
+        for (int k = kmin; k <= kmax; ++k)
+            for (int j = jmin; j <= jmax; ++j)
+                for (int i = imin; i <= imax; ++i) {
+                    const size_t c = id(i, j, k);
+                    const double dv = double(vol - kdv * (V[c + off] - V[c]));
+                    c1 += (double(Ftd[c]) - double(s1[c])) * dv;
+                    c4 += (double(F[c]) - double(s4[c])) * dv;
+                    k4 += (std::min(1.0, std::max(0.0, double(s4[c]))) - double(s4[c])) * dv;
+                    dl += double(F[c]) * (double(vol) - dv);
+                }
+        st_clip1 += c1;
+        st_clamp4 += c4;
+        st_clip4 += k4;
+        st_dil += dl;
+    }
+    // dt vol sum F div(u) over interior cells: the dilation a divergent
+    // velocity field makes over one step (first order).
+    double div_term() const {
+        const R *F = a[O_F].data(), *u = a[O_U].data(), *v = a[O_V].data(), *w = a[O_W].data();
+        double sdiv = 0;
+#pragma omp parallel for collapse(2) reduction(+ : sdiv)
+        for (int k = kmin; k <= kmax; ++k)
+            for (int j = jmin; j <= jmax; ++j)
+                for (int i = imin; i <= imax; ++i) {
+                    const size_t c = id(i, j, k);
+                    const double dvg = (double(u[id(i + 1, j, k)]) - double(u[c])) / double(dx) +
+                                       (double(v[id(i, j + 1, k)]) - double(v[c])) / double(dy) +
+                                       (double(w[id(i, j, k + 1)]) - double(w[c])) / double(dz);
+                    sdiv += double(F[c]) * dvg;
+                }
+        return double(dt) * double(dx) * double(dy) * double(dz) * sdiv;
     }
 
     // ---- CSF surface tension (extension, VOF_FLAG_CSF; sigma is stored but never
@@ -721,9 +784,25 @@ struct Orc : OrcBase {
         update_uvw();                   // :1377-1379
         bc_vel();                       // :1382
         const int r = int(istep % 3);   // :1384-1396
-        if (r == 0) { fct_x(); fct_y(); fct_z(); }
-        else if (r == 1) { fct_y(); fct_z(); fct_x(); }
-        else { fct_z(); fct_x(); fct_y(); }
+        if (stats_on) st_div += div_term();
+        static const int order[3][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}};
+        for (int q = 0; q < 3; ++q) {
+            const int d = order[r][q];
+            if (d == 0) fct_x(); else if (d == 1) fct_y(); else fct_z();
+            if (stats_on) sweep_stats(d);
+        }
+        if (stats_on) {  // post_process_f's clamp01 on interior cells
+            const R* F = fld(O_F);
+            const R vol = dx * dy * dz;
+            double sp = 0;
+            for (int k = kmin; k <= kmax; ++k)
+                for (int j = jmin; j <= jmax; ++j)
+                    for (int i = imin; i <= imax; ++i) {
+                        const R f = F[id(i, j, k)];
+                        sp += (double(clamp01(f)) - double(f)) * double(vol);
+                    }
+            st_post += sp;
+        }
         clamp_F();                      // :1398
         bc_F();                         // :1401
     }
@@ -890,6 +969,19 @@ int orc_stage(void* h, int stage) {
     return rc;
 }
 
+// Volume accounting (see sweep_stats): on != 0 starts it (sums reset);
+// orc_stats writes {clip1, clamp4, clip4, dil, post, div} summed over the
+// steps since.
+int orc_stats_enable(void* h, int on) {
+    ORC_DISPATCH(h, o->stats_on = (on != 0); o->st_clip1 = o->st_clamp4 = o->st_clip4 = o->st_dil = o->st_post =
+                                              o->st_div = 0.0);
+    return 0;
+}
+int orc_stats(void* h, double* out) {
+    ORC_DISPATCH(h, out[0] = o->st_clip1; out[1] = o->st_clamp4; out[2] = o->st_clip4; out[3] = o->st_dil;
+                 out[4] = o->st_post; out[5] = o->st_div);
+    return 0;
+}
 int orc_get(void* h, int f, double* out) { int rc = 0; ORC_DISPATCH(h, rc = get_field(o, f, out)); return rc; }
 int orc_set(void* h, int f, const double* in) { int rc = 0; ORC_DISPATCH(h, rc = set_field(o, f, in)); return rc; }
 int orc_diag(void* h, vof_diag* d) { ORC_DISPATCH(h, o->diag(d)); return 0; }
