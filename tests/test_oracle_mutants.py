@@ -124,8 +124,8 @@ MUTANTS = [
      "                    cy[cyp] = (ay[cyp] >= R(0)) ? std::fmin(rp[cyp], rm[c]) : std::fmin(rp[c], rm[cyp]);\n"
      "                }\n        // stage 4 (:619-646)"),
     ("fct_x: stage 4 without the dilation factor",
-     "ay[c] * cy[c]) / dy) * vol / dv;\n                    F[c] = clamp01(out);\n                }\n    }\n\n    // ================= FCT y",
-     "ay[c] * cy[c]) / dy);\n                    F[c] = clamp01(out);\n                }\n    }\n\n    // ================= FCT y"),
+     "ay[c] * cy[c]) / dy) * vol / dv;\n                    if (stats_on) a[O_S4][c] = out;\n                    F[c] = clamp01(out);\n                }\n    }\n\n    // ================= FCT y",
+     "ay[c] * cy[c]) / dy);\n                    if (stats_on) a[O_S4][c] = out;\n                    F[c] = clamp01(out);\n                }\n    }\n\n    // ================= FCT y"),
     ("fct_x: stage 1 dilation sign",
      "const R dv = vol - dt * dy * dz * (u[cp] - u[c]);", "const R dv = vol + dt * dy * dz * (u[cp] - u[c]);"),
     ("fct_x: Zalesak P+ from the wrong faces",
