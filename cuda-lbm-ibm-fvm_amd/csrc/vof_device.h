@@ -173,6 +173,29 @@ __device__ __forceinline__ R div_rn(R a, R b, R y, bool ok_b) {
     return q;
 }
 
+// The same quotient without divergent branches on the common path: the
+// corrected quotient is formed on every lane, and only when some lane of the
+// wave needs it (a wave-uniform test) do those lanes redo it as the IEEE
+// division.  Fewer exec-mask (scalar) instructions than div_rn.
+template <class R>
+__device__ __forceinline__ R div_rn_wave(R a, R b, R y, bool ok_b) {
+    const R q0 = a * y;
+    const R r0 = fma_r(-b, q0, a);
+    const R q1 = fma_r(r0, y, q0);
+    const R r1 = fma_r(-b, q1, a);
+    R q = fma_r(r1, y, q1);
+    const bool bad = !(ok_b && div_a_ok(a));
+    if (__builtin_amdgcn_ballot_w64(bad) != 0) {
+        // The empty volatile asm keeps the division inside this branch (the
+        // compiler would otherwise compute it on every lane and select).
+        R aa = a;
+        asm volatile("" : "+v"(aa));
+        const R qs = aa / b;
+        q = bad ? qs : q;
+    }
+    return q;
+}
+
 template <class R> __device__ __forceinline__ R vmax(R a, R b);
 template <class R> __device__ __forceinline__ R vmin(R a, R b);
 template <> __device__ __forceinline__ float vmax<float>(float a, float b) { return fmaxf(a, b); }

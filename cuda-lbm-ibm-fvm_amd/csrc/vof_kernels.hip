@@ -385,6 +385,8 @@ struct JacAp {
     }
     // num / ap on an interior plane (1 < s < nz): reused reciprocal
     __device__ __forceinline__ R div_in(R num) const { return div_rn(num, in, rin, kin); }
+    // the same with the wave-uniform fallback test (div_rn_wave)
+    __device__ __forceinline__ R div_in_wave(R num) const { return div_rn_wave(num, in, rin, kin); }
     // num / ap of plane s (s wave-uniform); the first and last planes (two
     // of every column's nz) divide the IEEE way.  Bit-identical either way.
     __device__ __forceinline__ R div(R num, int s, int nz) const {
@@ -592,7 +594,10 @@ __global__ __launch_bounds__(1024) void k_jacobi2(const EnsArg<R> ea,
 template <class R, int KZ>
 __global__ __launch_bounds__(1024) void k_jacobi2_blk(const EnsArg<R> ea, int psrc) {
     VOF_PROLOGUE(R)
-    __shared__ R ring[3][kJ2Rows][64];
+    // sweep-1 planes shared across rows: one slot per plane of the chunk in an
+    // interior chunk (slot index known at compile time, no slot reuse), the
+    // first three as a ring (plane mod 3) in the others
+    __shared__ R ring[KZ + 2][kJ2Rows][64];
     const int nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
     const int nxt = (nx + kJ2OutLanes - 1) / kJ2OutLanes;
     const int nbd = (ny + kJ2OutRows - 1) / kJ2OutRows;
@@ -631,6 +636,40 @@ __global__ __launch_bounds__(1024) void k_jacobi2_blk(const EnsArg<R> ea, int ps
     // sweep 1 at plane s, sweep 2 at plane s-1, for s = k0-1 .. k1+1; an
     // interior chunk (IN) has every coefficient and divisor of its interior
     // planes fixed, the others select them per plane
+    if (interior) {
+        // Every plane the chunk forms is interior: coefficients and divisor
+        // fixed, sweep 1 formed on every lane and masked by a select, and one
+        // barrier per two planes.  Sweep 1 of planes 2t, 2t+1 (relative index
+        // e = plane - (k0-1)) is written to its own slot, then after the
+        // barrier sweep 2 of planes 2t-1 and 2t reads the rows around it; its
+        // z neighbours are in registers (q1v).
+        R q1v[KZ + 2];
+#pragma unroll
+        for (int t = 0; t < (KZ + 2) / 2; ++t) {
+#pragma unroll
+            for (int e = 2 * t; e < 2 * t + 2; ++e) {
+                const R q0 = pc[e + 1];
+                const R q0w = sh_up(q0), q0e = sh_dn(q0);
+                const R num = jac_num(bb[e], ae, aw, an, as, azz, azz, q0e, q0w, pjp[e], pjm[e], pc[e + 2], pc[e]);
+                const R q = apk.div_in_wave(num);
+                q1v[e] = in1 ? q : R(0);
+                ring[e][w][lane] = q1v[e];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int e = 2 * t - 1; e <= 2 * t; ++e) {
+                if (e < 1 || e > KZ) continue;  // sweep-2 planes k0 .. k1
+                const R q1c = q1v[e];
+                const R q1w = sh_up(q1c), q1e = sh_dn(q1c);
+                const R q1s = ring[e][w == 0 ? 0 : w - 1][lane];
+                const R q1n = ring[e][w == kJ2Rows - 1 ? w : w + 1][lane];
+                const R num = jac_num(bb[e], ae, aw, an, as, azz, azz, q1e, q1w, q1n, q1s, q1v[e + 1], q1v[e - 1]);
+                const R q = apk.div_in_wave(num);
+                if (st2) pt.st(q, tc, (k0 - 1 + e) * sy);
+            }
+        }
+        return;
+    }
     auto march = [&](auto in_tag) __attribute__((always_inline)) {
         constexpr bool IN = decltype(in_tag)::value;
         R q1mm = R(0), q1m = R(0);
@@ -661,8 +700,7 @@ __global__ __launch_bounds__(1024) void k_jacobi2_blk(const EnsArg<R> ea, int ps
             q1m = q1;
         }
     };
-    if (interior) march(std::true_type{});
-    else march(std::false_type{});
+    march(std::false_type{});
 }
 
 // ---------------------------------------------------------------------------
