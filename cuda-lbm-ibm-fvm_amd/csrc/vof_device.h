@@ -127,33 +127,37 @@ __device__ __forceinline__ double lane_dn(double x) {
 }
 
 // ---------------------------------------------------------------------------
-// Division by a divisor whose reciprocal y = RN(1/b) is formed once (by an
-// IEEE division) and reused: Markstein's correction.  q0 = RN(a y) is within
-// two ulps of a/b; one correction with the residual r0 = a - b q0 (formed by
-// an FMA) gives a faithful q1; the residual of a faithful quotient is exact,
-// and a second correction then returns RN(a/b), the IEEE quotient (Markstein,
-// IBM J. Res. Dev. 34 (1990); Muller et al., Handbook of Floating-Point
-// Arithmetic, Thm. "Markstein").  The proof needs every intermediate in the
-// normal range: with |b| in [2^-40, 2^40] (ok_b, checked once per divisor)
-// that holds for |a| in [2^-60, 2^80]; any other a (0, tiny, huge, inf, NaN)
-// takes the IEEE division.  So the result is the IEEE quotient's bits for
-// every a and b; the common case costs one multiply and four FMAs instead of
-// the ~10-instruction scaled division with its quarter-rate reciprocal.
+// Division by a negative divisor whose reciprocal y = RN(1/b) is formed once
+// (by an IEEE division) and reused: Markstein's correction.  q0 = RN(a y) is
+// within two ulps of a/b; one correction with the residual r0 = a - b q0
+// (formed by an FMA) gives a faithful q1; the residual of a faithful quotient
+// is exact, and a second correction then returns RN(a/b), the IEEE quotient
+// (Markstein, IBM J. Res. Dev. 34 (1990); Muller et al., Handbook of
+// Floating-Point Arithmetic, Thm. "Markstein").  The proof needs every
+// intermediate in the normal range: with -b in [2^-40, 2^40] (ok_b, checked
+// once per divisor) that holds for |a| in [2^-60, 2^80].  A zero a also gives
+// the IEEE result when b < 0: q0 = a y is the signed zero a/b, both residuals
+// are +0, and the corrections keep q0's sign (with b > 0 and a = -0 they would
+// not).  Any other a (tiny, huge, inf, NaN) takes the IEEE division.  So the
+// result is the IEEE quotient's bits for every a; the common case costs one
+// multiply and four FMAs instead of the ~10-instruction scaled division with
+// its quarter-rate reciprocal.
 __device__ __forceinline__ bool div_b_ok(float b) {
-    const unsigned u = __float_as_uint(b) & 0x7fffffffu;
-    return u - 0x2b800000u <= 0x53800000u - 0x2b800000u;  // 2^-40 <= |b| <= 2^40
+    const unsigned u = __float_as_uint(b);
+    return u - 0xab800000u <= 0xd3800000u - 0xab800000u;  // 2^-40 <= -b <= 2^40
 }
 __device__ __forceinline__ bool div_b_ok(double b) {
-    const unsigned u = unsigned(__double2hiint(b)) & 0x7fffffffu;
-    return u - 0x3d700000u <= 0x42700000u - 0x3d700000u;  // 2^-40 <= |b| < 2^40 (1 + 2^-20)
+    const unsigned u = unsigned(__double2hiint(b));
+    return u - 0xbd700000u <= 0xc2700000u - 0xbd700000u;  // 2^-40 <= -b < 2^40 (1 + 2^-20)
 }
 __device__ __forceinline__ bool div_a_ok(float a) {
     const unsigned u = __float_as_uint(a) & 0x7fffffffu;
-    return u - 0x21800000u <= 0x67800000u - 0x21800000u;  // 2^-60 <= |a| <= 2^80
+    return u - 0x21800000u <= 0x67800000u - 0x21800000u || u == 0u;  // 2^-60 <= |a| <= 2^80, or a = +-0
 }
 __device__ __forceinline__ bool div_a_ok(double a) {
     const unsigned u = unsigned(__double2hiint(a)) & 0x7fffffffu;
-    return u - 0x3c300000u <= 0x44f00000u - 0x3c300000u;  // 2^-60 <= |a| < 2^80 (1 + 2^-20)
+    // 2^-60 <= |a| < 2^80 (1 + 2^-20), or a = +-0 (high word 0 and low word 0)
+    return u - 0x3c300000u <= 0x44f00000u - 0x3c300000u || (u | unsigned(__double2loint(a))) == 0u;
 }
 __device__ __forceinline__ float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
@@ -177,21 +181,41 @@ __device__ __forceinline__ R div_rn(R a, R b, R y, bool ok_b) {
 // corrected quotient is formed on every lane, and only when some lane of the
 // wave needs it (a wave-uniform test) do those lanes redo it as the IEEE
 // division.  Fewer exec-mask (scalar) instructions than div_rn.
+// The same quotient without divergent branches on the common path: the
+// corrected quotient is formed on every lane, and only when some lane of the
+// wave needs it (a wave-uniform test) do those lanes redo it as the IEEE
+// division.  The lane sets are wave masks formed once by the caller: need,
+// the lanes whose result is kept; must, those of them whose divisor fails
+// ok_b.  The range test is one compare per point whose lane mask is used as
+// is; zeros (exact with b < 0) are sorted out only when the range test fails.
+// a outside the range of div_a_ok (zero, NaN and inf included): above it
+// (or unordered) or below it, each one compare of |a| (a source modifier)
+// whose lane mask the ballot takes as is
+__device__ __forceinline__ bool div_a_above(float a) { return !(fabsf(a) <= 0x1p80f); }
+__device__ __forceinline__ bool div_a_below(float a) { return fabsf(a) < 0x1p-60f; }
+__device__ __forceinline__ bool div_a_above(double a) { return !(fabs(a) < 0x1.00001p80); }
+__device__ __forceinline__ bool div_a_below(double a) { return fabs(a) < 0x1p-60; }
+
 template <class R>
-__device__ __forceinline__ R div_rn_wave(R a, R b, R y, bool ok_b) {
+__device__ __forceinline__ R div_rn_wave(R a, R b, R y, uint64_t must, uint64_t need) {
     const R q0 = a * y;
     const R r0 = fma_r(-b, q0, a);
     const R q1 = fma_r(r0, y, q0);
     const R r1 = fma_r(-b, q1, a);
     R q = fma_r(r1, y, q1);
-    const bool bad = !(ok_b && div_a_ok(a));
-    if (__builtin_amdgcn_ballot_w64(bad) != 0) {
-        // The empty volatile asm keeps the division inside this branch (the
-        // compiler would otherwise compute it on every lane and select).
-        R aa = a;
-        asm volatile("" : "+v"(aa));
-        const R qs = aa / b;
-        q = bad ? qs : q;
+    uint64_t bad =
+        ((__builtin_amdgcn_ballot_w64(div_a_above(a)) | __builtin_amdgcn_ballot_w64(div_a_below(a))) & need) | must;
+    if (bad != 0) {
+        bad &= ~__builtin_amdgcn_ballot_w64(a == R(0)) | must;
+        if (bad != 0) {
+            // The empty volatile asm keeps the division inside this branch (the
+            // compiler would otherwise compute it on every lane and select).
+            R aa = a;
+            asm volatile("" : "+v"(aa));
+            const R qs = aa / b;
+            const int lane = (int)__lane_id();
+            q = ((bad >> lane) & 1u) ? qs : q;
+        }
     }
     return q;
 }

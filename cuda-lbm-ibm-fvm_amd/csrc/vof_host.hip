@@ -299,6 +299,8 @@ struct Ensemble : EnsBase {
         VOF_HIP(hipMemcpy(g.boff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
         g.nblk = off.back();
         g.kz = kz;
+        g.min_nz = ih.empty() ? 0 : ih[0].nz;
+        for (const auto& I : ih) g.min_nz = std::min(g.min_nz, I.nz);
         return VOF_OK;
     }
 
@@ -840,7 +842,8 @@ struct Ensemble : EnsBase {
         }
         // Jacobi temporal-block depth with its own chunk lengths
         const int other_tb = (tun.jacobi_tblock >= 2) ? 1 : 2;
-        for (int c : kzs) {
+        const int kzj[] = {2, 4, 8, 16, vof::kJ2DmaKz};
+        for (int c : kzj) {
             if (rc != VOF_OK) break;
             vof_tuning t = best;
             t.jacobi_tblock = other_tb;

@@ -12,6 +12,7 @@ struct Geo {
     int* boff = nullptr;
     int nblk = 0;
     int kz = 1;
+    int min_nz = 0;  // fewest planes over the instances
 };
 
 template <class R>
@@ -39,6 +40,10 @@ inline EnsArg<R> ens_arg(const Ens<R>& e, const Geo& g) {
 }
 
 constexpr int kNumReduce = 7;
+// Chunk length of the FP32 two-sweep Jacobi streamed through LDS requests
+// (k_jacobi2_dma); any other length uses the preloaded or register-streamed
+// kernels.
+constexpr int kJ2DmaKz = 25;
 // Largest k_reduce1 launch the allocation-free diagnostics serve: its
 // partials fill a static device array of this many work-groups x kNumReduce.
 constexpr int kNoAllocMaxBlocks = 65536;

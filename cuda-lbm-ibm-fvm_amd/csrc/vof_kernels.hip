@@ -385,8 +385,17 @@ struct JacAp {
     }
     // num / ap on an interior plane (1 < s < nz): reused reciprocal
     __device__ __forceinline__ R div_in(R num) const { return div_rn(num, in, rin, kin); }
-    // the same with the wave-uniform fallback test (div_rn_wave)
-    __device__ __forceinline__ R div_in_wave(R num) const { return div_rn_wave(num, in, rin, kin); }
+    // the same with the wave-uniform fallback test (div_rn_wave) for the
+    // lanes of the wave mask need (formed by need_mask)
+    struct Need {
+        uint64_t must, need;
+    };
+    __device__ __forceinline__ Need need_mask(bool need) const {
+        return Need{__builtin_amdgcn_ballot_w64(need && !kin), __builtin_amdgcn_ballot_w64(need)};
+    }
+    __device__ __forceinline__ R div_in_wave(R num, const Need& m) const {
+        return div_rn_wave(num, in, rin, m.must, m.need);
+    }
     // num / ap of plane s (s wave-uniform); the first and last planes (two
     // of every column's nz) divide the IEEE way.  Bit-identical either way.
     __device__ __forceinline__ R div(R num, int s, int nz) const {
@@ -594,10 +603,14 @@ __global__ __launch_bounds__(1024) void k_jacobi2(const EnsArg<R> ea,
 template <class R, int KZ>
 __global__ __launch_bounds__(1024) void k_jacobi2_blk(const EnsArg<R> ea, int psrc) {
     VOF_PROLOGUE(R)
-    // sweep-1 planes shared across rows: one slot per plane of the chunk in an
-    // interior chunk (slot index known at compile time, no slot reuse), the
-    // first three as a ring (plane mod 3) in the others
-    __shared__ R ring[KZ + 2][kJ2Rows][64];
+    // Sweep-1 planes shared across rows: a ring of five slots (plane mod 5)
+    // in an interior chunk, of three (plane mod 3) in the others.  Where it
+    // fits beside the ring within half a CU's LDS (two work-groups per CU),
+    // the p rows of an interior chunk's planes, with the row beyond each tile
+    // edge, are shared too (prow).
+    constexpr bool kRowsLds = sizeof(R) * 64 * ((KZ + 2) * (kJ2Rows + 2) + 5 * kJ2Rows) <= 80 * 1024;
+    __shared__ R ring[5][kJ2Rows][64];
+    __shared__ R prow[kRowsLds ? KZ + 2 : 1][kJ2Rows + 2][64];
     const int nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
     const int nxt = (nx + kJ2OutLanes - 1) / kJ2OutLanes;
     const int nbd = (ny + kJ2OutRows - 1) / kJ2OutRows;
@@ -624,36 +637,57 @@ __global__ __launch_bounds__(1024) void k_jacobi2_blk(const EnsArg<R> ea, int ps
     auto po = [&](int s) { return min(max(s, 0) * sy, olast); };  // clamped plane offset
     R pc[KZ + 4];                          // column, planes k0-2+e
     R pjm[KZ + 2], pjp[KZ + 2], bb[KZ + 2];  // planes k0-1+e
+    auto load_rows = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int e = 0; e < KZ + 2; ++e) {
+            const int o = po(k0 - 1 + e);
+            pjm[e] = p.ld(tjm, o);
+            pjp[e] = p.ld(tjp, o);
+        }
+    };
 #pragma unroll
     for (int e = 0; e < KZ + 4; ++e) pc[e] = p.ld(tc, po(k0 - 2 + e));
 #pragma unroll
-    for (int e = 0; e < KZ + 2; ++e) {
-        const int o = po(k0 - 1 + e);
-        pjm[e] = p.ld(tjm, o);
-        pjp[e] = p.ld(tjp, o);
-        bb[e] = rhs.ld(tc, o);
-    }
-    // sweep 1 at plane s, sweep 2 at plane s-1, for s = k0-1 .. k1+1; an
-    // interior chunk (IN) has every coefficient and divisor of its interior
-    // planes fixed, the others select them per plane
+    for (int e = 0; e < KZ + 2; ++e) bb[e] = rhs.ld(tc, po(k0 - 1 + e));
     if (interior) {
         // Every plane the chunk forms is interior: coefficients and divisor
         // fixed, sweep 1 formed on every lane and masked by a select, and one
         // barrier per two planes.  Sweep 1 of planes 2t, 2t+1 (relative index
-        // e = plane - (k0-1)) is written to its own slot, then after the
+        // e = plane - (k0-1)) is written to ring slot e mod 5, then after the
         // barrier sweep 2 of planes 2t-1 and 2t reads the rows around it; its
-        // z neighbours are in registers (q1v).
+        // z neighbours are in registers (q1v).  Five slots keep the pair a
+        // wave may write before the next barrier (2t+2, 2t+3) apart from the
+        // pair other waves may still read (2t-1, 2t).
+        if constexpr (kRowsLds) {
+            // Each wave stores its own column of p; the first and last waves
+            // also load the row beyond the tile.  A lane then issues 22 loads
+            // (32 on those two waves) instead of 42.
+            const bool edge = (w == 0 || w == kJ2Rows - 1);  // uniform per wave
+#pragma unroll
+            for (int e = 0; e < KZ + 2; ++e) prow[e][w + 1][lane] = pc[e + 1];
+            if (edge) {
+                const int th = (w == 0) ? tjm : tjp, hrow = (w == 0) ? 0 : kJ2Rows + 1;
+#pragma unroll
+                for (int e = 0; e < KZ + 2; ++e) prow[e][hrow][lane] = p.ld(th, po(k0 - 1 + e));
+            }
+            __syncthreads();
+        } else {
+            load_rows();
+        }
         R q1v[KZ + 2];
+        const auto nd1 = apk.need_mask(in1), nd2 = apk.need_mask(st2);
 #pragma unroll
         for (int t = 0; t < (KZ + 2) / 2; ++t) {
 #pragma unroll
             for (int e = 2 * t; e < 2 * t + 2; ++e) {
                 const R q0 = pc[e + 1];
                 const R q0w = sh_up(q0), q0e = sh_dn(q0);
-                const R num = jac_num(bb[e], ae, aw, an, as, azz, azz, q0e, q0w, pjp[e], pjm[e], pc[e + 2], pc[e]);
-                const R q = apk.div_in_wave(num);
+                const R qs = kRowsLds ? prow[e][w][lane] : pjm[e];
+                const R qn = kRowsLds ? prow[e][w + 2][lane] : pjp[e];
+                const R num = jac_num(bb[e], ae, aw, an, as, azz, azz, q0e, q0w, qn, qs, pc[e + 2], pc[e]);
+                const R q = apk.div_in_wave(num, nd1);
                 q1v[e] = in1 ? q : R(0);
-                ring[e][w][lane] = q1v[e];
+                ring[e % 5][w][lane] = q1v[e];
             }
             __syncthreads();
 #pragma unroll
@@ -661,15 +695,19 @@ __global__ __launch_bounds__(1024) void k_jacobi2_blk(const EnsArg<R> ea, int ps
                 if (e < 1 || e > KZ) continue;  // sweep-2 planes k0 .. k1
                 const R q1c = q1v[e];
                 const R q1w = sh_up(q1c), q1e = sh_dn(q1c);
-                const R q1s = ring[e][w == 0 ? 0 : w - 1][lane];
-                const R q1n = ring[e][w == kJ2Rows - 1 ? w : w + 1][lane];
+                const R q1s = ring[e % 5][w == 0 ? 0 : w - 1][lane];
+                const R q1n = ring[e % 5][w == kJ2Rows - 1 ? w : w + 1][lane];
                 const R num = jac_num(bb[e], ae, aw, an, as, azz, azz, q1e, q1w, q1n, q1s, q1v[e + 1], q1v[e - 1]);
-                const R q = apk.div_in_wave(num);
+                const R q = apk.div_in_wave(num, nd2);
                 if (st2) pt.st(q, tc, (k0 - 1 + e) * sy);
             }
         }
         return;
     }
+    // Chunks touching the first or last planes: sweep 1 at plane s, sweep 2
+    // at plane s-1, for s = k0-1 .. k1+1, coefficients and divisor selected
+    // per plane.
+    load_rows();
     auto march = [&](auto in_tag) __attribute__((always_inline)) {
         constexpr bool IN = decltype(in_tag)::value;
         R q1mm = R(0), q1m = R(0);
@@ -701,6 +739,198 @@ __global__ __launch_bounds__(1024) void k_jacobi2_blk(const EnsArg<R> ea, int ps
         }
     };
     march(std::false_type{});
+}
+
+// Streamed two sweeps (FP32) with the p planes and the rhs brought into LDS by
+// the buffer unit (buffer_load ... lds): a request for a plane ahead holds no
+// register, so kJ2DmaAhead groups of planes stay in flight behind the march
+// at no cost in occupancy, and each wave requests only its own row (the
+// first and last waves also the row beyond the tile) instead of its row and
+// both neighbours.  The tile marches along z through a chunk of KZ planes;
+// sweep 1 is formed at plane s and sweep 2 at plane s-1 between the same two
+// barriers.  Every chunk has KZ planes (the last one starts at nz-KZ+1 and
+// recomputes planes of the one before it with the same inputs, so both write
+// the same values), which makes the whole march, its ring slots, requests
+// and waits, known at compile time; only chunks that reach plane 1 or nz
+// test the plane per step.
+//
+// Ordering.  A request completes in LDS when the wave's vector-memory count
+// says so (loads retire in issue order; the count is taken as the number of
+// loads issued after the awaited ones, so stores still in flight only make
+// the wait longer), and the barrier then publishes every wave's rows.  The
+// barrier is the bare s_barrier with LDS writes waited for before it: a
+// work-group fence would also wait for the requests still in flight.  Step e
+// requests p plane e+A+3 (A = kJ2DmaAhead) into the slot of plane
+// e+A+3-kJ2DmaSlots <= e.  Plane r is read at step r-2 on the wave's own row,
+// which only that wave writes, and at step r-1 on the rows of the
+// neighbouring waves, which have all passed the barrier of step r-1 before
+// any wave reaches step r.  Rhs rows are read only by the wave that requests
+// them, at step r, and replaced at step r+kJ2DmaSlots-A-1 > r.
+constexpr int kJ2DmaAhead = 3;  // groups requested beyond the one awaited
+constexpr int kJ2DmaSlots = 6;  // p and rhs ring slots
+static_assert(kJ2DmaSlots >= kJ2DmaAhead + 3, "p slot reused before its last read");
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Lane l's element (per-lane offset vo, plane offset so) into row[l] in LDS.
+// Written as the instruction itself: the compiler's own wait for a request
+// into LDS is for every request in flight, which would drain the ring.  The
+// asm has no register result; it waits 5 states for an SGPR operand written
+// by a vector instruction and 1 after writing M0.
+__device__ __forceinline__ void row_to_lds(const Buf<float>& b, float* row, int vo, int so) {
+    const unsigned lds = (unsigned)(uintptr_t)(lds_void*)row;
+    asm volatile(
+        "s_nop 4\n\t"
+        "s_mov_b32 m0, %0\n\t"
+        "s_nop 0\n\t"
+        "buffer_load_dword %1, %2, %3 offen lds"
+        :
+        : "s"(lds), "v"(byte_off(vo, 4u)), "s"(b.r), "s"(byte_off(so, 4u))
+        : "memory", "m0");
+}
+
+// s_waitcnt vmcnt(N), other counters not waited on (gfx9 encoding).
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | (0x7 << 4) | (0xF << 8) | ((N >> 4) << 14));
+}
+
+// This wave's requests older than its last m groups (three each) complete.
+__device__ __forceinline__ void wait_groups(int m) {
+    switch (m) {  // a constant in the unrolled march
+        case 0: wait_vm<0>(); break;
+        case 1: wait_vm<3>(); break;
+        case 2: wait_vm<6>(); break;
+        default: wait_vm<9>(); break;
+    }
+}
+static_assert(kJ2DmaAhead <= 3, "wait_groups covers three groups ahead");
+
+// LDS writes of this wave done, then the work-group barrier; no code moves
+// across it.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_s_waitcnt(0xF | (0x7 << 4) | (0x0 << 8) | (0x3 << 14));  // lgkmcnt(0)
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+template <int KZ>
+__global__ __launch_bounds__(1024) void k_jacobi2_dma(const EnsArg<float> ea, int psrc) {
+    using R = float;
+    VOF_PROLOGUE(R)
+    constexpr int NS = kJ2DmaSlots, A = kJ2DmaAhead;
+    static_assert(KZ >= A, "prologue requests fit in the chunk");
+    __shared__ R P[NS][kJ2Rows + 2][64];  // p, rows j0-1 .. j0+kJ2Rows of the tile
+    __shared__ R B[NS][kJ2Rows][64];      // rhs
+    __shared__ R Q[3][kJ2Rows][64];       // sweep 1
+    const int nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
+    const int nxt = (nx + kJ2OutLanes - 1) / kJ2OutLanes;
+    const int nbd = (ny + kJ2OutRows - 1) / kJ2OutRows;
+    const int per_z = nxt * nbd;
+    const int zc = lb / per_z, rem = lb - zc * per_z;
+    const int bd = rem / nxt, xt = rem - bd * nxt;
+    const int lane = (int)threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int i = kJ2OutLanes * xt - 1 + lane;
+    const int j = kJ2OutRows * bd + w;
+    const int k0 = min(1 + zc * KZ, nz - KZ + 1);  // planes k0 .. k0+KZ-1 (the launch has nz >= KZ)
+    const bool in1 = (i >= 1 && i <= nx && j >= 1 && j <= ny);
+    const bool st2 = in1 && lane >= 2 && lane <= 61 && w >= 1 && w <= kJ2OutRows;
+    const int ic = min(max(i, 0), nx + 1), jc = min(max(j, 0), ny + 1);
+    const int tc = jc * sx + ic;
+    // third request of a group: the row beyond the tile on the first and last
+    // waves, the wave's own row again (same values) on the others, so that
+    // every wave counts three requests per group
+    const int th = (w == 0) ? max(jc - 1, 0) * sx + ic : (w == kJ2Rows - 1) ? min(jc + 1, ny + 1) * sx + ic : tc;
+    const int hrow = (w == 0) ? 0 : (w == kJ2Rows - 1) ? kJ2Rows + 1 : w + 1;
+    const Buf<R> p(sel2(I.p, psrc), I.ntot), pt(sel2(I.p, psrc ^ 1), I.ntot), rhs(I.rhs, I.ntot);
+    const R ae = (i != nx) ? I.axx : R(0), aw = (i != 1) ? I.axx : R(0);
+    const R an = (j != ny) ? I.ayy : R(0), as = (j != 1) ? I.ayy : R(0);
+    const R azz = I.azz;
+    const JacAp<R> apk(ae, aw, an, as, azz, nz);
+    const auto nd1 = apk.need_mask(in1), nd2 = apk.need_mask(st2);
+    // interior chunk: every plane the march forms, k0-1 .. k0+KZ, lies in 2 .. nz-1
+    const bool interior = (k0 >= 3 && k0 + KZ - 1 <= nz - 2);
+    const int olast = (nz + 1) * sy;
+    auto po = [&](int s) { return min(max(s, 0) * sy, olast); };  // clamped plane offset
+    // p plane r of the chunk is plane k0-2+r; rhs plane r is plane k0-1+r.
+    // Group x: p plane x+2 and rhs plane x; step e needs groups 0..e and p
+    // planes 0, 1.
+    auto req_p = [&](int slot, int o) __attribute__((always_inline)) {
+        row_to_lds(p, &P[slot][w + 1][0], tc, o);
+        row_to_lds(p, &P[slot][hrow][0], th, o);
+    };
+    auto req_b = [&](int slot, int o) __attribute__((always_inline)) { row_to_lds(rhs, &B[slot][w][0], tc, o); };
+    req_p(0, po(k0 - 2));
+    req_p(1, po(k0 - 1));
+#pragma unroll
+    for (int x = 0; x <= A; ++x) {
+        req_p((x + 2) % NS, po(k0 + x));
+        req_b(x % NS, po(k0 - 1 + x));
+    }
+    wait_vm<3 * A>();  // groups 1 .. A may stay in flight
+    lds_barrier();
+    const R q0m0 = P[0][w + 1][lane], q00 = P[1][w + 1][lane];  // p column at k0-2, k0-1
+    auto march = [&](auto in_tag) __attribute__((always_inline)) {
+        constexpr bool IN = decltype(in_tag)::value;
+        // plane offsets of the next requests (unclamped in an interior chunk)
+        R q0m = q0m0, q0 = q00;           // p column at s-1, s
+        R q1mm = R(0), q1m = R(0), bm = R(0);  // sweep 1 at s-2, s-1; rhs at s-1
+#pragma unroll
+        for (int e = 0; e <= KZ + 1; ++e) {
+            const int s = k0 - 1 + e;
+            // sweep 1 at plane s
+            const R q0p = P[(e + 2) % NS][w + 1][lane];
+            const R pjm = P[(e + 1) % NS][w][lane], pjp = P[(e + 1) % NS][w + 2][lane];
+            const R b = B[e % NS][w][lane];
+            const R q0w = sh_up(q0), q0e = sh_dn(q0);
+            R q1;
+            if (IN || (s >= 2 && s <= nz - 1)) {
+                const R q = apk.div_in_wave(jac_num(b, ae, aw, an, as, azz, azz, q0e, q0w, pjp, pjm, q0p, q0m), nd1);
+                q1 = in1 ? q : R(0);
+            } else {
+                q1 = R(0);
+                if (in1 && s >= 1 && s <= nz) {
+                    const R af = (s != nz) ? azz : R(0), ab = (s != 1) ? azz : R(0);
+                    q1 = apk.div(jac_num(b, ae, aw, an, as, af, ab, q0e, q0w, pjp, pjm, q0p, q0m), s, nz);
+                }
+            }
+            Q[e % 3][w][lane] = q1;
+            // request group e+A+1 (p plane k0+e+A+1, rhs plane k0+e+A), then
+            // wait for group e+1, the next step's
+            if (e + A + 1 <= KZ + 1) {
+                const int pp = k0 + e + A + 1, pb = pp - 1;
+                req_p((e + A + 3) % NS, IN ? pp * sy : po(pp));
+                req_b((e + A + 1) % NS, IN ? pb * sy : po(pb));
+            }
+            if (e <= KZ) wait_groups(min(A, KZ - e));
+            lds_barrier();
+            // sweep 2 at plane s-1 (planes k0 .. k0+KZ-1: steps 2 .. KZ+1)
+            if (e >= 2) {
+                const int s2 = s - 1;
+                const R q1w = sh_up(q1m), q1e = sh_dn(q1m);
+                const R q1s = Q[(e + 2) % 3][w == 0 ? 0 : w - 1][lane];
+                const R q1n = Q[(e + 2) % 3][w == kJ2Rows - 1 ? w : w + 1][lane];
+                R q;
+                if (IN || (s2 >= 2 && s2 <= nz - 1)) {
+                    q = apk.div_in_wave(jac_num(bm, ae, aw, an, as, azz, azz, q1e, q1w, q1n, q1s, q1, q1mm), nd2);
+                } else {
+                    const R af = (s2 != nz) ? azz : R(0), ab = (s2 != 1) ? azz : R(0);
+                    q = apk.div(jac_num(bm, ae, aw, an, as, af, ab, q1e, q1w, q1n, q1s, q1, q1mm), s2, nz);
+                }
+                if (st2) pt.st(q, tc, s2 * sy);
+            }
+            q0m = q0;
+            q0 = q0p;
+            q1mm = q1m;
+            q1m = q1;
+            bm = b;
+        }
+    };
+    if (interior) march(std::true_type{});
+    else march(std::false_type{});
 }
 
 // ---------------------------------------------------------------------------
@@ -1838,6 +2068,12 @@ void launch_jacobi2(const Ens<R>& e, const Geo& g, int psrc, hipStream_t st) {
     // One plane of prefetch: a deeper ring (2 planes in FP64, 4 in FP32, tried
     // at 200^3) costs the registers of the second work-group per CU and ran
     // 20-35% slower.
+    if constexpr (sizeof(R) == 4) {
+        if (g.kz == kJ2DmaKz && g.min_nz >= kJ2DmaKz) {  // streamed through LDS by the buffer unit
+            hipLaunchKernelGGL((k_jacobi2_dma<kJ2DmaKz>), grid, blk, 0, st, ens_arg(e, g), psrc);
+            return;
+        }
+    }
     hipLaunchKernelGGL((k_jacobi2<R, 1>), grid, blk, 0, st, ens_arg(e, g), psrc, g.kz);
 }
 template <class R>

@@ -270,14 +270,9 @@ def test_degenerate_grids(vofmod, orc, grid):
     o.close()
 
 
-@pytest.mark.parametrize("prec", [4, 8])
-@pytest.mark.parametrize("kz", [3, 4, 8, 16])
-def test_jacobi2_tiles_and_chunks(vofmod, orc, kz, prec):
-    """Two-sweep launches on a grid with several x tiles, row bands and z chunks
-    (partial last chunks); kz 3 and FP64 stream the planes, FP32 with 4, 8 and
-    16 preloads them."""
-    print(f"two-sweep jacobi kz={kz} prec={prec}", flush=True)
-    s, o = make_pair(vofmod, orc, nx=131, ny=33, nz=21, precision=prec, dt=1e-3)
+def _jacobi2_pair(vofmod, orc, kz, prec, nx=131, ny=33, nz=21):
+    print(f"two-sweep jacobi kz={kz} prec={prec} grid={(nx, ny, nz)}", flush=True)
+    s, o = make_pair(vofmod, orc, nx=nx, ny=ny, nz=nz, precision=prec, dt=1e-3)
     s.set_tuning(kz_jacobi=kz)
     rng = np.random.default_rng(SEED + 11)
     st = random_state(s.shape, rng, s.dtype)
@@ -291,6 +286,28 @@ def test_jacobi2_tiles_and_chunks(vofmod, orc, kz, prec):
         o.stage("jacobi")
         o.stage("jacobi")
         assert_same(s.download("P"), o.get("P"), f"P double sweep {it}")
+    s.close()
+    o.close()
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("kz", [1, 2, 3, 4, 5, 8, 16, 25])
+def test_jacobi2_tiles_and_chunks(vofmod, orc, kz, prec):
+    """Two-sweep launches on a grid with several x tiles, row bands and z chunks
+    (partial last chunks; 25 is one chunk over all 21 planes).  FP32 with 4, 8
+    and 16 preloads the chunk, other lengths and FP64 stream the planes through
+    registers."""
+    _jacobi2_pair(vofmod, orc, kz, prec)
+
+
+@pytest.mark.parametrize("grid", [(70, 20, 25), (125, 29, 26), (62, 15, 31), (70, 20, 60), (131, 33, 81)])
+def test_jacobi2_lds_streamed_chunks(vofmod, orc, grid):
+    """FP32 two sweeps streamed through LDS requests (chunks of 25 planes, the
+    last one shifted back to end at nz): one chunk that holds the first and
+    the last plane (nz 25), two overlapping chunks (26, 31), and columns with
+    interior chunks between them (60, 81), with one and several x tiles and
+    row bands."""
+    _jacobi2_pair(vofmod, orc, 25, 4, *grid)
 
 
 @pytest.mark.parametrize("tblock", [1, 2])
