@@ -70,6 +70,9 @@ def parse():
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="run one plain copy of this many bytes first (byte-counter calibration)")
     ap.add_argument("--jacobi-tblock", type=int, default=0)
+    ap.add_argument("--case", choices=["dam", "ripple"], default="dam",
+                    help="initial state: the dam break of the metric (default), or the liquid layer with a rippled "
+                         "surface (mostly liquid: every FCT wave does the limiter arithmetic), for profiling")
     ap.add_argument("--study", choices=["precision", "long"], default=None,
                     help="instead of the benchmark line, print the FP32-vs-FP64 study of the 200^3 dam break "
                          "(1000 steps) and the 384^3 ripple layer (500 steps); 'long': the 200^3 dam break "
@@ -77,27 +80,36 @@ def parse():
     return ap.parse_args()
 
 
-def load_traffic(kernel):
+def load_traffic(kernel, tuning):
     """HBM bytes per launch of `kernel` from the committed counter summary
     (profiles/traffic_200.json, written from rocprofv3 FETCH_SIZE / WRITE_SIZE
-    runs scaled by a copy kernel of known size), or None."""
+    runs scaled by a copy kernel of known size) and a label naming that file
+    and the launch shapes it was profiled with; (None, reason) when the file
+    is absent or was profiled with other launch shapes than `tuning`."""
     path = os.path.join(ROOT, "profiles", "traffic_200.json")
     try:
         with open(path) as fh:
             t = json.load(fh)
-        return t["kernels"][kernel]["bytes_per_launch"]
+        byts = t["kernels"][kernel]["bytes_per_launch"]
+        prof = t.get("tuning", {})
     except (OSError, KeyError, ValueError):
-        return None
-
-
-def make_solver(vof, n, prec):
-    prm = vof.default_params(n=n, precision=prec, g=(0.0, 0.0, -9.81))
-    s = vof.Solver(prm)
-    s.init_box(0.04, 0.04, 0.06)
-    return s
+        return None, "no counter summary for this kernel"
+    if any(tuning.get(k) != v for k, v in prof.items()) or not prof:
+        return None, f"profiles/traffic_200.json was profiled with tuning {prof}, this run uses {tuning}"
+    return byts, f"rocprofv3 FETCH_SIZE + WRITE_SIZE from profiles/traffic_200.json, tuning {prof}"
 
 
 from vof_cases import SEED, ensemble_config, field_errors, ripple_F  # noqa: E402
+
+
+def make_solver(vof, n, prec, case="dam"):
+    prm = vof.default_params(n=n, precision=prec, g=(0.0, 0.0, -9.81))
+    s = vof.Solver(prm)
+    if case == "ripple":
+        s.upload("F", ripple_F(n, dtype=s.dtype))
+    else:
+        s.init_box(0.04, 0.04, 0.06)
+    return s
 
 
 def batched_leg(vof, steps=100, warmup=10):
@@ -457,7 +469,7 @@ def main():
     n, prec = a.n, a.precision
     esz = 8 if prec == 8 else 4
     cells = n ** 3
-    s = make_solver(vof, n, prec)
+    s = make_solver(vof, n, prec, a.case)
     if a.jacobi_tblock:
         s.set_tuning(jacobi_tblock=a.jacobi_tblock)
     if not a.no_autotune:
@@ -509,12 +521,18 @@ def main():
         kern[k] = {"ms_per_launch": t_launch * 1e3, "bytes_per_launch": byts, "gbs": byts / t_launch / 1e9,
                    "frac": byts / t_launch / 1e9 / HBM_PEAK_GBS}
     kern["jacobi"]["sweeps_per_launch"] = sweeps_per_launch
+    # what a launch must move whatever its sweeps: read p and rhs, write p
+    kern["jacobi"]["compulsory_bytes_per_launch"] = STAGE_BYTES["jacobi"] * (esz // 4) * cells
+    kern["jacobi"]["frac_compulsory"] = (kern["jacobi"]["compulsory_bytes_per_launch"] /
+                                         (kern["jacobi"]["ms_per_launch"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
     for k in ("fct_zx", "fct_xy"):
         if k in kern:
             kern[k]["sweeps_per_launch"] = 2
             kern[k]["compulsory_bytes_per_launch"] = FCT_PAIR_COMPULSORY_BYTES * (esz // 4) * cells
     dom = max(kern, key=lambda k: per_step[k])
     d = kern[dom]
+    traffic, traffic_source = (load_traffic(dom, tuning) if (n == 200 and prec == 4 and a.case == "dam")
+                               else (None, "counter bytes are profiled for the 200^3 FP32 dam break only"))
     step_bytes = sum(STAGE_BYTES[k] * LAUNCHES_PER_STEP[k] for k in STAGE_BYTES) * (esz // 4) * cells
     diag = s.diagnostics()
     pcie = transfer_leg(s, cells, ms_step)
@@ -536,8 +554,11 @@ def main():
         "dtype": "f64" if prec == 8 else "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"dam-break {n}^3 {'FP64' if prec == 8 else 'FP32'}, 0.1 m cube, liquid x<0.04 y<0.04 "
-                        f"z<0.06 m, rho 1000/50, g=(0,0,-9.81), dt=4e-6 s, 10 Jacobi sweeps/step, "
+            "workload": ((f"dam-break {n}^3 {'FP64' if prec == 8 else 'FP32'}, 0.1 m cube, liquid x<0.04 y<0.04 "
+                          f"z<0.06 m") if a.case == "dam" else
+                         (f"ripple layer {n}^3 {'FP64' if prec == 8 else 'FP32'}, 0.1 m cube, liquid below 0.06 m "
+                          f"with 4 seeded plane waves")) +
+                        f", rho 1000/50, g=(0,0,-9.81), dt=4e-6 s, 10 Jacobi sweeps/step, "
                         f"{'eager launches' if a.eager else 'hipGraph per step'}",
             "grid": [n, n, n],
             "interior_cells": cells,
@@ -550,7 +571,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": d["frac"],
-            "traffic": (load_traffic(dom) if (n == 200 and prec == 4) else None),
+            "traffic": traffic,
+            "traffic_source": traffic_source,
+            "compulsory_bytes_per_launch": d.get("compulsory_bytes_per_launch"),
+            "frac_compulsory": d.get("frac_compulsory"),
             "bytes_per_launch": d["bytes_per_launch"],
             "ms_per_launch": d["ms_per_launch"],
             "sweeps_per_launch": d.get("sweeps_per_launch"),

@@ -1123,6 +1123,13 @@ __global__ __launch_bounds__(256) void k_correct_blk(const EnsArg<R> ea, int fse
 // its value to every ghost cell that clamps onto it (in the kernels that end a
 // plain step, the ghost-column lanes store the x ghosts: store_F_row_fin).
 
+// x is +0 (not -0).  Where the antidiffusive fluxes a of a cell's two faces
+// are +0, their limited products a*C are +0 (C is finite and >= 0), so the
+// stage-4 correction ((+0 - +0) / s) * vol / dv is +0 for dv > 0 and
+// Fnew = clamp01(Ftd - +0) = clamp01(Ftd) bit for bit.
+__device__ __forceinline__ bool pos_zero(float x) { return __float_as_uint(x) == 0u; }
+__device__ __forceinline__ bool pos_zero(double x) { return __double_as_longlong(x) == 0; }
+
 template <class R, int DIR>
 __device__ __forceinline__ R fct_stage1_scale(const Inst<R>& I, R d) {
     if (DIR == 0) return d * I.dy * I.dz / I.vol;
@@ -1248,6 +1255,9 @@ __device__ __forceinline__ R fct_x_value(const Inst<R>& I, const R f0, const R u
     const R c1 = cell ? ((a1 >= R(0)) ? vmin(rp1, rm) : vmin(rp, rm1)) : R(0);
     const R ac1 = a1 * c1;
     const R ac0 = sh_up(ac1);
+    // No limited antidiffusion through either face of any cell of the wave
+    // (liquid and gas interiors): the correction is +0 and Fnew = clamp01(Ftd).
+    if (__all(!cell || (pos_zero(ac1) && pos_zero(ac0) && dv > R(0)))) return cell ? clamp01(ftd) : R(0);
     return cell ? clamp01(ftd - ((ac1 - ac0) / I.dy) * I.vol / dv) : R(0);
 }
 
@@ -1385,8 +1395,12 @@ __device__ __forceinline__ R fct_win_step(const Inst<R>& I, FctWin<R>& w, int c,
         // limiter on face c-1
         cm1 = R(0);
         if (c - 1 >= 2 && c - 1 <= nd + 1) cm1 = (w.am1 >= R(0)) ? vmin(w.rpm1, w.rmm2) : vmin(w.rpm2, w.rmm1);
-        // corrected value of cell c-2
-        if (emit) out = clamp01(w.ftm2 - ((w.am1 * cm1 - w.am2 * w.cm2) / s4) * vol / w.dvm2);
+        // corrected value of cell c-2; with no antidiffusion through either of
+        // its faces on any lane of the wave, the correction is +0
+        if (emit) {
+            if (__all(pos_zero(w.am1) && pos_zero(w.am2) && w.dvm2 > R(0))) out = clamp01(w.ftm2);
+            else out = clamp01(w.ftm2 - ((w.am1 * cm1 - w.am2 * w.cm2) / s4) * vol / w.dvm2);
+        }
     }
     // shift the window
     w.Fc = Fcp; w.Vc = Vcp; w.Lc = Lcp;

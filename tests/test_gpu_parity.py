@@ -347,6 +347,40 @@ def test_whole_steps_random_velocity_large_dt(vofmod, orc):
     assert float(interior(s.download("F")).min()) >= F32_LO and float(interior(s.download("F")).max()) <= F32_HI
 
 
+FUSED = dict(jacobi_tblock=2, kz_jacobi=8, kz_fct_x=8, fct_chunk=8, fct_zx_fused=2, fct_xy_fused=2)
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("tuning", [None, FUSED], ids=["default", "fused"])
+def test_whole_steps_ripple_layer(vofmod, orc, prec, tuning):
+    """Liquid-rich state: the ripple layer (liquid below a surface of four
+    seeded plane waves) at 48^3 with a large time step, six steps (each sweep
+    order twice), with the default and the fused launch shapes.  Below the
+    surface whole waves have no antidiffusion and take the FCT's stage-4
+    shortcut; across it they do the limiter arithmetic.  F is compared bit for
+    bit (zero signs included), the other fields by value."""
+    from vof_cases import ripple_F
+    n = 48
+    print(f"ripple layer {n}^3 prec={prec} tuning={tuning}", flush=True)
+    s, o = make_pair(vofmod, orc, n=n, precision=prec, g=(0.0, 0.0, -9.81), dt=2e-4)
+    if tuning:
+        s.set_tuning(**tuning)
+    F = ripple_F(n, dtype=s.dtype)
+    s.upload("F", F)
+    o.set("F", F)
+    for step in range(1, 7):
+        s.step(step, 1, graph=(step > 3))
+        o.step(step, 1)
+        a, b = s.download("F"), o.get("F")
+        ui = np.uint64 if a.dtype == np.float64 else np.uint32
+        assert np.array_equal(a.view(ui), b.view(ui)), f"F bits after step {step}"
+        for f in ("U", "V", "W"):
+            assert_same(s.download(f), o.get(f), f"{f} after step {step}")
+        assert_same(interior(s.download("P")), interior(o.get("P")), f"P after step {step}")
+    s.close()
+    o.close()
+
+
 def test_graph_equals_eager(vofmod):
     print("graph == eager", flush=True)
     a = vofmod.Solver(n=20)
