@@ -142,6 +142,25 @@ def test_cli_help_is_the_reference_usage():
         assert r.stdout == REF_USAGE
 
 
+# The reference's banner (voftrial1.cu:1295-1301) for its defaults: 200^3,
+# dt 4e-6 (std::scientific), rho 1000/50, gy -5, sigma 0.007, nu 1e-6/1.5e-5
+# (float quotients in the stream's default 6 significant digits).
+REF_BANNER = (">>> A 3D VOF solver ported to CUDA (speed-tuned path)\n"
+              ">>> Grid resolution: 200 x 200 x 200, dt = 4.000000e-06\n"
+              ">>> Density ratio: 20, gravity : -5, sigma : 0.007\n"
+              ">>> Viscosity ratio: 0.0666667\n")
+
+
+def test_cli_banner_known_answer(tmp_path):
+    """The banner is printed before the device is touched, so its text is
+    checked here whether or not a GPU is present (without one the run then
+    stops with the runtime's error and status 1; with one it runs a step)."""
+    r = cli("--n", "8", "--max_steps", "1", "--outdir", str(tmp_path))
+    assert r.stdout.startswith(REF_BANNER.replace("200 x 200 x 200", "8 x 8 x 8"))
+    r = cli("--max_steps", "1", "--outdir", str(tmp_path), "--load-state", str(tmp_path / "absent"))
+    assert r.stdout.startswith(REF_BANNER)
+
+
 def test_cli_rejects_bad_ic():
     r = cli("-ic", "4")
     assert r.returncode == 1

@@ -102,6 +102,51 @@ def test_cli_restart_and_async_output(vofmod, tmp_path):
         (tmp_path / "c" / "output" / "step-00010.vtk").read_bytes()
 
 
+def test_cli_sigint_stops_after_the_current_step(vofmod, tmp_path):
+    """--max_steps 0 runs until Ctrl+C (voftrial1.cu:46-50, :1229, :1361): after
+    SIGINT the loop ends with the step in progress, the pending files are
+    written, and the run closes with '>>> Finished at step N in T s' and
+    status 0 (:1420)."""
+    import re
+    import select
+    import signal
+    import time
+
+    print("CLI SIGINT stop, 24^3", flush=True)
+    p = subprocess.Popen([vofmod.CLI_PATH, "--n", "24", "--max_steps", "0", "--nstep", "20", "--outdir",
+                          str(tmp_path)], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    lines = []
+    try:
+        deadline = time.monotonic() + 60
+        while time.monotonic() < deadline:
+            ready, _, _ = select.select([p.stdout], [], [], 1.0)
+            if not ready:
+                continue
+            line = p.stdout.readline()
+            if not line:
+                break
+            lines.append(line)
+            if line.startswith(">>> Exporting step-00040"):
+                break
+        assert any(ln.startswith(">>> Exporting step-00040") for ln in lines), lines
+        p.send_signal(signal.SIGINT)
+        out, err = p.communicate(timeout=60)
+    finally:
+        if p.poll() is None:
+            p.kill()
+            p.wait()
+    lines += out.splitlines(keepends=True)
+    assert p.returncode == 0, err
+    m = re.fullmatch(r">>> Finished at step (\d+) in [0-9.eE+-]+ s\n", lines[-1])
+    assert m, lines[-1]
+    n = int(m.group(1))
+    assert n >= 40
+    exported = sorted(f for f in os.listdir(tmp_path) if f.endswith(".vtk"))
+    assert exported == [f"step-{k:05d}.vtk" for k in range(20, n + 1, 20)]
+    assert "".join(lines).startswith(">>> A 3D VOF solver ported to CUDA (speed-tuned path)\n"
+                                     ">>> Grid resolution: 24 x 24 x 24, dt = 4.000000e-06\n")
+
+
 def test_negative_step_indices_are_refused(vofmod, tmp_path):
     """Step indices select the sweep rotation istep % 3 and are 1-based; a
     negative one is refused by every entry point that takes one, and a
