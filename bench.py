@@ -48,6 +48,13 @@ LAUNCHES_PER_STEP = {"predictor": 1, "rhs": 1, "jacobi": 10, "correction": 1, "f
 # (read F and the two velocities; write F) is 16.
 FCT_PAIR_BYTES = 24
 FCT_PAIR_COMPULSORY_BYTES = 16
+# Fields each launch must read and write once (the compulsory stream mix):
+# a plain stream kernel moving the same mix (vof_stream_bandwidth, inputs from
+# HBM) gives the attainable time that launch is compared with.
+STREAM_MIX = {
+    "predictor": (4, 3), "rhs": (4, 1), "jacobi": (2, 1), "correction": (5, 3),
+    "fct_x": (2, 1), "fct_y": (2, 1), "fct_z": (2, 1), "fct_zx": (3, 1), "fct_xy": (3, 1),
+}
 
 
 def parse():
@@ -529,6 +536,19 @@ def main():
         if k in kern:
             kern[k]["sweeps_per_launch"] = 2
             kern[k]["compulsory_bytes_per_launch"] = FCT_PAIR_COMPULSORY_BYTES * (esz // 4) * cells
+    # attainable time of each launch's compulsory stream mix at this field size
+    field_bytes = (n + 2) ** 3 * esz
+    stream_us = {}
+    for k in kern:
+        mix = STREAM_MIX[k]
+        if mix not in stream_us:
+            stream_us[mix] = (vof.stream_time(field_bytes, *mix, resident=False),
+                              vof.stream_time(field_bytes, *mix, resident=True))
+        t_hbm, t_res = stream_us[mix]
+        kern[k]["stream_mix"] = f"{mix[0]} read, {mix[1]} written"
+        kern[k]["attainable_us"] = t_hbm
+        kern[k]["attainable_us_resident"] = t_res
+        kern[k]["frac_attainable"] = t_hbm / (kern[k]["ms_per_launch"] * 1e3)
     dom = max(kern, key=lambda k: per_step[k])
     d = kern[dom]
     traffic, traffic_source = (load_traffic(dom, tuning) if (n == 200 and prec == 4 and a.case == "dam")
@@ -579,6 +599,10 @@ def main():
             "ms_per_launch": d["ms_per_launch"],
             "sweeps_per_launch": d.get("sweeps_per_launch"),
             "attainable_copy_gbs": {"hbm_1GiB": round(copy_hbm, 1), "mall_64MiB": round(copy_mall, 1)},
+            "attainable": {"stream_mix": d["stream_mix"], "us_hbm": d["attainable_us"],
+                           "us_resident": d["attainable_us_resident"], "frac": d["frac_attainable"],
+                           "source": "vof_stream_bandwidth: plain 16 B/lane stream kernel over the same number of "
+                                     "fields of the same size, inputs from HBM (us_resident: one set repeated)"},
         },
         "tuning": tuning,
         "step_hbm_frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,

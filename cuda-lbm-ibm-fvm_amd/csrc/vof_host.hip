@@ -1761,6 +1761,58 @@ vof_status vof_copy_bandwidth(int64_t bytes, int reps, double* gbs) {
     return rc;
 }
 
+vof_status vof_stream_bandwidth(int64_t field_bytes, int nread, int nwrite, int resident, int reps,
+                                double* us_per_pass) {
+    if (field_bytes < (1 << 20) || field_bytes > (int64_t(1) << 32) || nread < 1 || nread > 6 || nwrite < 1 ||
+        nwrite > 3 || reps < 1 || !us_per_pass) {
+        set_err("stream_bandwidth: bad argument");
+        return VOF_ERR_INVALID;
+    }
+    const int64_t pitch16 = (field_bytes + 255) / 256 * 16;  // 16-byte elements per array, 256-byte aligned
+    const long n16 = long(field_bytes / 16);
+    const int64_t set_bytes = int64_t(nread + nwrite) * pitch16 * 16;
+    // Unless `resident`, passes rotate over enough array sets that a pass's
+    // arrays were last touched 1 GiB (four times the MALL) of traffic earlier:
+    // its reads come from HBM, as a step kernel's inputs do.
+    const int sets = resident ? 1 : int(((int64_t(1) << 30) + set_bytes - 1) / set_bytes) + 1;
+    char *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    vof_status rc = VOF_OK;
+    hipError_t e = hipSuccess;
+    const size_t abytes = size_t(sets) * nread * pitch16 * 16, bbytes = size_t(sets) * nwrite * pitch16 * 16;
+    auto pass = [&](int q) {
+        return vof::launch_stream(nread, nwrite, a + size_t(q) * nread * pitch16 * 16,
+                                  b + size_t(q) * nwrite * pitch16 * 16, n16, long(pitch16), st);
+    };
+    if ((e = hipMalloc(&a, abytes)) != hipSuccess || (e = hipMalloc(&b, bbytes)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess ||
+        (e = hipMemsetAsync(a, 0, abytes, st)) != hipSuccess) {
+        set_err("HIP failure in stream_bandwidth: %s", hipGetErrorString(e));
+        rc = VOF_ERR_HIP;
+    } else {
+        for (int q = 0; q < sets; ++q) (void)pass(q);  // warm-up: every set once
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < reps; ++r) (void)pass(r % sets);
+        (void)hipEventRecord(e1, st);
+        if ((e = hipEventSynchronize(e1)) != hipSuccess || (e = hipGetLastError()) != hipSuccess) {
+            set_err("HIP failure in stream_bandwidth: %s", hipGetErrorString(e));
+            rc = VOF_ERR_HIP;
+        } else {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            *us_per_pass = double(ms) * 1e3 / reps;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+}
+
 vof_status vof_write_vtk_host(const char* path, const void* data, int precision, int nxTot, int nyTot,
                               int nzTot) {
     if (!path || !data || nxTot < 2 || nyTot < 2 || nzTot < 2 || (precision != 4 && precision != 8)) {

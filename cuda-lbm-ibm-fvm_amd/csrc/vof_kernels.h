@@ -49,6 +49,11 @@ constexpr int kJ2DmaKz = 25;
 constexpr int kNoAllocMaxBlocks = 65536;
 
 void launch_copy(const void* a, void* b, long n16, hipStream_t st);
+// Work-groups of the stream-mix kernel (grid-stride over the arrays).
+constexpr int kStreamBlocks = 8192;
+// nread in 1..6 arrays read, nwrite in 1..3 written, n16 16-byte elements each,
+// consecutive arrays pitch elements apart in a and in b; false for another mix.
+bool launch_stream(int nread, int nwrite, const void* a, void* b, long n16, long pitch, hipStream_t st);
 template <class R> void launch_predict(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
 template <class R> void launch_predict_rho(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
 template <class R> void launch_rhs(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);

@@ -2030,6 +2030,47 @@ void launch_copy(const void* a, void* b, long n16, hipStream_t st) {
                        n16);
 }
 
+// Stream mix: NR arrays read and NW arrays written, n16 16-byte elements
+// each, consecutive arrays `pitch` elements apart; every written element
+// depends on the elements read at its index, so nothing is elided.  The
+// attainable time of a step kernel's compulsory traffic (its fields read and
+// written once each, with no arithmetic to hide).
+template <int NR, int NW>
+__global__ __launch_bounds__(256) void k_stream(const float4* __restrict__ a, float4* __restrict__ b, long n16,
+                                                long pitch) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n16; t += stride) {
+        float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const float4 x = a[r * pitch + t];
+            s.x += x.x;
+            s.y += x.y;
+            s.z += x.z;
+            s.w += x.w;
+        }
+#pragma unroll
+        for (int q = 0; q < NW; ++q) b[q * pitch + t] = s;
+    }
+}
+
+bool launch_stream(int nread, int nwrite, const void* a, void* b, long n16, long pitch, hipStream_t st) {
+    const dim3 grid(kStreamBlocks), blk(256);
+    const auto* x = (const float4*)a;
+    auto* y = (float4*)b;
+#define VOF_STREAM_CASE(R, W) \
+    case (R) * 8 + (W): hipLaunchKernelGGL((k_stream<R, W>), grid, blk, 0, st, x, y, n16, pitch); return true;
+    switch (nread * 8 + nwrite) {
+        VOF_STREAM_CASE(1, 1) VOF_STREAM_CASE(2, 1) VOF_STREAM_CASE(3, 1) VOF_STREAM_CASE(4, 1)
+        VOF_STREAM_CASE(5, 1) VOF_STREAM_CASE(6, 1) VOF_STREAM_CASE(1, 2) VOF_STREAM_CASE(2, 2)
+        VOF_STREAM_CASE(3, 2) VOF_STREAM_CASE(4, 2) VOF_STREAM_CASE(5, 2) VOF_STREAM_CASE(6, 2)
+        VOF_STREAM_CASE(1, 3) VOF_STREAM_CASE(2, 3) VOF_STREAM_CASE(3, 3) VOF_STREAM_CASE(4, 3)
+        VOF_STREAM_CASE(5, 3) VOF_STREAM_CASE(6, 3)
+        default: return false;
+    }
+#undef VOF_STREAM_CASE
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers.
 template <class R>

@@ -133,6 +133,7 @@ def load_library(path: str = LIB_PATH):
         "vof_write_vtk_host": (c_int, [c_char_p, c_void_p, c_int, c_int, c_int, c_int]),
         "vof_debug_vtk_thread_limit": (None, [c_int]),
         "vof_copy_bandwidth": (c_int, [c_int64, c_int, POINTER(c_double)]),
+        "vof_stream_bandwidth": (c_int, [c_int64, c_int, c_int, c_int, c_int, POINTER(c_double)]),
         "vof_sync": (c_int, [S]),
         "vof_time_steps": (c_int, [S, c_int64, c_int, c_int, c_int, POINTER(c_double), POINTER(c_double)]),
         "vof_set_tuning": (c_int, [S, POINTER(VofTuning)]),
@@ -419,6 +420,17 @@ def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20) -> float:
     g = c_double()
     _check(lib.vof_copy_bandwidth(int(nbytes), int(reps), byref(g)))
     return g.value
+
+
+def stream_time(field_bytes: int, nread: int, nwrite: int, resident: bool = False, reps: int = 20) -> float:
+    """Microseconds one pass of a plain stream kernel needs to read `nread` and
+    write `nwrite` arrays of `field_bytes` bytes each (inputs from HBM unless
+    `resident`): the attainable time of a step kernel's compulsory traffic."""
+    lib = load_library()
+    us = c_double()
+    _check(lib.vof_stream_bandwidth(int(field_bytes), int(nread), int(nwrite), int(bool(resident)), int(reps),
+                                    byref(us)))
+    return us.value
 
 
 def write_vtk_host(path: str, field: np.ndarray) -> None:

@@ -210,6 +210,20 @@ void vof_debug_vtk_thread_limit(int n);
  * bytes repeated `reps` times; *gbs = (read + written bytes) / time. */
 vof_status vof_copy_bandwidth(int64_t bytes, int reps, double* gbs);
 
+/* Attainable time of a stream mix (extension, a measurement with no
+ * counterpart in the reference): a plain kernel reads `nread` arrays (1..6)
+ * and writes `nwrite` arrays (1..3) of `field_bytes` bytes each (1 MiB ..
+ * 4 GiB), 16 bytes per lane, every written element formed from the elements
+ * read at its index.  *us_per_pass receives the average microseconds of one
+ * pass over `reps` passes.  resident = 0 rotates the passes over enough array
+ * sets that each pass's inputs come from HBM (as a step kernel's inputs
+ * mostly do); resident = 1 repeats one set (inputs cached when they fit).
+ * With a step kernel's compulsory fields (e.g. 4 read, 3 written for the
+ * predictor) it is the time that kernel's traffic needs with no arithmetic or
+ * stencil to hide.  Allocates and frees its arrays. */
+vof_status vof_stream_bandwidth(int64_t field_bytes, int nread, int nwrite, int resident, int reps,
+                                double* us_per_pass);
+
 /* Synchronise the handle's stream. */
 vof_status vof_sync(vof_solver* s);
 

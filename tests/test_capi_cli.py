@@ -56,6 +56,14 @@ def test_invalid_parameters_rejected_before_device_use(vofmod, bad):
     assert e.value.status == 1
 
 
+@pytest.mark.parametrize("args", [(1 << 19, 2, 1), (1 << 20, 0, 1), (1 << 20, 7, 1), (1 << 20, 2, 0),
+                                  (1 << 20, 2, 4), ((1 << 32) + 16, 2, 1)])
+def test_stream_measurement_refuses_bad_mixes_before_device_use(vofmod, args):
+    with pytest.raises(vofmod.VofError) as e:
+        vofmod.stream_time(*args)
+    assert e.value.status == 1
+
+
 def test_undefined_flags_are_reported(vofmod):
     # every defined flag is built; a bit outside them is refused before device use
     with pytest.raises(vofmod.VofError) as e:

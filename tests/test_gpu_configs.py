@@ -186,3 +186,18 @@ def test_cli_default_case_writes_the_oracle_vtk(vofmod, orc, tmp_path):
     orc.write_vtk_float(str(ref), o.get("F"))
     o.close()
     assert (out / "step-00010.vtk").read_bytes() == ref.read_bytes()
+
+
+def test_stream_mix_measurement(vofmod):
+    """The attainable-time measurement the bench compares each launch with:
+    plain passes over 3 and 7 fields of the 200^3 FP32 size take between the
+    8 TB/s peak's time (16 TB/s for cache-resident inputs) and 1 TB/s's, and
+    more fields take longer."""
+    print("stream-mix measurement at 202^3 x 4 B", flush=True)
+    fb = 202 ** 3 * 4
+    t21 = vofmod.stream_time(fb, 2, 1, reps=10)
+    t43 = vofmod.stream_time(fb, 4, 3, reps=10)
+    t43r = vofmod.stream_time(fb, 4, 3, resident=True, reps=10)
+    for t, narr, peak in ((t21, 3, 8e12), (t43, 7, 8e12), (t43r, 7, 16e12)):
+        assert narr * fb / peak * 1e6 <= t <= narr * fb / 1e12 * 1e6, (t, narr)
+    assert t43 > t21
