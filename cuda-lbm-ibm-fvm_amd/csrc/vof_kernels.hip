@@ -873,10 +873,22 @@ __global__ __launch_bounds__(1024) void k_jacobi2_dma(const EnsArg<float> ea, in
     wait_vm<3 * A>();  // groups 1 .. A may stay in flight
     lds_barrier();
     const R q0m0 = P[0][w + 1][lane], q00 = P[1][w + 1][lane];  // p column at k0-2, k0-1
-    auto march = [&](auto in_tag) __attribute__((always_inline)) {
-        constexpr bool IN = decltype(in_tag)::value;
-        // plane offsets of the next requests (unclamped in an interior chunk)
-        R q0m = q0m0, q0 = q00;           // p column at s-1, s
+    // Chunk kinds: 0 every plane the march forms is interior; 1 the first
+    // chunk (k0 = 1), 2 the last one (ending at nz): their two boundary planes
+    // (the ghost plane 0 and plane 1, or plane nz and the ghost plane nz+1)
+    // sit at fixed steps, so kinds 0-2 share one march whose code differs
+    // only at those steps, by a wave-uniform branch on the kind (one copy of
+    // the march: the instruction cache a CU pair shares holds the code of all
+    // the work-groups it runs).  3: any other chunk (short grids), with the
+    // plane tested per step.
+    constexpr int kInterior = 0, kFirst = 1, kLast = 2, kAny = 3;
+    const int kind = interior ? kInterior
+                     : (k0 == 1 && nz >= KZ + 2)          ? kFirst   // planes 2 .. KZ+1 <= nz-1 interior
+                     : (k0 + KZ - 1 == nz && k0 >= 3)     ? kLast    // planes k0-1 >= 2 .. nz-1 interior
+                                                          : kAny;
+    auto march = [&](auto any_tag) __attribute__((always_inline)) {
+        constexpr bool ANY = decltype(any_tag)::value;
+        R q0m = q0m0, q0 = q00;                // p column at s-1, s
         R q1mm = R(0), q1m = R(0), bm = R(0);  // sweep 1 at s-2, s-1; rhs at s-1
 #pragma unroll
         for (int e = 0; e <= KZ + 1; ++e) {
@@ -887,23 +899,34 @@ __global__ __launch_bounds__(1024) void k_jacobi2_dma(const EnsArg<float> ea, in
             const R b = B[e % NS][w][lane];
             const R q0w = sh_up(q0), q0e = sh_dn(q0);
             R q1;
-            if (IN || (s >= 2 && s <= nz - 1)) {
-                const R q = apk.div_in_wave(jac_num(b, ae, aw, an, as, azz, azz, q0e, q0w, pjp, pjm, q0p, q0m), nd1);
-                q1 = in1 ? q : R(0);
-            } else {
+            if (ANY) {
                 q1 = R(0);
                 if (in1 && s >= 1 && s <= nz) {
                     const R af = (s != nz) ? azz : R(0), ab = (s != 1) ? azz : R(0);
                     q1 = apk.div(jac_num(b, ae, aw, an, as, af, ab, q0e, q0w, pjp, pjm, q0p, q0m), s, nz);
                 }
+            } else if ((e == 0 && kind == kFirst) || (e == KZ + 1 && kind == kLast)) {
+                q1 = R(0);  // ghost plane 0 / nz+1
+            } else if (e == 1 && kind == kFirst) {  // plane 1: no bottom neighbour (:428-429)
+                const R q = jac_num(b, ae, aw, an, as, azz, R(0), q0e, q0w, pjp, pjm, q0p, q0m) / apk.lo;
+                q1 = in1 ? q : R(0);
+            } else if (e == KZ && kind == kLast) {  // plane nz: no top neighbour
+                const R q = jac_num(b, ae, aw, an, as, R(0), azz, q0e, q0w, pjp, pjm, q0p, q0m) / apk.hi;
+                q1 = in1 ? q : R(0);
+            } else {
+                const R q = apk.div_in_wave(jac_num(b, ae, aw, an, as, azz, azz, q0e, q0w, pjp, pjm, q0p, q0m), nd1);
+                q1 = in1 ? q : R(0);
             }
             Q[e % 3][w][lane] = q1;
             // request group e+A+1 (p plane k0+e+A+1, rhs plane k0+e+A), then
-            // wait for group e+1, the next step's
+            // wait for group e+1, the next step's.  Plane offsets are formed
+            // unclamped where they lie in 0 .. nz+1 (kinds 0 and 1; the last
+            // chunk's final p plane would be nz+2).
             if (e + A + 1 <= KZ + 1) {
                 const int pp = k0 + e + A + 1, pb = pp - 1;
-                req_p((e + A + 3) % NS, IN ? pp * sy : po(pp));
-                req_b((e + A + 1) % NS, IN ? pb * sy : po(pb));
+                const bool clamp = ANY || (e + A + 1 == KZ + 1 && kind == kLast);
+                req_p((e + A + 3) % NS, clamp ? po(pp) : pp * sy);
+                req_b((e + A + 1) % NS, clamp ? po(pb) : pb * sy);
             }
             if (e <= KZ) wait_groups(min(A, KZ - e));
             lds_barrier();
@@ -914,11 +937,19 @@ __global__ __launch_bounds__(1024) void k_jacobi2_dma(const EnsArg<float> ea, in
                 const R q1s = Q[(e + 2) % 3][w == 0 ? 0 : w - 1][lane];
                 const R q1n = Q[(e + 2) % 3][w == kJ2Rows - 1 ? w : w + 1][lane];
                 R q;
-                if (IN || (s2 >= 2 && s2 <= nz - 1)) {
-                    q = apk.div_in_wave(jac_num(bm, ae, aw, an, as, azz, azz, q1e, q1w, q1n, q1s, q1, q1mm), nd2);
+                if (ANY) {
+                    if (s2 >= 2 && s2 <= nz - 1) {
+                        q = apk.div_in_wave(jac_num(bm, ae, aw, an, as, azz, azz, q1e, q1w, q1n, q1s, q1, q1mm), nd2);
+                    } else {
+                        const R af = (s2 != nz) ? azz : R(0), ab = (s2 != 1) ? azz : R(0);
+                        q = apk.div(jac_num(bm, ae, aw, an, as, af, ab, q1e, q1w, q1n, q1s, q1, q1mm), s2, nz);
+                    }
+                } else if (e == 2 && kind == kFirst) {  // plane 1
+                    q = jac_num(bm, ae, aw, an, as, azz, R(0), q1e, q1w, q1n, q1s, q1, q1mm) / apk.lo;
+                } else if (e == KZ + 1 && kind == kLast) {  // plane nz
+                    q = jac_num(bm, ae, aw, an, as, R(0), azz, q1e, q1w, q1n, q1s, q1, q1mm) / apk.hi;
                 } else {
-                    const R af = (s2 != nz) ? azz : R(0), ab = (s2 != 1) ? azz : R(0);
-                    q = apk.div(jac_num(bm, ae, aw, an, as, af, ab, q1e, q1w, q1n, q1s, q1, q1mm), s2, nz);
+                    q = apk.div_in_wave(jac_num(bm, ae, aw, an, as, azz, azz, q1e, q1w, q1n, q1s, q1, q1mm), nd2);
                 }
                 if (st2) pt.st(q, tc, s2 * sy);
             }
@@ -929,8 +960,8 @@ __global__ __launch_bounds__(1024) void k_jacobi2_dma(const EnsArg<float> ea, in
             bm = b;
         }
     };
-    if (interior) march(std::true_type{});
-    else march(std::false_type{});
+    if (kind != kAny) march(std::false_type{});
+    else march(std::true_type{});
 }
 
 // ---------------------------------------------------------------------------

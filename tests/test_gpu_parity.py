@@ -290,6 +290,16 @@ def _jacobi2_pair(vofmod, orc, kz, prec, nx=131, ny=33, nz=21):
     o.close()
 
 
+@pytest.mark.parametrize("nz", [25, 26, 27, 51, 60])
+def test_jacobi2_streamed_chunk_kinds(vofmod, orc, nz):
+    """The FP32 two-sweep kernel streamed through LDS (25-plane chunks) in each
+    of its marches: interior chunks, the first chunk (k0 = 1) and the last one
+    (ending at nz) with their boundary planes at fixed steps, and the chunks
+    that fit neither (nz 25 and 26; the chunk k0 = 26 of nz 51, which ends at
+    nz - 1); two x tiles and three row bands."""
+    _jacobi2_pair(vofmod, orc, 25, 4, nx=70, ny=30, nz=nz)
+
+
 @pytest.mark.parametrize("prec", [4, 8])
 @pytest.mark.parametrize("kz", [1, 2, 3, 4, 5, 8, 16, 25])
 def test_jacobi2_tiles_and_chunks(vofmod, orc, kz, prec):
