@@ -87,12 +87,22 @@ def parse():
     return ap.parse_args()
 
 
+# Launch-shape fields that change what each kernel moves (the others only
+# change other kernels' launches).
+TUNE_FIELDS = {
+    "predictor": ("kz_predict",), "rhs": ("kz_rhs",), "jacobi": ("kz_jacobi", "jacobi_tblock"),
+    "correction": ("kz_correct",), "fct_x": ("kz_fct_x",), "fct_y": ("fct_chunk",), "fct_z": ("fct_chunk",),
+    "fct_zx": ("kz_fct_x", "fct_zx_fused"), "fct_xy": ("fct_chunk", "fct_xy_fused"),
+}
+
+
 def load_traffic(kernel, tuning):
     """HBM bytes per launch of `kernel` from the committed counter summary
     (profiles/traffic_200.json, written from rocprofv3 FETCH_SIZE / WRITE_SIZE
     runs scaled by a copy kernel of known size) and a label naming that file
     and the launch shapes it was profiled with; (None, reason) when the file
-    is absent or was profiled with other launch shapes than `tuning`."""
+    is absent or the kernel was profiled with other launch shapes than this
+    run's (only the fields that change that kernel's launches are compared)."""
     path = os.path.join(ROOT, "profiles", "traffic_200.json")
     try:
         with open(path) as fh:
@@ -101,9 +111,11 @@ def load_traffic(kernel, tuning):
         prof = t.get("tuning", {})
     except (OSError, KeyError, ValueError):
         return None, "no counter summary for this kernel"
-    if any(tuning.get(k) != v for k, v in prof.items()) or not prof:
+    fields = TUNE_FIELDS.get(kernel, tuple(prof))
+    if not prof or any(tuning.get(k) != prof.get(k) for k in fields):
         return None, f"profiles/traffic_200.json was profiled with tuning {prof}, this run uses {tuning}"
-    return byts, f"rocprofv3 FETCH_SIZE + WRITE_SIZE from profiles/traffic_200.json, tuning {prof}"
+    same = {k: prof[k] for k in fields}
+    return byts, f"rocprofv3 FETCH_SIZE + WRITE_SIZE from profiles/traffic_200.json, {kernel} launch shapes {same}"
 
 
 from vof_cases import SEED, ensemble_config, field_errors, ripple_F  # noqa: E402
