@@ -35,6 +35,11 @@ template <class R> void adj_correct(const Inst<R>* tab, const Inst<R>& h, const 
 // 0 arhs untouched, 1 arhs (+)= input h, 2 arhs (+)= input h + output h.
 template <class R> void adj_jacobi(const Inst<R>* tab, const Inst<R>& h, const double* in, double* out,
                                    double* arhs, int in_g, int out_g, int first, int acc, hipStream_t st);
+// Two such sweeps in one launch: arhs (+)= input h + the first sweep's h (the
+// two sweep inputs), out = the second sweep's h (gbar when out_g); the same
+// bits as two adj_jacobi launches with acc 2 and 0.
+template <class R> void adj_jacobi2(const Inst<R>* tab, const Inst<R>& h, const double* in, double* out,
+                                    double* arhs, int in_g, int out_g, int first, hipStream_t st);
 template <class R> void adj_rhs(const Inst<R>* tab, const Inst<R>& h, const R* F, const R* us, const R* vs,
                                 const R* ws, const double* arhs, double* aus, double* avs, double* aws, double* arho,
                                 hipStream_t st);

@@ -721,6 +721,111 @@ __global__ __launch_bounds__(256) void k_jacobi_adj_col(const Inst<R>* __restric
     }
 }
 
+// Two of those reverse sweeps per launch (temporal blocking, as the forward's
+// two-sweep kernels): sweep a takes the input h (gbar divided by ap on load
+// when in_g) to h1 = D^-1 (-N h), sweep b takes h1 to the output (-N h1, or
+// its D^-1 multiple unless out_g).  rhs_bar takes h + h1 of every cell (the
+// two sweep inputs, added in that order, assigned when first), so a pair
+// launch leaves out and rhs_bar bit for bit as two single-sweep launches of
+// k_jacobi_adj_col (acc 2 then 0) do: every sum keeps that kernel's order.
+// Tile and march as the forward's k_jacobi2: a work-group of 16 waves covers
+// 16 rows x 64 lanes, wave w row j = 14*band + w, lane l column
+// i = 60*xtile - 1 + l; sweep a is formed on every lane of every row and
+// shared across rows through a 3-plane LDS ring, sweep b is stored on lanes
+// 2..61 of rows 1..14, one plane behind sweep a; z neighbours of both sweeps
+// in registers, one plane of loads in flight.  Every wave reaches every
+// barrier: the chunk bounds are uniform over the work-group.
+template <class R>
+__global__ __launch_bounds__(1024) void k_jacobi2_adj(const Inst<R>* __restrict__ tab,
+                                                      const double* __restrict__ in, double* __restrict__ out,
+                                                      double* __restrict__ arhs, int in_g, int out_g, int first,
+                                                      int kz, double ri_lo, double ri_in, double ri_hi) {
+    const Inst<R> I = tab[0];
+    __shared__ double ring[3][16][64];
+    const int nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
+    const int nxt = (nx + 59) / 60, nbd = (ny + 13) / 14;
+    const int per_z = nxt * nbd;
+    const int b = int(blockIdx.x);
+    const int zc = b / per_z, rem = b - zc * per_z;
+    const int bd = rem / nxt, xt = rem - bd * nxt;
+    const int lane = int(threadIdx.x) & 63, w = int(threadIdx.x) >> 6;
+    const int i = 60 * xt - 1 + lane, j = 14 * bd + w;
+    const int k0 = 1 + zc * kz, k1 = min(nz, k0 + kz - 1);
+    const bool in1 = (i >= 1 && i <= nx && j >= 1 && j <= ny);
+    const bool st2 = in1 && lane >= 2 && lane <= 61 && w >= 1 && w <= 14;
+    const int ic = min(max(i, 0), nx + 1), jc = min(max(j, 0), ny + 1);
+    const int tc = jc * sx + ic;
+    const int tjm = max(jc - 1, 0) * sx + ic, tjp = min(jc + 1, ny + 1) * sx + ic;
+    const double ax = I.axx, ay = I.ayy, az = I.azz;
+    // 1/ap of rows j-1, j, j+1 (index 0, 1, 2) for k = 1, interior planes, k = nz,
+    // as k_jacobi_adj_col forms them (the side rows matter only when in_g)
+    double rlo[3] = {ri_lo, ri_lo, ri_lo}, rin[3] = {ri_in, ri_in, ri_in}, rhi[3] = {ri_hi, ri_hi, ri_hi};
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const int jr = min(max(j - 1 + r, 0), ny + 1);
+        if ((r != 1 && !in_g) || (ic >= 2 && ic <= nx - 1 && jr >= 2 && jr <= ny - 1)) continue;
+        rlo[r] = 1.0 / jac_ap(I, ic, jr, 1);
+        rhi[r] = 1.0 / jac_ap(I, ic, jr, nz);
+        rin[r] = (nz > 2) ? 1.0 / jac_ap(I, ic, jr, 2) : 0.0;
+    }
+    auto rk = [&](int r, int k) { return (k == 1) ? rlo[r] : ((k == nz) ? rhi[r] : rin[r]); };
+    auto as_h = [&](double x, int r, int k) { return in_g ? x * rk(r, k) : x; };
+    const int olast = (nz + 1) * sy;
+    auto po = [&](int s) { return min(max(s, 0) * sy, olast); };  // clamped plane offset
+    const Buf<double> hb(in, I.ntot);
+    // march: sweep a at plane s, sweep b at plane s-1, for s = k0-1 .. k1+1
+    const int s_beg = k0 - 1, s_end = k1 + 1;
+    double h0m = as_h(hb.ld(tc, po(s_beg - 1)), 1, s_beg - 1), h0 = as_h(hb.ld(tc, po(s_beg)), 1, s_beg);
+    double h1mm = 0.0, h1m = 0.0;  // sweep a at s-2, s-1
+    double nq = hb.ld(tc, po(s_beg + 1)), njm = hb.ld(tjm, po(s_beg)), njp = hb.ld(tjp, po(s_beg));
+    for (int s = s_beg; s <= s_end; ++s) {
+        const double h0p = as_h(nq, 1, s + 1), h0jm = as_h(njm, 0, s), h0jp = as_h(njp, 2, s);
+        nq = hb.ld(tc, po(s + 2));
+        njm = hb.ld(tjm, po(s + 1));
+        njp = hb.ld(tjp, po(s + 1));
+        // sweep a at plane s
+        const double hw = lane_up(h0), he = lane_dn(h0);
+        double h1 = 0.0;
+        if (in1 && s >= 1 && s <= nz) {
+            double sm = 0.0;
+            if (i < nx) sm -= ax * he;
+            if (i > 1) sm -= ax * hw;
+            if (j < ny) sm -= ay * h0jp;
+            if (j > 1) sm -= ay * h0jm;
+            if (s < nz) sm -= az * h0p;
+            if (s > 1) sm -= az * h0m;
+            h1 = sm * rk(1, s);
+        }
+        ring[s % 3][w][lane] = h1;
+        __syncthreads();
+        // sweep b at plane s-1
+        const int s2 = s - 1;
+        const double h1w = lane_up(h1m), h1e = lane_dn(h1m);
+        if (st2 && s2 >= k0) {
+            const int ps = (s + 2) % 3;
+            const double h1s = ring[ps][w - 1][lane], h1n = ring[ps][w + 1][lane];
+            double sm = 0.0;
+            if (i < nx) sm -= ax * h1e;
+            if (i > 1) sm -= ax * h1w;
+            if (j < ny) sm -= ay * h1n;
+            if (j > 1) sm -= ay * h1s;
+            if (s2 < nz) sm -= az * h1;
+            if (s2 > 1) sm -= az * h1mm;
+            const int c = s2 * sy + tc;
+            out[c] = out_g ? sm : sm * rk(1, s2);
+            const double add = h0m + h1m;  // the two sweep inputs at plane s-1
+            if (first) arhs[c] = add;
+            else arhs[c] += add;
+        }
+        // the slot written at step s+3 was last read at step s+1, before the
+        // barrier of step s+2 that every wave passes first
+        h0m = h0;
+        h0 = h0p;
+        h1mm = h1m;
+        h1m = h1;
+    }
+}
+
 // Transpose of rhs = rho/dt * div(u*) on interior cells: each interior face
 // i >= 2 (j >= 2, k >= 2) gathers from its two cells; arho += arhs div / dt.
 template <class R>
@@ -1126,6 +1231,23 @@ void adj_jacobi(const Inst<R>* tab, const Inst<R>& h, const double* in, double* 
                        in_g, out_g, first, acc, kz, ri_lo, ri_in, ri_hi);
 }
 template <class R>
+void adj_jacobi2(const Inst<R>* tab, const Inst<R>& h, const double* in, double* out, double* arhs, int in_g,
+                 int out_g, int first, hipStream_t st) {
+    // z chunk: the longest of 16, 8, 4, 2 planes that still gives a work-group
+    // per CU (256); a chunk re-forms the first sweep on its two end planes, so
+    // shorter chunks cost more (at 96^3: 4-plane chunks and 336 work-groups,
+    // backward 132 ms; 2-plane chunks, 672 work-groups, 138 ms; 8-plane, 134)
+    const int per_z = ((h.nx + 59) / 60) * ((h.ny + 13) / 14);
+    int kz = 16;
+    while (kz > 2 && per_z * ((h.nz + kz - 1) / kz) < 256) kz /= 2;
+    const R ax = h.axx, ay = h.ayy, az = h.azz, zf = (h.nz != 1) ? h.azz : R(0);
+    const double ri_lo = 1.0 / double(-(ax + ax + ay + ay + zf + R(0)));
+    const double ri_hi = 1.0 / double(-(ax + ax + ay + ay + R(0) + zf));
+    const double ri_in = 1.0 / double(-(ax + ax + ay + ay + az + az));
+    hipLaunchKernelGGL(k_jacobi2_adj<R>, dim3(per_z * ((h.nz + kz - 1) / kz)), dim3(1024), 0, st, tab, in, out, arhs,
+                       in_g, out_g, first, kz, ri_lo, ri_in, ri_hi);
+}
+template <class R>
 void adj_rhs(const Inst<R>* tab, const Inst<R>& h, const R* F, const R* us, const R* vs, const R* ws,
              const double* arhs, double* aus, double* avs, double* aws, double* arho, hipStream_t st) {
     hipLaunchKernelGGL(k_rhs_adj<R>, grid_interior(h), dim3(256), 0, st, tab, F, us, vs, ws, arhs, aus, avs, aws,
@@ -1196,6 +1318,8 @@ template void copy_fields<double>(const FieldCopy<double>&, long, hipStream_t);
                                  double*, hipStream_t);                                                            \
     template void adj_jacobi<R>(const Inst<R>*, const Inst<R>&, const double*, double*, double*, int, int, int,   \
                                 int, hipStream_t);                                                                 \
+    template void adj_jacobi2<R>(const Inst<R>*, const Inst<R>&, const double*, double*, double*, int, int, int,  \
+                                 hipStream_t);                                                                     \
     template void adj_rhs<R>(const Inst<R>*, const Inst<R>&, const R*, const R*, const R*, const R*,              \
                              const double*, double*, double*, double*, double*, hipStream_t);                      \
     template void adj_predict<R>(const Inst<R>*, const Inst<R>&, const R*, const R*, const R*, const R*,          \

@@ -1181,17 +1181,19 @@ struct Ensemble : EnsBase {
         if (sweeps == 0) {
             VOF_HIP(hipMemcpyAsync(adj.ap[x], adj.ap[c], nt * sizeof(double), hipMemcpyDeviceToDevice, st));
         } else {
-            // one sweep per launch; the outputs alternate so that the last
-            // lands in ap[x]
-            const int nl = sweeps;
+            // two sweeps per launch (an odd count ends with one single
+            // sweep); the outputs alternate so that the last lands in ap[x]
+            const int nl = (sweeps + 1) / 2;
             const double* in = adj.ap[c];
             for (int l = 0; l < nl; ++l) {
                 double* out = ((nl - 1 - l) % 2 == 0) ? adj.ap[x] : adj.gt;
-                // rhs_bar = sum of every launch's input h: even launches take
-                // their input and output (the next launch's input; launch 0
-                // assigns), a last even launch only its input, odd ones nothing
-                const int acc = (l % 2 == 0) ? ((l + 1 < nl) ? 2 : 1) : 0;
-                vof::adj_jacobi(e.tab, I, in, out, adj.arhs, l == 0, l == nl - 1, l == 0, acc, st);
+                // rhs_bar = the sum of every sweep's input h: a pair launch adds
+                // its input and its first sweep's output (launch 0 assigns), a
+                // last single sweep its input only
+                if (2 * l + 1 < sweeps)
+                    vof::adj_jacobi2(e.tab, I, in, out, adj.arhs, l == 0, l == nl - 1, l == 0, st);
+                else
+                    vof::adj_jacobi(e.tab, I, in, out, adj.arhs, l == 0, 1, l == 0, 1, st);
                 in = out;
             }
             vof::adj_rhs(e.tab, I, SF, us, vs, ws, adj.arhs, adj.aus, adj.avs, adj.aws, adj.arho, st);

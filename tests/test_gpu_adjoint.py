@@ -100,6 +100,36 @@ def test_vjp_matches_taped_reference_fp64(vofmod, first, nsteps, every):
     assert sc["sigma"] == 0.0
 
 
+@pytest.mark.parametrize("sweeps", [3, 4, 7])
+def test_reverse_jacobi_pairs_on_several_tiles(vofmod, sweeps):
+    """The reverse Jacobi runs two sweeps per launch (a last single sweep when
+    the count is odd) on 64 x 16 tiles marching through z chunks: a grid of
+    two x tiles, three row bands and several chunks, odd and even sweep counts,
+    against the taped FP64 reference."""
+    nx, ny, nz = 70, 30, 21
+    dt, g = 2e-4, (0.5, -5.0, -9.81)
+    prm = vofmod.default_params(nx=nx, ny=ny, nz=nz, precision=8, dt=dt, g=g, jacobi_sweeps=sweeps)
+    st = random_state((nz + 2, ny + 2, nx + 2), 60 + sweeps, 0.8)
+    rng = np.random.default_rng(17 + sweeps)
+    seeds = {k: rng.normal(0, 1, st["F"].shape) for k in "FUVWP"}
+    print(f"reverse Jacobi pairs: {sweeps} sweeps on {nx}x{ny}x{nz}", flush=True)
+    J, sc, gg = gpu_vjp(vofmod, prm, st, 1, 2, 2, 0.7, seeds)
+    leaf = lambda x: torch.tensor(float(x), dtype=T, requires_grad=True)
+    P = ar.Prm(nx, ny, nz, dt=dt, g=tuple(leaf(x) for x in g), sweeps=sweeps)
+    P.rho_l, P.rho_g, P.nu_l, P.nu_g = leaf(1000.0), leaf(50.0), leaf(1e-6), leaf(1.5e-5)
+    x0 = [torch.tensor(st[k], dtype=T, requires_grad=True) for k in "FUVWP"]
+    s = tuple(x0)
+    coeffs = ar.jacobi_coeffs(P)
+    for m in range(2):
+        s = ar.step(P, s, 1 + m, coeffs)
+    L = 0.7 * ar.centroid_height(P, s[0]) + sum((torch.tensor(seeds[k]) * t).sum() for k, t in zip("FUVWP", s))
+    L.backward()
+    for k, t in zip("FUVWP", x0):
+        e = rel_err(gg[k], t.grad.numpy())
+        assert e < 1e-9, f"d/d{k}0: relative error {e:.2e}"
+    assert sc["rho_l"] == pytest.approx(P.rho_l.grad.item(), rel=1e-9, abs=1e-30)
+
+
 def centroid(vofmod, prm, st, first, nsteps):
     s = vofmod.Solver(prm)
     for k, a in st.items():
