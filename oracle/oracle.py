@@ -25,7 +25,7 @@ from vof import VofDiag, VofParams, default_params  # noqa: E402  (struct layout
 # Oracle array ids for the scratch arrays (raw access: 100 + id).
 RAW = {"F": 0, "FTD": 1, "AX": 2, "AY": 3, "AZ": 4, "CX": 5, "CY": 6, "CZ": 7, "RP": 8, "RM": 9,
        "U": 10, "V": 11, "W": 12, "USTAR": 13, "VSTAR": 14, "WSTAR": 15, "P": 16, "PT": 17, "RHO": 18, "NU": 19,
-       "FSM": 20, "KAPPA": 21}
+       "FSM": 20, "KAPPA": 21, "S1": 22, "S4": 23}
 FIELDS = {"F": 0, "U": 1, "V": 2, "W": 3, "P": 4, "USTAR": 5, "VSTAR": 6, "WSTAR": 7, "RHS": 8, "RHO": 9,
           "NU": 10, "KAPPA": 11}
 STAGES = {"nu_rho": 0, "advect_u": 1, "advect_v": 2, "advect_w": 3, "jacobi": 4, "update": 5, "bc_vel": 6,
