@@ -76,7 +76,9 @@ def parse():
     ap.add_argument("--stage-steps", type=int, default=30)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="run one plain copy of this many bytes first (byte-counter calibration)")
-    ap.add_argument("--jacobi-tblock", type=int, default=0)
+    ap.add_argument("--jacobi-tblock", type=int, default=0,
+                    help="Jacobi sweeps per launch (1 .. 5) before the autotuner (use with --no-autotune to keep it)")
+    ap.add_argument("--kz-jacobi", type=int, default=0, help="Jacobi z-chunk length, as --jacobi-tblock")
     ap.add_argument("--case", choices=["dam", "ripple"], default="dam",
                     help="initial state: the dam break of the metric (default), or the liquid layer with a rippled "
                          "surface (mostly liquid: every FCT wave does the limiter arithmetic), for profiling")
@@ -489,8 +491,8 @@ def main():
     esz = 8 if prec == 8 else 4
     cells = n ** 3
     s = make_solver(vof, n, prec, a.case)
-    if a.jacobi_tblock:
-        s.set_tuning(jacobi_tblock=a.jacobi_tblock)
+    if a.jacobi_tblock or a.kz_jacobi:
+        s.set_tuning(jacobi_tblock=a.jacobi_tblock, kz_jacobi=a.kz_jacobi)
     if not a.no_autotune:
         # launch shapes chosen by timing candidates on this GPU (state restored)
         s.autotune(1, 4, a.tune_cache)
@@ -514,12 +516,16 @@ def main():
     istep += st_steps
     per_step = {k: v / st_steps for k, v in stages.items()}
     tuning = s.get_tuning()
-    # With two sweeps per launch (jacobi_tblock 2, temporal blocking) a launch
-    # performs two sweeps of 12 algorithmic B/cell each: the "effective" figure
-    # of the roofline, beside the counter bytes (about half of it).
+    # With T sweeps per launch (jacobi_tblock T, temporal blocking: T-sweep
+    # launches while T sweeps are left, then pairs, then one) a launch performs
+    # on average sweeps / launches sweeps of 12 algorithmic B/cell each: the
+    # "effective" figure of the roofline, beside the counter bytes.
     launches = dict(LAUNCHES_PER_STEP)
-    sweeps_per_launch = 2 if tuning.get("jacobi_tblock", 1) >= 2 else 1
-    launches["jacobi"] = LAUNCHES_PER_STEP["jacobi"] // sweeps_per_launch
+    tb, sweeps = tuning.get("jacobi_tblock", 1), LAUNCHES_PER_STEP["jacobi"]
+    deep = sweeps // tb if tb >= 3 else 0
+    left = sweeps - deep * tb
+    launches["jacobi"] = deep + ((left // 2 + left % 2) if tb >= 2 else left)
+    sweeps_per_launch = sweeps / launches["jacobi"]
     stage_bytes = dict(STAGE_BYTES, fct_zx=FCT_PAIR_BYTES, fct_xy=FCT_PAIR_BYTES)
     # FCT launches of the three sweep orders (:1384-1396), with fused pairs
     zx = tuning.get("fct_zx_fused", 1) == 2 and tuning.get("kz_fct_x") in (4, 8, 16)

@@ -180,10 +180,6 @@ __device__ __forceinline__ R div_rn(R a, R b, R y, bool ok_b) {
 // The same quotient without divergent branches on the common path: the
 // corrected quotient is formed on every lane, and only when some lane of the
 // wave needs it (a wave-uniform test) do those lanes redo it as the IEEE
-// division.  Fewer exec-mask (scalar) instructions than div_rn.
-// The same quotient without divergent branches on the common path: the
-// corrected quotient is formed on every lane, and only when some lane of the
-// wave needs it (a wave-uniform test) do those lanes redo it as the IEEE
 // division.  The lane sets are wave masks formed once by the caller: need,
 // the lanes whose result is kept; must, those of them whose divisor fails
 // ok_b.  The range test is one compare per point whose lane mask is used as

@@ -185,7 +185,7 @@ struct Ensemble : EnsBase {
     std::vector<vof_params> prm;
     std::vector<vof::Inst<R>> ih;
     vof::Ens<R> e;
-    vof::Geo g_pred, g_rhs, g_jac, g_jac2, g_corr, g_fx, g_fy, g_fz, g_fzx, g_fxy, g_ghost, g_red, g_sub, g_cell;
+    vof::Geo g_pred, g_rhs, g_jac, g_jac2, g_jacT, g_corr, g_fx, g_fy, g_fz, g_fzx, g_fxy, g_ghost, g_red, g_sub, g_cell;
     vof_tuning tun{4, 4, 4, 4, 8, 1, 4, 2, 2};
     int fsel = 0, psel = 0;
     int sweeps = 10;
@@ -311,6 +311,20 @@ struct Ensemble : EnsBase {
         VOF_TRY(make_geo(g_jac, tun.kz_jacobi, col1));
         VOF_TRY(make_geo(g_jac2, tun.kz_jacobi, [](const vof::Inst<R>& I, int kz) {
             return vof::blocks_jacobi2(I.nx, I.ny, I.nz, kz); }));
+        if (tun.jacobi_tblock < 1 || tun.jacobi_tblock > vof::kJacobiMaxT) {
+            set_err("jacobi_tblock %d: sweeps per Jacobi launch must be 1 .. %d", tun.jacobi_tblock,
+                    vof::kJacobiMaxT);
+            return VOF_ERR_INVALID;
+        }
+        // the T-sweep tiles depend on T; built for T = 3 when fewer are fused (unused then)
+        switch (std::max(tun.jacobi_tblock, 3)) {
+            case 3: VOF_TRY(make_geo(g_jacT, tun.kz_jacobi, [](const vof::Inst<R>& I, int kz) {
+                        return vof::blocks_jacobiT(I.nx, I.ny, I.nz, kz, 3); })); break;
+            case 4: VOF_TRY(make_geo(g_jacT, tun.kz_jacobi, [](const vof::Inst<R>& I, int kz) {
+                        return vof::blocks_jacobiT(I.nx, I.ny, I.nz, kz, 4); })); break;
+            default: VOF_TRY(make_geo(g_jacT, tun.kz_jacobi, [](const vof::Inst<R>& I, int kz) {
+                        return vof::blocks_jacobiT(I.nx, I.ny, I.nz, kz, 5); })); break;
+        }
         VOF_TRY(make_geo(g_sub, 8, col1));
         VOF_TRY(make_geo(g_corr, tun.kz_correct, col1));
         VOF_TRY(make_geo(g_red, 8, [](const vof::Inst<R>& I, int kz) {
@@ -448,13 +462,22 @@ struct Ensemble : EnsBase {
         return VOF_OK;
     }
 
-    // ---- the Jacobi sweeps: pairs fused per launch when jacobi_tblock == 2,
-    // one launch per sweep otherwise; psel follows the launch ping-pong.
+    // ---- the Jacobi sweeps: T = jacobi_tblock sweeps per launch while at
+    // least T are left (T >= 3: the T-sweep tiles), then pairs (T >= 2), then
+    // single sweeps; psel follows the launch ping-pong.
     int jacobi_launches() const {
-        return (tun.jacobi_tblock >= 2) ? (sweeps / 2 + (sweeps & 1)) : sweeps;
+        const int T = tun.jacobi_tblock;
+        const int deep = (T >= 3) ? sweeps / T : 0, left = sweeps - deep * T;
+        return deep + ((T >= 2) ? (left / 2 + (left & 1)) : left);
     }
     void enqueue_jacobi() {
         int left = sweeps;
+        if (tun.jacobi_tblock >= 3) {
+            for (; left >= tun.jacobi_tblock; left -= tun.jacobi_tblock) {
+                if (!vof::launch_jacobiT(e, g_jacT, tun.jacobi_tblock, psel, st)) launch_failed = true;
+                psel ^= 1;
+            }
+        }
         if (tun.jacobi_tblock >= 2) {
             for (; left >= 2; left -= 2) {
                 vof::launch_jacobi2(e, g_jac2, psel, st);
@@ -620,6 +643,13 @@ struct Ensemble : EnsBase {
             case 9:
                 if (!(flags & VOF_FLAG_CSF)) { set_err("stage 9 (CSF) needs VOF_FLAG_CSF"); return VOF_ERR_INVALID; }
                 vof::launch_csf(e, g_sub, fsel, st);
+                break;
+            case 10:
+                if (!vof::launch_jacobiT(e, g_jacT, tun.jacobi_tblock, psel, st)) {
+                    set_err("stage 10 needs jacobi_tblock 3 .. %d (is %d)", vof::kJacobiMaxT, tun.jacobi_tblock);
+                    return VOF_ERR_INVALID;
+                }
+                psel ^= 1;
                 break;
             default: set_err("unknown stage %d", stage); return VOF_ERR_INVALID;
         }
@@ -851,6 +881,18 @@ struct Ensemble : EnsBase {
             double stage[VOF_NSTAGE_TIMES];
             rc = trial(t, stage);
             if (rc == VOF_OK && stage[2] < bestt[2]) { bestt[2] = stage[2]; best.kz_jacobi = c; best.jacobi_tblock = other_tb; }
+        }
+        // deeper blocks: T sweeps per launch (the remainder in pairs)
+        for (int T = 3; T <= vof::kJacobiMaxT && T <= sweeps; ++T) {
+            for (int c : {16, 25, 50}) {
+                if (rc != VOF_OK) break;
+                vof_tuning t = best;
+                t.jacobi_tblock = T;
+                t.kz_jacobi = c;
+                double stage[VOF_NSTAGE_TIMES];
+                rc = trial(t, stage);
+                if (rc == VOF_OK && stage[2] < bestt[2]) { bestt[2] = stage[2]; best.kz_jacobi = c; best.jacobi_tblock = T; }
+            }
         }
         // y/z march chunk
         const int chs[] = {4, 8, 16, 32};

@@ -44,6 +44,8 @@ constexpr int kNumReduce = 7;
 // (k_jacobi2_dma); any other length uses the preloaded or register-streamed
 // kernels.
 constexpr int kJ2DmaKz = 25;
+// Most Jacobi sweeps one launch performs (jacobi_tblock 1 .. kJacobiMaxT).
+constexpr int kJacobiMaxT = 5;
 // Largest k_reduce1 launch the allocation-free diagnostics serve: its
 // partials fill a static device array of this many work-groups x kNumReduce.
 constexpr int kNoAllocMaxBlocks = 65536;
@@ -59,6 +61,8 @@ template <class R> void launch_predict_rho(const Ens<R>& e, const Geo& g, int fs
 template <class R> void launch_rhs(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st);
 template <class R> void launch_jacobi(const Ens<R>& e, const Geo& g, int psrc, hipStream_t st);
 template <class R> void launch_jacobi2(const Ens<R>& e, const Geo& g, int psrc, hipStream_t st);
+// sweeps in 3 .. kJacobiMaxT sweeps in one launch; false for another count.
+template <class R> bool launch_jacobiT(const Ens<R>& e, const Geo& g, int sweeps, int psrc, hipStream_t st);
 template <class R> void launch_correct(const Ens<R>& e, const Geo& g, int fsel, int psel, hipStream_t st);
 template <class R> void launch_fct(const Ens<R>& e, const Geo& g, int dir, int src, int fin, hipStream_t st);
 // z sweep followed by the x sweep in one launch, chunks of g.kz in {4, 8, 16}
@@ -102,12 +106,18 @@ inline int blocks_xy(int nx, int ny, int nz, int ch) {
     const int wpp = (lines + 57) / 58;
     return ((((wpp + 3) / 4) + 7) & ~7) * ((ny + ch - 1) / ch);
 }
-// Two-sweep Jacobi tiles: 60 stored lanes x 14 stored rows per work-group
-// (1024 lanes), one work-group row per chunk of kz planes.
 // k_predict_rho: one lane per interior cell, 256 per work-group.
 inline int blocks_cells(int nx, int ny, int nz) { return int((int64_t(nx) * ny * nz + 255) / 256); }
+// Two-sweep Jacobi tiles: 60 stored lanes x 14 stored rows per work-group
+// (1024 lanes), one work-group row per chunk of kz planes.
 inline int blocks_jacobi2(int nx, int ny, int nz, int kz) {
     return ((nx + 59) / 60) * ((ny + 13) / 14) * ((nz + kz - 1) / kz);
+}
+// T-sweep Jacobi tiles (3 <= T <= kJacobiMaxT): 64 - 2T stored lanes x
+// 18 - 2T stored rows per work-group of 1024 lanes.
+inline int blocks_jacobiT(int nx, int ny, int nz, int kz, int T) {
+    const int ol = 64 - 2 * T, orow = 18 - 2 * T;
+    return ((nx + ol - 1) / ol) * ((ny + orow - 1) / orow) * ((nz + kz - 1) / kz);
 }
 // y/z line kernels: 256 lanes per work-group over the no*(nx+2) lines, the
 // count rounded up to a multiple of 8 so that under the round-robin XCD

@@ -16,8 +16,8 @@
 //  * most kernels have a register-blocked form for chunks of 4/8/16 planes or
 //    cells: every load of a chunk is issued before its first point is formed;
 //  * derived fields (rho, nu) are recomputed from F in registers;
-//  * the Jacobi right-hand side is formed once per step, and two sweeps can
-//    share a launch (LDS ring between the rows of a 16-row tile);
+//  * the Jacobi right-hand side is formed once per step, and two to five
+//    sweeps can share a launch (LDS ring between the rows of a 16-row tile);
 //  * each FCT direction is one fused kernel, the last one of a step also
 //    applies post_process_f and writes the F ghost layer;
 //  * work-groups go to XCDs in contiguous ranges (neighbouring tiles share an
@@ -590,6 +590,114 @@ __global__ __launch_bounds__(1024) void k_jacobi2(const EnsArg<R> ea,
             q0m = q0; q0 = q0p;
             q1mm = q1m; q1m = q1;
             bm = b;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// T Jacobi sweeps per launch, T = 3..5 (temporal blocking deeper than the
+// two-sweep kernels).  The tile is the two-sweep one with the halo of T
+// sweeps: 16 waves x 64 lanes, wave w on row j = R*band + w - T + 2, lane l on
+// i = L*xtile + l - T + 1, storing lanes T .. 63-T (L = 64 - 2T) of rows
+// T-1 .. 16-T (R = 18 - 2T).  While the tile marches along z (input plane s),
+// sweep t forms plane s - t + 1: its x neighbours by lane shifts, its own z
+// neighbours from the registers of sweep t-1 (the planes that sweep formed in
+// this step and the two before), its y neighbours from an LDS ring of sweep
+// t-1 (two plane slots per sweep: a slot written in one step is read in the
+// next, with one barrier per step between).  Sweep t is formed on rows
+// t-1 .. 16-t, those whose y neighbours sweep t-1 holds; points outside the
+// interior (ghost rows, lanes and planes) are zero, as the never-written
+// ghost p of the reference.  Each sweep keeps the reference's operation
+// order, so one launch equals T single sweeps bit for bit.
+template <int T>
+struct JacTile {
+    static constexpr int lanes = 64 - 2 * T, rows = 18 - 2 * T;
+};
+
+template <class R, int T>
+__global__ __launch_bounds__(1024) void k_jacobiT(const EnsArg<R> ea, int psrc, int kz) {
+    static_assert(T >= 3 && T <= kJacobiMaxT, "sweeps per launch");
+    VOF_PROLOGUE(R)
+    constexpr int OL = JacTile<T>::lanes, OR = JacTile<T>::rows;
+    __shared__ R ring[T - 1][2][kJ2Rows][64];
+    const int nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
+    const int nxt = (nx + OL - 1) / OL;
+    const int nbd = (ny + OR - 1) / OR;
+    const int per_z = nxt * nbd;
+    const int zc = lb / per_z, rem = lb - zc * per_z;
+    const int bd = rem / nxt, xt = rem - bd * nxt;
+    const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+    const int i = OL * xt + lane - T + 1;
+    const int j = OR * bd + w - T + 2;
+    const int k0 = 1 + zc * kz, k1 = min(nz, k0 + kz - 1);
+    const bool in1 = (i >= 1 && i <= nx && j >= 1 && j <= ny);
+    const bool stT = in1 && lane >= T && lane <= 63 - T && w >= T - 1 && w <= 16 - T;
+    const int ic = min(max(i, 0), nx + 1), jc = min(max(j, 0), ny + 1);
+    const int tc = jc * sx + ic;
+    const int tjm = max(jc - 1, 0) * sx + ic, tjp = min(jc + 1, ny + 1) * sx + ic;
+    const Buf<R> p(sel2(I.p, psrc), I.ntot), pt(sel2(I.p, psrc ^ 1), I.ntot), rhs(I.rhs, I.ntot);
+    const R ae = (i != nx) ? I.axx : R(0), aw = (i != 1) ? I.axx : R(0);
+    const R an = (j != ny) ? I.ayy : R(0), as = (j != 1) ? I.ayy : R(0);
+    const R azz = I.azz;
+    const JacAp<R> apk(ae, aw, an, as, azz, nz);
+    const int olast = (nz + 1) * sy;
+    auto po = [&](int s) { return min(max(s, 0) * sy, olast); };  // clamped plane offset
+    const int s_beg = k0 - T + 1, s_end = k1 + T - 1;
+    // input p: the column at s-1, s; plane s+1 of the column, rows j-1, j+1
+    // and rhs of plane s fetched one step ahead
+    R q0m = p.ld(tc, po(s_beg - 1)), q0 = p.ld(tc, po(s_beg));
+    R fq = p.ld(tc, po(s_beg + 1)), fjm = p.ld(tjm, po(s_beg)), fjp = p.ld(tjp, po(s_beg));
+    R fb = rhs.ld(tc, po(s_beg));
+    // sweep t (1 <= t < T): lc[t-1] its value at the plane it formed in the
+    // last step, lm[t-1] at the one before; bh[t-1] rhs of the plane sweep t
+    // forms in this step
+    R lc[T - 1], lm[T - 1], bh[T];
+#pragma unroll
+    for (int t = 0; t < T - 1; ++t) lc[t] = lm[t] = R(0);
+#pragma unroll
+    for (int t = 0; t < T; ++t) bh[t] = R(0);
+    for (int s = s_beg; s <= s_end; ++s) {
+        const R q0p = fq, q0jm = fjm, q0jp = fjp;
+#pragma unroll
+        for (int t = T - 1; t > 0; --t) bh[t] = bh[t - 1];
+        bh[0] = fb;
+        fq = p.ld(tc, po(s + 2));
+        fjm = p.ld(tjm, po(s + 1));
+        fjp = p.ld(tjp, po(s + 1));
+        fb = rhs.ld(tc, po(s + 1));
+        R cur[T];
+        {  // sweep 1 at plane s
+            const R qw = sh_up(q0), qe = sh_dn(q0);
+            R q = R(0);
+            if (in1 && s >= 1 && s <= nz) {
+                const R af = (s != nz) ? azz : R(0), ab = (s != 1) ? azz : R(0);
+                q = apk.div(jac_num(bh[0], ae, aw, an, as, af, ab, qe, qw, q0jp, q0jm, q0p, q0m), s, nz);
+            }
+            cur[0] = q;
+            ring[0][s & 1][w][lane] = q;
+        }
+#pragma unroll
+        for (int t = 2; t <= T; ++t) {  // sweep t at plane z = s - t + 1
+            const int z = s - t + 1;
+            const R c = lc[t - 2];  // sweep t-1 at plane z
+            const R cw = sh_up(c), ce = sh_dn(c);
+            R q = R(0);
+            if (w >= t - 1 && w <= 16 - t && in1 && z >= 1 && z <= nz) {
+                const R cs = ring[t - 2][z & 1][w - 1][lane], cn = ring[t - 2][z & 1][w + 1][lane];
+                const R af = (z != nz) ? azz : R(0), ab = (z != 1) ? azz : R(0);
+                q = apk.div(jac_num(bh[t - 1], ae, aw, an, as, af, ab, ce, cw, cn, cs, cur[t - 2], lm[t - 2]), z, nz);
+            }
+            cur[t - 1] = q;
+            if (t < T) ring[t - 1][z & 1][w][lane] = q;
+            else if (stT && z >= k0 && z <= k1) pt.st(q, tc, z * sy);
+        }
+        __syncthreads();
+        q0m = q0;
+        q0 = q0p;
+#pragma unroll
+        for (int t = 0; t < T - 1; ++t) {
+            lm[t] = lc[t];
+            lc[t] = cur[t];
         }
     }
 }
@@ -2163,6 +2271,16 @@ void launch_jacobi2(const Ens<R>& e, const Geo& g, int psrc, hipStream_t st) {
     hipLaunchKernelGGL((k_jacobi2<R, 1>), grid, blk, 0, st, ens_arg(e, g), psrc, g.kz);
 }
 template <class R>
+bool launch_jacobiT(const Ens<R>& e, const Geo& g, int sweeps, int psrc, hipStream_t st) {
+    const dim3 grid(g.nblk), blk(1024);
+    switch (sweeps) {
+        case 3: hipLaunchKernelGGL((k_jacobiT<R, 3>), grid, blk, 0, st, ens_arg(e, g), psrc, g.kz); return true;
+        case 4: hipLaunchKernelGGL((k_jacobiT<R, 4>), grid, blk, 0, st, ens_arg(e, g), psrc, g.kz); return true;
+        case 5: hipLaunchKernelGGL((k_jacobiT<R, 5>), grid, blk, 0, st, ens_arg(e, g), psrc, g.kz); return true;
+        default: return false;
+    }
+}
+template <class R>
 void launch_correct(const Ens<R>& e, const Geo& g, int fsel, int psel, hipStream_t st) {
     const dim3 grid(g.nblk), blk(256);
     switch (g.kz) {  // chunks of 4 or 8 planes: operands preloaded
@@ -2253,6 +2371,7 @@ void launch_reduce(const Ens<R>& e, const Geo& g, int fsel, int psel, int mode, 
     template void launch_rhs<R>(const Ens<R>&, const Geo&, int, hipStream_t);                            \
     template void launch_jacobi<R>(const Ens<R>&, const Geo&, int, hipStream_t);                         \
     template void launch_jacobi2<R>(const Ens<R>&, const Geo&, int, hipStream_t);                        \
+    template bool launch_jacobiT<R>(const Ens<R>&, const Geo&, int, int, hipStream_t);                   \
     template void launch_correct<R>(const Ens<R>&, const Geo&, int, int, hipStream_t);                   \
     template void launch_fct<R>(const Ens<R>&, const Geo&, int, int, int, hipStream_t);                  \
     template bool launch_fct_zx<R>(const Ens<R>&, const Geo&, int, int, hipStream_t);                    \

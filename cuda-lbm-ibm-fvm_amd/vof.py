@@ -36,7 +36,7 @@ STAGE_TIME_NAMES = ("predictor", "rhs", "jacobi", "p_mean", "correction", "fct_x
 
 # Stages for Solver.debug_stage (test hook).
 (STAGE_PREDICT, STAGE_RHS, STAGE_JACOBI, STAGE_CORRECT, STAGE_FCT_X, STAGE_FCT_Y, STAGE_FCT_Z, STAGE_GHOST_F,
- STAGE_JACOBI2, STAGE_CSF) = range(10)
+ STAGE_JACOBI2, STAGE_CSF, STAGE_JACOBI_T) = range(11)
 
 
 class VofParams(ctypes.Structure):

@@ -93,7 +93,7 @@ typedef struct vof_params {
     double dt;                  /* time step [s] */
     int32_t jacobi_sweeps;      /* Jacobi sweeps per step (reference: 10, voftrial1.cu:1371) */
     uint32_t flags;             /* VOF_FLAG_* */
-    int32_t jacobi_tblock;      /* sweeps fused per Jacobi launch (temporal blocking); 0 = auto */
+    int32_t jacobi_tblock;      /* Jacobi sweeps per launch, 1 .. 5 (temporal blocking); 0 = auto */
     int32_t reserved;
 } vof_params;
 
@@ -241,7 +241,9 @@ vof_status vof_time_steps(vof_solver* s, int64_t first_istep, int warmup, int st
 
 /* Launch-shape tuning (no reference counterpart).  kz_* = planes each wave
  * walks in the column kernels and the x sweep, fct_chunk = cells per lane in
- * the y/z FCT marches, jacobi_tblock = Jacobi sweeps fused per launch,
+ * the y/z FCT marches, jacobi_tblock = Jacobi sweeps fused per launch (1 .. 5:
+ * T >= 3 runs T sweeps per launch while T are left, then pairs, then one;
+ * another value is VOF_ERR_INVALID),
  * fct_zx_fused = FCT sweeps per launch where the z sweep is directly followed
  * by the x sweep (sweep orders y-z-x and z-x-y; 2 = one launch with kz_fct_x
  * planes per chunk, 1 = two launches), fct_xy_fused the same for the x and y
@@ -267,7 +269,8 @@ vof_status vof_autotune(vof_solver* s, int64_t first_istep, int steps_per_trial,
 /* Test hook: run one stage on the current state.  stage: 0 predictor,
  * 1 rhs, 2 one Jacobi sweep, 3 correction + velocity BC, 4/5/6 FCT sweep
  * x/y/z, 7 clamp + F ghost fill, 8 two Jacobi sweeps in one launch, 9 CSF
- * smoothing + curvature + face force (VOF_FLAG_CSF only).  Not part of the
+ * smoothing + curvature + face force (VOF_FLAG_CSF only), 10 jacobi_tblock
+ * Jacobi sweeps in one launch (jacobi_tblock 3 .. 5 only).  Not part of the
  * reference surface. */
 vof_status vof_debug_stage(vof_solver* s, int stage);
 

@@ -320,7 +320,57 @@ def test_jacobi2_lds_streamed_chunks(vofmod, orc, grid):
     _jacobi2_pair(vofmod, orc, 25, 4, *grid)
 
 
-@pytest.mark.parametrize("tblock", [1, 2])
+def _jacobiT_pair(vofmod, orc, T, kz, prec, nx, ny, nz, launches=2):
+    print(f"{T}-sweep jacobi kz={kz} prec={prec} grid={(nx, ny, nz)}", flush=True)
+    s, o = make_pair(vofmod, orc, nx=nx, ny=ny, nz=nz, precision=prec, dt=1e-3)
+    s.set_tuning(kz_jacobi=kz, jacobi_tblock=T)
+    rng = np.random.default_rng(SEED + 12)
+    st = random_state(s.shape, rng, s.dtype)
+    load_state(s, o, st)
+    s.debug_stage(vofmod.STAGE_PREDICT)
+    s.debug_stage(vofmod.STAGE_RHS)
+    for name in ("nu_rho", "advect_u", "advect_v", "advect_w"):
+        o.stage(name)
+    for it in range(launches):
+        s.debug_stage(vofmod.STAGE_JACOBI_T)
+        for _ in range(T):
+            o.stage("jacobi")
+        assert_same(s.download("P"), o.get("P"), f"P after {T}-sweep launch {it}")
+    s.close()
+    o.close()
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+@pytest.mark.parametrize("kz", [1, 3, 8, 25])
+@pytest.mark.parametrize("T", [3, 4, 5])
+def test_jacobiT_tiles_and_chunks(vofmod, orc, T, kz, prec):
+    """T sweeps per launch (T = 3, 4, 5) on a grid with several x tiles, row
+    bands and z chunks (partial last chunks; 25 is one chunk over all 21
+    planes), two launches in a row: bitwise equal to 2T single sweeps."""
+    _jacobiT_pair(vofmod, orc, T, kz, prec, 131, 33, 21)
+
+
+@pytest.mark.parametrize("grid", [(7, 3, 2), (1, 1, 1), (58, 12, 4), (3, 50, 9), (57, 11, 3)])
+@pytest.mark.parametrize("T", [3, 5])
+def test_jacobiT_small_grids(vofmod, orc, T, grid):
+    """T-sweep launches on grids thinner than the sweep halo: fewer planes than
+    sweeps, one cell, one exact tile (58 x 12 for T = 3), one row band of
+    many rows."""
+    _jacobiT_pair(vofmod, orc, T, 4, 4, *grid)
+
+
+def test_jacobi_tblock_range(vofmod):
+    s = vofmod.Solver(n=12)
+    with pytest.raises(vofmod.VofError, match="sweeps per Jacobi launch must be 1 .. 5"):
+        s.set_tuning(jacobi_tblock=6)
+    assert s.get_tuning()["jacobi_tblock"] != 6
+    s.set_tuning(jacobi_tblock=2)
+    with pytest.raises(vofmod.VofError, match="stage 10 needs jacobi_tblock 3"):
+        s.debug_stage(vofmod.STAGE_JACOBI_T)
+    s.close()
+
+
+@pytest.mark.parametrize("tblock", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("prec", [4, 8])
 def test_whole_steps_reference_case(vofmod, orc, prec, tblock):
     print(f"whole steps prec={prec} n=24 tblock={tblock}", flush=True)
@@ -434,7 +484,8 @@ def test_watchdog(vofmod):
 
 
 @pytest.mark.parametrize("tuning", [None, dict(jacobi_tblock=2, kz_jacobi=8, kz_correct=8, kz_rhs=8, kz_fct_x=8),
-                                    dict(jacobi_tblock=2, kz_jacobi=3, fct_chunk=16, kz_fct_x=16, kz_predict=4)])
+                                    dict(jacobi_tblock=2, kz_jacobi=3, fct_chunk=16, kz_fct_x=16, kz_predict=4),
+                                    dict(jacobi_tblock=4, kz_jacobi=5), dict(jacobi_tblock=3, kz_jacobi=8)])
 def test_batch_equals_plain(vofmod, tuning):
     """Instances of a ragged batch reproduce plain runs bit for bit, also when
     the batch runs other launch shapes (two-sweep Jacobi launches, preloaded or
@@ -459,7 +510,7 @@ def test_batch_equals_plain(vofmod, tuning):
         s.close()
 
 
-@pytest.mark.parametrize("tblock", [1, 2])
+@pytest.mark.parametrize("tblock", [1, 2, 5])
 def test_full_size_200_parity_three_steps(vofmod, orc, tblock):
     """Reference default case at full size: bitwise after one step of each rotation."""
     print(f"200^3 parity, 3 steps, tblock={tblock}", flush=True)
