@@ -79,6 +79,8 @@ def parse():
     ap.add_argument("--jacobi-tblock", type=int, default=0,
                     help="Jacobi sweeps per launch (1 .. 5) before the autotuner (use with --no-autotune to keep it)")
     ap.add_argument("--kz-jacobi", type=int, default=0, help="Jacobi z-chunk length, as --jacobi-tblock")
+    ap.add_argument("--kz-predict", type=int, default=0,
+                    help="predictor z-chunk length (4 or 8: bands of 4 rows per lane in FP32), as --jacobi-tblock")
     ap.add_argument("--case", choices=["dam", "ripple"], default="dam",
                     help="initial state: the dam break of the metric (default), or the liquid layer with a rippled "
                          "surface (mostly liquid: every FCT wave does the limiter arithmetic), for profiling")
@@ -491,8 +493,8 @@ def main():
     esz = 8 if prec == 8 else 4
     cells = n ** 3
     s = make_solver(vof, n, prec, a.case)
-    if a.jacobi_tblock or a.kz_jacobi:
-        s.set_tuning(jacobi_tblock=a.jacobi_tblock, kz_jacobi=a.kz_jacobi)
+    if a.jacobi_tblock or a.kz_jacobi or a.kz_predict:
+        s.set_tuning(jacobi_tblock=a.jacobi_tblock, kz_jacobi=a.kz_jacobi, kz_predict=a.kz_predict)
     if not a.no_autotune:
         # launch shapes chosen by timing candidates on this GPU (state restored)
         s.autotune(1, 4, a.tune_cache)

@@ -306,7 +306,17 @@ struct Ensemble : EnsBase {
 
     vof_status build_geos() {
         auto col1 = [](const vof::Inst<R>& I, int kz) { return vof::blocks_wcol(I.nx, I.ny, I.nz, kz, 1); };
-        VOF_TRY(make_geo(g_pred, tun.kz_predict, col1));
+        // the predictor's band mapping for chunks of 4 or 8 planes when every
+        // instance's ny is a multiple of the band height
+        bool band = sizeof(R) == 4 && (tun.kz_predict == 4 || tun.kz_predict == 8);
+        for (const auto& I : ih) band = band && (I.ny % vof::kPredictBandRows == 0);
+        if (band) {
+            VOF_TRY(make_geo(g_pred, tun.kz_predict, [](const vof::Inst<R>& I, int kz) {
+                return vof::blocks_band(I.nx, I.ny, I.nz, kz, vof::kPredictBandRows); }));
+        } else {
+            VOF_TRY(make_geo(g_pred, tun.kz_predict, col1));
+        }
+        g_pred.rb = band ? vof::kPredictBandRows : 0;
         VOF_TRY(make_geo(g_rhs, tun.kz_rhs, col1));
         VOF_TRY(make_geo(g_jac, tun.kz_jacobi, col1));
         VOF_TRY(make_geo(g_jac2, tun.kz_jacobi, [](const vof::Inst<R>& I, int kz) {

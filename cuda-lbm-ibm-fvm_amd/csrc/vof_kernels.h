@@ -13,6 +13,7 @@ struct Geo {
     int nblk = 0;
     int kz = 1;
     int min_nz = 0;  // fewest planes over the instances
+    int rb = 0;      // predictor: rows per lane of the band mapping (0: one row per lane)
 };
 
 template <class R>
@@ -105,6 +106,17 @@ inline int blocks_xy(int nx, int ny, int nz, int ch) {
     const int lines = (nx + 2) * nz;
     const int wpp = (lines + 57) / 58;
     return ((((wpp + 3) / 4) + 7) & ~7) * ((ny + ch - 1) / ch);
+}
+// Predictor with bands of rb rows per lane (k_predict_band): overlapped waves
+// of 62 stored lanes over the (nx+2) x ny/rb positions of band rows, per chunk
+// of kz planes.  Used for chunks of 4 or 8 planes when every instance's ny is a
+// multiple of rb (the bands then end at ny, and their upper neighbour row is
+// the ghost row).
+constexpr int kPredictBandRows = 4;
+inline int blocks_band(int nx, int ny, int nz, int kz, int rb) {
+    const int pos = (nx + 2) * ((ny + rb - 1) / rb);
+    const int wpp = (pos + 61) / 62;
+    return ((wpp + 3) / 4) * ((nz + kz - 1) / kz);
 }
 // k_predict_rho: one lane per interior cell, 256 per work-group.
 inline int blocks_cells(int nx, int ny, int nz) { return int((int64_t(nx) * ny * nz + 255) / 256); }

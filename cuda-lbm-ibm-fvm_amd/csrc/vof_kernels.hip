@@ -195,6 +195,122 @@ __global__ __launch_bounds__(256) void k_predict(const EnsArg<R> ea,
     for (; k <= m.k1; ++k, o += sy) plane();
 }
 
+// The same predictor with each lane holding a band of RB rows (j = 1 + J*RB +
+// r, r = 0..RB-1) of one column position: the y neighbours of the band's rows
+// are registers, and only the two rows bordering the band are extra loads
+// (3 + 6/RB loads of u, v, w per cell and plane instead of 9).  Waves overlap
+// along the flattened (i, J) plane of band rows as k_predict's do along (i, j),
+// storing the middle 62 lanes; a row's offset from the lane's band base is the
+// wave-uniform r*sx, added to the plane offset.  Each face is formed by the
+// same mom() with the same operands as k_predict, so the results are equal bit
+// for bit.
+template <class R, int RB, int KZ>
+__global__ __launch_bounds__(256) void k_predict_band(const EnsArg<R> ea, int fsel) {
+    VOF_PROLOGUE(R)
+    const int nx = I.nx, ny = I.ny, sx = I.sx, sy = I.sy;
+    const int nb = (ny + RB - 1) / RB;
+    const int P = sx * nb;  // positions of the (i, J) plane
+    const int wpp = (P + 61) / 62, bpp = (wpp + 3) >> 2;
+    const int zb = lb / bpp;
+    const int wv = (lb - zb * bpp) * 4 + ((int)threadIdx.x >> 6);
+    if (wv >= wpp) return;  // wave-uniform
+    const int lane = (int)threadIdx.x & 63;
+    const int t = wv * 62 - 1 + lane;
+    const int tc = min(max(t, 0), P - 1);
+    const int J = tc / sx, i = tc - J * sx;
+    const int j0 = 1 + J * RB;
+    const bool own = lane >= 1 && lane < 63 && t >= 0 && t < P && i >= 1 && i <= nx;
+    const int vo = j0 * sx + i;  // lane offset of row r = 0 within a plane
+    const int k0 = 1 + zb * KZ, k1 = min(I.nz, k0 + KZ - 1);
+    const Buf<R> F(sel2(I.F, fsel), I.ntot), u(I.u, I.ntot), v(I.v, I.ntot), w(I.w, I.ntot);
+    const Buf<R> us(I.us, I.ntot), vs(I.vs, I.ntot), ws(I.ws, I.ntot);
+    // rows q = 0 .. RB+1 are j0-1 .. j0+RB (row offset (q-1)*sx)
+    int o = k0 * sy;
+    R uk[RB + 2], vk[RB + 2], wk[RB + 2];  // plane k
+    R um[RB], vm[RB + 1], wm[RB];          // plane k-1 (v: one row more, v(j+1, k-1))
+    R nu[RB + 2], nv[RB + 2], nw[RB + 2];  // plane k+1, fetched one plane ahead
+    R nf[RB];                              // F of plane k, fetched one plane ahead
+#pragma unroll
+    for (int q = 0; q < RB + 2; ++q) {
+        const int so = o + (q - 1) * sx;
+        uk[q] = u.ld(vo, so); vk[q] = v.ld(vo, so); wk[q] = w.ld(vo, so);
+        nu[q] = u.ld(vo, so + sy); nv[q] = v.ld(vo, so + sy); nw[q] = w.ld(vo, so + sy);
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+        const int so = o - sy + r * sx;
+        um[r] = u.ld(vo, so); vm[r] = v.ld(vo, so); wm[r] = w.ld(vo, so);
+        nf[r] = F.ld(vo, o + r * sx);
+    }
+    vm[RB] = v.ld(vo, o - sy + RB * sx);
+    const int olim = I.nz * sy;
+    int k = k0;
+    auto plane = [&]() __attribute__((always_inline)) {
+        R up[RB + 2], vp[RB + 2], wp[RB + 2], f[RB];
+#pragma unroll
+        for (int q = 0; q < RB + 2; ++q) { up[q] = nu[q]; vp[q] = nv[q]; wp[q] = nw[q]; }
+#pragma unroll
+        for (int r = 0; r < RB; ++r) f[r] = nf[r];
+        {
+            const int o2 = min(o + sy, olim), o3 = o2 + sy;
+#pragma unroll
+            for (int q = 0; q < RB + 2; ++q) {
+                const int so = o3 + (q - 1) * sx;
+                nu[q] = u.ld(vo, so); nv[q] = v.ld(vo, so); nw[q] = w.ld(vo, so);
+            }
+#pragma unroll
+            for (int r = 0; r < RB; ++r) nf[r] = F.ld(vo, o2 + r * sx);
+        }
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+            const int q = r + 1;  // row j = j0 + r
+            const R u_k = uk[q], v_k = vk[q], w_k = wk[q];
+            const R ujm = uk[q - 1], ujp = uk[q + 1], vjm = vk[q - 1], vjp = vk[q + 1];
+            const R wjm = wk[q - 1], wjp = wk[q + 1];
+            const R u_km = um[r], v_km = vm[r], w_km = wm[r], vjp_km = vm[r + 1];
+            const R u_kp = up[q], v_kp = vp[q], w_kp = wp[q], wjm_n = wp[q - 1];
+            const R uup = sh_up(u_k), udn = sh_dn(u_k);
+            const R vup = sh_up(v_k), vdn = sh_dn(v_k);
+            const R wup_k = sh_up(w_k), wdn = sh_dn(w_k);
+            const R vjp_up = sh_up(vjp);   // v(i-1, j+1, k)
+            const R ujm_dn = sh_dn(ujm);   // u(i+1, j-1, k)
+            const R wup_kp = sh_up(w_kp);  // w(i-1, j, k+1)
+            const R udn_km = sh_dn(u_km);  // u(i+1, j, k-1)
+            const int j = j0 + r;
+            if (own && j <= ny) {
+                const R n0 = nu_of(I, f[r]);
+                if (i >= 2) {  // u* (:253-284)
+                    const R vh = R(0.25) * (vup + vjp_up + v_k + vjp);
+                    const R wh = R(0.25) * (wup_k + wup_kp + w_k + w_kp);
+                    us.st(mom(I, u_k, uup, udn, ujm, ujp, u_km, u_kp, u_k, vh, wh, n0, I.gx), vo, o + r * sx);
+                }
+                if (j >= 2) {  // v* (:310-341)
+                    const R uh = R(0.25) * (ujm + u_k + ujm_dn + udn);
+                    const R wh = R(0.25) * (wjm_n + wjm + w_k + w_kp);
+                    vs.st(mom(I, v_k, vup, vdn, vjm, vjp, v_km, v_kp, uh, v_k, wh, n0, I.gy), vo, o + r * sx);
+                }
+                if (k >= 2) {  // w* (:367-398)
+                    const R uh = R(0.25) * (udn_km + u_km + udn + u_k);
+                    const R vh = R(0.25) * (vjp_km + v_km + v_k + vjp);
+                    ws.st(mom(I, w_k, wup_k, wdn, wjm, wjp, w_km, w_kp, uh, vh, w_k, n0, I.gz), vo, o + r * sx);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RB; ++r) { um[r] = uk[r + 1]; wm[r] = wk[r + 1]; }
+#pragma unroll
+        for (int r = 0; r <= RB; ++r) vm[r] = vk[r + 1];
+#pragma unroll
+        for (int q = 0; q < RB + 2; ++q) { uk[q] = up[q]; vk[q] = vp[q]; wk[q] = wp[q]; }
+    };
+    if (k1 - k0 + 1 == KZ) {  // every chunk but a short last one
+#pragma unroll
+        for (int e = 0; e < KZ; ++e, ++k, o += sy) plane();
+        return;
+    }
+    for (; k <= k1; ++k, o += sy) plane();
+}
+
 // ---------------------------------------------------------------------------
 // Predictor with the density-weighted viscous term (VOF_FLAG_RHO_PREDICTOR;
 // extension: the reference's predictor takes rho and never reads it, :233,
@@ -2215,6 +2331,16 @@ bool launch_stream(int nread, int nwrite, const void* a, void* b, long n16, long
 template <class R>
 void launch_predict(const Ens<R>& e, const Geo& g, int fsel, hipStream_t st) {
     const dim3 grid(g.nblk), blk(256);
+    if (sizeof(R) == 4 && g.rb == kPredictBandRows) {  // bands of rows per lane (the launch shape was counted for them)
+        if (g.kz == 4) {
+            hipLaunchKernelGGL((k_predict_band<R, kPredictBandRows, 4>), grid, blk, 0, st, ens_arg(e, g), fsel);
+            return;
+        }
+        if (g.kz == 8) {
+            hipLaunchKernelGGL((k_predict_band<R, kPredictBandRows, 8>), grid, blk, 0, st, ens_arg(e, g), fsel);
+            return;
+        }
+    }
     switch (g.kz) {  // chunks of 4, 8 or 16 planes: unrolled plane loop
         case 4: hipLaunchKernelGGL((k_predict<R, 4>), grid, blk, 0, st, ens_arg(e, g), fsel, g.kz); break;
         case 8: hipLaunchKernelGGL((k_predict<R, 8>), grid, blk, 0, st, ens_arg(e, g), fsel, g.kz); break;

@@ -108,6 +108,29 @@ def test_stages(vofmod, orc, prec, grid):
     o.close()
 
 
+@pytest.mark.parametrize("kz", [4, 8, 16])
+@pytest.mark.parametrize("grid", [(70, 20, 9), (5, 4, 3), (131, 36, 17), (62, 8, 5), (61, 22, 10), (1, 4, 1)])
+def test_predictor_row_bands(vofmod, orc, grid, kz):
+    """FP32 predictor with bands of 4 rows per lane (chunks of 4 or 8 planes,
+    ny a multiple of 4: one band of 4 rows, several, positions wrapping from
+    one band row into the next, a one-cell-wide grid) and the one-row mapping
+    it falls back to (ny 22, chunks of 16), against the oracle's u*, v*, w*."""
+    nx, ny, nz = grid
+    print(f"predictor bands grid={grid} kz={kz}", flush=True)
+    s, o = make_pair(vofmod, orc, nx=nx, ny=ny, nz=nz, precision=4, dt=1e-3)
+    s.set_tuning(kz_predict=kz)
+    rng = np.random.default_rng(SEED + 31 + nx)
+    st = random_state(s.shape, rng, s.dtype)
+    load_state(s, o, st)
+    s.debug_stage(vofmod.STAGE_PREDICT)
+    for name in ("nu_rho", "advect_u", "advect_v", "advect_w"):
+        o.stage(name)
+    for f in ("USTAR", "VSTAR", "WSTAR"):
+        assert_same(s.download(f), o.get(f), f)
+    s.close()
+    o.close()
+
+
 @pytest.mark.parametrize("prec", [4, 8])
 def test_fct_each_direction_random(vofmod, orc, prec):
     """Each sweep direction alone on a random interface field with a large CFL."""
