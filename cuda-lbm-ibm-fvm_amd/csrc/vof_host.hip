@@ -319,8 +319,15 @@ struct Ensemble : EnsBase {
         g_pred.rb = band ? vof::kPredictBandRows : 0;
         VOF_TRY(make_geo(g_rhs, tun.kz_rhs, col1));
         VOF_TRY(make_geo(g_jac, tun.kz_jacobi, col1));
-        VOF_TRY(make_geo(g_jac2, tun.kz_jacobi, [](const vof::Inst<R>& I, int kz) {
-            return vof::blocks_jacobi2(I.nx, I.ny, I.nz, kz); }));
+        int min_nz = ih.empty() ? 0 : ih[0].nz;
+        for (const auto& I : ih) min_nz = std::min(min_nz, I.nz);
+        if (vof::jacobi2_streams_lds(tun.kz_jacobi, min_nz, int(sizeof(R)))) {
+            VOF_TRY(make_geo(g_jac2, tun.kz_jacobi, [](const vof::Inst<R>& I, int kz) {
+                return vof::blocks_jacobi2_flat(I.nx, I.ny, I.nz, kz); }));
+        } else {
+            VOF_TRY(make_geo(g_jac2, tun.kz_jacobi, [](const vof::Inst<R>& I, int kz) {
+                return vof::blocks_jacobi2(I.nx, I.ny, I.nz, kz); }));
+        }
         if (tun.jacobi_tblock < 1 || tun.jacobi_tblock > vof::kJacobiMaxT) {
             set_err("jacobi_tblock %d: sweeps per Jacobi launch must be 1 .. %d", tun.jacobi_tblock,
                     vof::kJacobiMaxT);
@@ -843,7 +850,10 @@ struct Ensemble : EnsBase {
             if (out) *out = tun;
             return VOF_OK;
         }
+        // Whole rotations of the sweep order, every trial at the same phase of
+        // it, so that each trial times the same mix of FCT launches.
         if (steps < 1) steps = 1;
+        steps = 3 * ((steps + 2) / 3);
         VOF_HIP(hipStreamSynchronize(st));
         void* snap = nullptr;
         if (hipMalloc(&snap, arena_bytes) != hipSuccess) { set_err("autotune: no memory for the state snapshot"); return VOF_ERR_ALLOC; }
@@ -858,7 +868,7 @@ struct Ensemble : EnsBase {
             if (s != VOF_OK) return s;
             double ms = 0.0;
             s = time_steps(is, 1, steps, false, &ms, stage);
-            is += steps + 1;
+            is += steps + 3;  // one warm-up step, rounded up to a whole rotation
             return s;
         };
         // column kernels and the x sweep: one z-chunk length for all, best per stage
@@ -882,7 +892,7 @@ struct Ensemble : EnsBase {
         }
         // Jacobi temporal-block depth with its own chunk lengths
         const int other_tb = (tun.jacobi_tblock >= 2) ? 1 : 2;
-        const int kzj[] = {2, 4, 8, 16, vof::kJ2DmaKz};
+        const int kzj[] = {2, 4, 8, 16, vof::kJ2DmaKzShort, vof::kJ2DmaKz};
         for (int c : kzj) {
             if (rc != VOF_OK) break;
             vof_tuning t = best;

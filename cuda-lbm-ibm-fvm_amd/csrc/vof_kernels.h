@@ -45,6 +45,9 @@ constexpr int kNumReduce = 7;
 // (k_jacobi2_dma); any other length uses the preloaded or register-streamed
 // kernels.
 constexpr int kJ2DmaKz = 25;
+// The same kernel with 20-plane chunks (at 200^3 two work-groups on nearly
+// every CU in one round).
+constexpr int kJ2DmaKzShort = 20;
 // Most Jacobi sweeps one launch performs (jacobi_tblock 1 .. kJacobiMaxT).
 constexpr int kJacobiMaxT = 5;
 // Largest k_reduce1 launch the allocation-free diagnostics serve: its
@@ -124,6 +127,17 @@ inline int blocks_cells(int nx, int ny, int nz) { return int((int64_t(nx) * ny *
 // (1024 lanes), one work-group row per chunk of kz planes.
 inline int blocks_jacobi2(int nx, int ny, int nz, int kz) {
     return ((nx + 59) / 60) * ((ny + 13) / 14) * ((nz + kz - 1) / kz);
+}
+// The LDS-streamed two-sweep kernel (FP32, chunks of kJ2DmaKz or
+// kJ2DmaKzShort planes on grids of at least that many planes) tiles the flattened plane of (i, row band)
+// positions instead: 60 stored positions per work-group, so x tiles need not
+// end at the row.
+inline bool jacobi2_streams_lds(int kz, int min_nz, int esz) {
+    return esz == 4 && (kz == kJ2DmaKz || kz == kJ2DmaKzShort) && min_nz >= kz;
+}
+inline int blocks_jacobi2_flat(int nx, int ny, int nz, int kz) {
+    const int pos = (nx + 2) * ((ny + 13) / 14);
+    return ((pos + 59) / 60) * ((nz + kz - 1) / kz);
 }
 // T-sweep Jacobi tiles (3 <= T <= kJacobiMaxT): 64 - 2T stored lanes x
 // 18 - 2T stored rows per work-group of 1024 lanes.

@@ -1050,14 +1050,21 @@ __global__ __launch_bounds__(1024) void k_jacobi2_dma(const EnsArg<float> ea, in
     __shared__ R B[NS][kJ2Rows][64];      // rhs
     __shared__ R Q[3][kJ2Rows][64];       // sweep 1
     const int nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
-    const int nxt = (nx + kJ2OutLanes - 1) / kJ2OutLanes;
+    // Lane l of tile xt holds position t = 60 xt - 2 + l of the flattened
+    // plane of (i, band) positions, t = band * sx + i, and wave w row
+    // j = 14 band + w of that band, so a tile may run from the end of one
+    // band's row into the next band's (lanes 2..61 store; the ghost columns
+    // i = 0, nx+1 between them are zero in sweep 1, as in the reference, and
+    // the x neighbours of every interior point lie in the same band row).
     const int nbd = (ny + kJ2OutRows - 1) / kJ2OutRows;
-    const int per_z = nxt * nbd;
-    const int zc = lb / per_z, rem = lb - zc * per_z;
-    const int bd = rem / nxt, xt = rem - bd * nxt;
+    const int npos = sx * nbd;
+    const int per_z = (npos + kJ2OutLanes - 1) / kJ2OutLanes;
+    const int zc = lb / per_z, xt = lb - zc * per_z;
     const int lane = (int)threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-    const int i = kJ2OutLanes * xt - 1 + lane;
+    const int t = min(max(kJ2OutLanes * xt - 2 + lane, 0), npos - 1);
+    const int bd = t / sx;
+    const int i = t - bd * sx;
     const int j = kJ2OutRows * bd + w;
     const int k0 = min(1 + zc * KZ, nz - KZ + 1);  // planes k0 .. k0+KZ-1 (the launch has nz >= KZ)
     const bool in1 = (i >= 1 && i <= nx && j >= 1 && j <= ny);
@@ -2389,8 +2396,9 @@ void launch_jacobi2(const Ens<R>& e, const Geo& g, int psrc, hipStream_t st) {
     // at 200^3) costs the registers of the second work-group per CU and ran
     // 20-35% slower.
     if constexpr (sizeof(R) == 4) {
-        if (g.kz == kJ2DmaKz && g.min_nz >= kJ2DmaKz) {  // streamed through LDS by the buffer unit
-            hipLaunchKernelGGL((k_jacobi2_dma<kJ2DmaKz>), grid, blk, 0, st, ens_arg(e, g), psrc);
+        if (jacobi2_streams_lds(g.kz, g.min_nz, 4)) {  // streamed through LDS by the buffer unit
+            if (g.kz == kJ2DmaKz) hipLaunchKernelGGL((k_jacobi2_dma<kJ2DmaKz>), grid, blk, 0, st, ens_arg(e, g), psrc);
+            else hipLaunchKernelGGL((k_jacobi2_dma<kJ2DmaKzShort>), grid, blk, 0, st, ens_arg(e, g), psrc);
             return;
         }
     }

@@ -260,7 +260,8 @@ vof_status vof_get_tuning(const vof_solver* s, vof_tuning* t);
 /* Autotuner (no reference counterpart): if cache_path holds an entry for
  * (GPU arch, grid, precision, instance count, sweeps) it is applied; otherwise
  * candidate z-chunk lengths, march chunk lengths and Jacobi temporal-block
- * depths are timed stage by stage over steps_per_trial steps each (state
+ * depths are timed stage by stage over steps_per_trial steps each, rounded up
+ * to whole rotations of the sweep order and at the same phase of it (state
  * snapshotted and restored on the device, so the run is unaffected), the
  * fastest per stage is applied and written to cache_path (JSON; NULL: no cache). */
 vof_status vof_autotune(vof_solver* s, int64_t first_istep, int steps_per_trial, const char* cache_path,

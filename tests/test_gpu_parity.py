@@ -323,6 +323,13 @@ def test_jacobi2_streamed_chunk_kinds(vofmod, orc, nz):
     _jacobi2_pair(vofmod, orc, 25, 4, nx=70, ny=30, nz=nz)
 
 
+@pytest.mark.parametrize("nz", [20, 21, 22, 41, 50])
+def test_jacobi2_streamed_short_chunks(vofmod, orc, nz):
+    """The same kernel with 20-plane chunks, each chunk kind, on a grid whose
+    flattened band positions wrap inside tiles (nx 61, ny 23)."""
+    _jacobi2_pair(vofmod, orc, 20, 4, nx=61, ny=23, nz=nz)
+
+
 @pytest.mark.parametrize("prec", [4, 8])
 @pytest.mark.parametrize("kz", [1, 2, 3, 4, 5, 8, 16, 25])
 def test_jacobi2_tiles_and_chunks(vofmod, orc, kz, prec):
