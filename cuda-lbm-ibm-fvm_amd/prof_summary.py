@@ -27,6 +27,8 @@ import sys
 # bench.py stage name of each step kernel (by the kernel's name up to its template arguments)
 STAGE_OF = {
     "vof::k_predict": "predictor",
+    "vof::k_predict_band": "predictor",
+    "vof::k_jacobiT": "jacobi",
     "vof::k_rhs_blk": "rhs",
     "vof::k_rhs": "rhs",
     "vof::k_jacobi2_dma": "jacobi",
