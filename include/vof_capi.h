@@ -240,7 +240,10 @@ vof_status vof_time_steps(vof_solver* s, int64_t first_istep, int warmup, int st
                           int use_graph, double* ms_total, double* stage_ms);
 
 /* Launch-shape tuning (no reference counterpart).  kz_* = planes each wave
- * walks in the column kernels and the x sweep, fct_chunk = cells per lane in
+ * walks in the column kernels and the x sweep (kz_predict 4 or 8 in FP32 with
+ * every instance's ny a multiple of 4: the predictor holds 4 rows per lane;
+ * kz_jacobi 20 or 25 in FP32 on grids of at least that many planes: the
+ * two-sweep Jacobi streams its planes through LDS), fct_chunk = cells per lane in
  * the y/z FCT marches, jacobi_tblock = Jacobi sweeps fused per launch (1 .. 5:
  * T >= 3 runs T sweeps per launch while T are left, then pairs, then one;
  * another value is VOF_ERR_INVALID),
