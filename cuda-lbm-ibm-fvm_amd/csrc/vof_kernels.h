@@ -41,12 +41,11 @@ inline EnsArg<R> ens_arg(const Ens<R>& e, const Geo& g) {
 }
 
 constexpr int kNumReduce = 7;
-// Chunk length of the FP32 two-sweep Jacobi streamed through LDS requests
-// (k_jacobi2_dma); any other length uses the preloaded or register-streamed
+// Chunk lengths of the FP32 two-sweep Jacobi streamed through LDS requests
+// (k_jacobi2_dma): 25 planes, or 20 (at 200^3 two work-groups on nearly every
+// CU in one round); any other length uses the preloaded or register-streamed
 // kernels.
 constexpr int kJ2DmaKz = 25;
-// The same kernel with 20-plane chunks (at 200^3 two work-groups on nearly
-// every CU in one round).
 constexpr int kJ2DmaKzShort = 20;
 // Most Jacobi sweeps one launch performs (jacobi_tblock 1 .. kJacobiMaxT).
 constexpr int kJacobiMaxT = 5;
