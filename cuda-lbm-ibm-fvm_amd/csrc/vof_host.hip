@@ -321,7 +321,7 @@ struct Ensemble : EnsBase {
         VOF_TRY(make_geo(g_jac, tun.kz_jacobi, col1));
         int min_nz = ih.empty() ? 0 : ih[0].nz;
         for (const auto& I : ih) min_nz = std::min(min_nz, I.nz);
-        if (vof::jacobi2_streams_lds(tun.kz_jacobi, min_nz, int(sizeof(R)))) {
+        if (vof::jacobi2_flat(tun.kz_jacobi, min_nz, int(sizeof(R)))) {
             VOF_TRY(make_geo(g_jac2, tun.kz_jacobi, [](const vof::Inst<R>& I, int kz) {
                 return vof::blocks_jacobi2_flat(I.nx, I.ny, I.nz, kz); }));
         } else {

@@ -134,6 +134,12 @@ inline int blocks_jacobi2(int nx, int ny, int nz, int kz) {
 inline bool jacobi2_streams_lds(int kz, int min_nz, int esz) {
     return esz == 4 && (kz == kJ2DmaKz || kz == kJ2DmaKzShort) && min_nz >= kz;
 }
+// The register-streamed two-sweep kernel (k_jacobi2: FP64, and FP32 chunks
+// other than 4, 8, 16) tiles the same flattened positions; the preloaded one
+// (k_jacobi2_blk: FP32 chunks of 4, 8, 16) keeps x tiles that end at the row.
+inline bool jacobi2_flat(int kz, int min_nz, int esz) {
+    return jacobi2_streams_lds(kz, min_nz, esz) || !(esz == 4 && (kz == 4 || kz == 8 || kz == 16));
+}
 inline int blocks_jacobi2_flat(int nx, int ny, int nz, int kz) {
     const int pos = (nx + 2) * ((ny + 13) / 14);
     return ((pos + 59) / 60) * ((nz + kz - 1) / kz);
