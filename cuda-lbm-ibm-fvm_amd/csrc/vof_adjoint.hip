@@ -729,8 +729,9 @@ __global__ __launch_bounds__(256) void k_jacobi_adj_col(const Inst<R>* __restric
 // launch leaves out and rhs_bar bit for bit as two single-sweep launches of
 // k_jacobi_adj_col (acc 2 then 0) do: every sum keeps that kernel's order.
 // Tile and march as the forward's k_jacobi2: a work-group of 16 waves covers
-// 16 rows x 64 lanes, wave w row j = 14*band + w, lane l column
-// i = 60*xtile - 1 + l; sweep a is formed on every lane of every row and
+// 16 rows x 64 lanes, lane l at position 60*xtile - 2 + l of the flattened
+// plane of (i, row band) positions, wave w at row j = 14*band + w of the
+// lane's band; sweep a is formed on every lane of every row and
 // shared across rows through a 3-plane LDS ring, sweep b is stored on lanes
 // 2..61 of rows 1..14, one plane behind sweep a; z neighbours of both sweeps
 // in registers, one plane of loads in flight.  Every wave reaches every
@@ -743,13 +744,16 @@ __global__ __launch_bounds__(1024) void k_jacobi2_adj(const Inst<R>* __restrict_
     const Inst<R> I = tab[0];
     __shared__ double ring[3][16][64];
     const int nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
-    const int nxt = (nx + 59) / 60, nbd = (ny + 13) / 14;
-    const int per_z = nxt * nbd;
+    // tiles over the flattened plane of (i, row band) positions, as the
+    // forward two-sweep kernels: lane l of tile xt holds position 60 xt - 2 + l
+    const int nbd = (ny + 13) / 14, npos = sx * nbd;
+    const int per_z = (npos + 59) / 60;
     const int b = int(blockIdx.x);
-    const int zc = b / per_z, rem = b - zc * per_z;
-    const int bd = rem / nxt, xt = rem - bd * nxt;
+    const int zc = b / per_z, xt = b - zc * per_z;
     const int lane = int(threadIdx.x) & 63, w = int(threadIdx.x) >> 6;
-    const int i = 60 * xt - 1 + lane, j = 14 * bd + w;
+    const int t = min(max(60 * xt - 2 + lane, 0), npos - 1);
+    const int bd = t / sx;
+    const int i = t - bd * sx, j = 14 * bd + w;
     const int k0 = 1 + zc * kz, k1 = min(nz, k0 + kz - 1);
     const bool in1 = (i >= 1 && i <= nx && j >= 1 && j <= ny);
     const bool st2 = in1 && lane >= 2 && lane <= 61 && w >= 1 && w <= 14;
@@ -1237,7 +1241,7 @@ void adj_jacobi2(const Inst<R>* tab, const Inst<R>& h, const double* in, double*
     // per CU (256); a chunk re-forms the first sweep on its two end planes, so
     // shorter chunks cost more (at 96^3: 4-plane chunks and 336 work-groups,
     // backward 132 ms; 2-plane chunks, 672 work-groups, 138 ms; 8-plane, 134)
-    const int per_z = ((h.nx + 59) / 60) * ((h.ny + 13) / 14);
+    const int per_z = ((h.nx + 2) * ((h.ny + 13) / 14) + 59) / 60;
     int kz = 16;
     while (kz > 2 && per_z * ((h.nz + kz - 1) / kz) < 256) kz /= 2;
     const R ax = h.axx, ay = h.ayy, az = h.azz, zf = (h.nz != 1) ? h.azz : R(0);
