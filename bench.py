@@ -84,10 +84,11 @@ def parse():
     ap.add_argument("--case", choices=["dam", "ripple"], default="dam",
                     help="initial state: the dam break of the metric (default), or the liquid layer with a rippled "
                          "surface (mostly liquid: every FCT wave does the limiter arithmetic), for profiling")
-    ap.add_argument("--study", choices=["precision", "long"], default=None,
+    ap.add_argument("--study", choices=["precision", "long", "adjoint-sizes"], default=None,
                     help="instead of the benchmark line, print the FP32-vs-FP64 study of the 200^3 dam break "
                          "(1000 steps) and the 384^3 ripple layer (500 steps); 'long': the 200^3 dam break "
-                         "over 20000 steps")
+                         "over 20000 steps; 'adjoint-sizes': reverse-mode cost over the plain step at 64^3 .. "
+                         "160^3")
     return ap.parse_args()
 
 
@@ -123,6 +124,40 @@ def load_traffic(kernel, tuning):
 
 
 from vof_cases import SEED, ensemble_config, field_errors, ripple_F  # noqa: E402
+
+
+def adjoint_sizes(vof, sizes=(64, 96, 128, 160), steps=100, every=20):
+    """Reverse-mode cost over the plain step across grid sizes: the dam break
+    from rest (FP32 forward, FP64 adjoint accumulation), `steps` steps with a
+    checkpoint every `every`; forward with checkpoints, backward, and the same
+    steps as plain graph replays, each timed on the host around a sync."""
+    out = {"workload": f"dam break from rest, FP32, {steps} steps, checkpoint every {every}", "sizes": {}}
+    for n in sizes:
+        s = vof.Solver(vof.default_params(n=n, precision=4, g=(0.0, 0.0, -9.81), dt=4e-6))
+        s.init_box(0.04, 0.04, 0.06)
+        s.adj_forward(1, steps, every)
+        s.adj_backward(1.0)  # warm-up
+        s.init_box(0.04, 0.04, 0.06)
+        s.sync()
+        t0 = time.perf_counter()
+        s.adj_forward(1, steps, every)
+        s.sync()
+        t1 = time.perf_counter()
+        s.adj_backward(1.0)
+        s.sync()
+        t2 = time.perf_counter()
+        s.init_box(0.04, 0.04, 0.06)
+        s.step(1, 10, graph=True)
+        s.sync()
+        t3 = time.perf_counter()
+        s.step(11, steps, graph=True)
+        s.sync()
+        t4 = time.perf_counter()
+        s.close()
+        out["sizes"][f"{n}^3"] = {"forward_ms": 1e3 * (t1 - t0), "backward_ms": 1e3 * (t2 - t1),
+                                  "plain_ms": 1e3 * (t4 - t3), "backward_over_plain": (t2 - t1) / (t4 - t3)}
+        print(f"adjoint sizes: {n}^3 backward / plain {(t2 - t1) / (t4 - t3):.2f}", file=sys.stderr, flush=True)
+    return out
 
 
 def make_solver(vof, n, prec, case="dam"):
@@ -486,6 +521,9 @@ def main():
     vof.load_library()
     if a.study in ("precision", "long"):
         print(json.dumps(precision_study(vof, long_run=(a.study == "long"))), flush=True)
+        return
+    if a.study == "adjoint-sizes":
+        print(json.dumps(adjoint_sizes(vof)), flush=True)
         return
     if a.calibrate_copy:
         vof.copy_bandwidth(a.calibrate_copy, 1)
