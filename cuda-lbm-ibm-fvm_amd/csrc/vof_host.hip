@@ -319,15 +319,8 @@ struct Ensemble : EnsBase {
         g_pred.rb = band ? vof::kPredictBandRows : 0;
         VOF_TRY(make_geo(g_rhs, tun.kz_rhs, col1));
         VOF_TRY(make_geo(g_jac, tun.kz_jacobi, col1));
-        int min_nz = ih.empty() ? 0 : ih[0].nz;
-        for (const auto& I : ih) min_nz = std::min(min_nz, I.nz);
-        if (vof::jacobi2_flat(tun.kz_jacobi, min_nz, int(sizeof(R)))) {
-            VOF_TRY(make_geo(g_jac2, tun.kz_jacobi, [](const vof::Inst<R>& I, int kz) {
-                return vof::blocks_jacobi2_flat(I.nx, I.ny, I.nz, kz); }));
-        } else {
-            VOF_TRY(make_geo(g_jac2, tun.kz_jacobi, [](const vof::Inst<R>& I, int kz) {
-                return vof::blocks_jacobi2(I.nx, I.ny, I.nz, kz); }));
-        }
+        VOF_TRY(make_geo(g_jac2, tun.kz_jacobi, [](const vof::Inst<R>& I, int kz) {
+            return vof::blocks_jacobi2_flat(I.nx, I.ny, I.nz, kz); }));
         if (tun.jacobi_tblock < 1 || tun.jacobi_tblock > vof::kJacobiMaxT) {
             set_err("jacobi_tblock %d: sweeps per Jacobi launch must be 1 .. %d", tun.jacobi_tblock,
                     vof::kJacobiMaxT);

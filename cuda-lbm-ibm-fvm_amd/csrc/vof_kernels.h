@@ -122,24 +122,15 @@ inline int blocks_band(int nx, int ny, int nz, int kz, int rb) {
 }
 // k_predict_rho: one lane per interior cell, 256 per work-group.
 inline int blocks_cells(int nx, int ny, int nz) { return int((int64_t(nx) * ny * nz + 255) / 256); }
-// Two-sweep Jacobi tiles: 60 stored lanes x 14 stored rows per work-group
-// (1024 lanes), one work-group row per chunk of kz planes.
-inline int blocks_jacobi2(int nx, int ny, int nz, int kz) {
-    return ((nx + 59) / 60) * ((ny + 13) / 14) * ((nz + kz - 1) / kz);
-}
-// The LDS-streamed two-sweep kernel (FP32, chunks of kJ2DmaKz or
-// kJ2DmaKzShort planes on grids of at least that many planes) tiles the flattened plane of (i, row band)
-// positions instead: 60 stored positions per work-group, so x tiles need not
-// end at the row.
+// The LDS-streamed two-sweep kernel: FP32, chunks of kJ2DmaKz or
+// kJ2DmaKzShort planes on grids of at least that many planes.
 inline bool jacobi2_streams_lds(int kz, int min_nz, int esz) {
     return esz == 4 && (kz == kJ2DmaKz || kz == kJ2DmaKzShort) && min_nz >= kz;
 }
-// The register-streamed two-sweep kernel (k_jacobi2: FP64, and FP32 chunks
-// other than 4, 8, 16) tiles the same flattened positions; the preloaded one
-// (k_jacobi2_blk: FP32 chunks of 4, 8, 16) keeps x tiles that end at the row.
-inline bool jacobi2_flat(int kz, int min_nz, int esz) {
-    return jacobi2_streams_lds(kz, min_nz, esz) || !(esz == 4 && (kz == 4 || kz == 8 || kz == 16));
-}
+// Two-sweep Jacobi tiles (every two-sweep kernel): 16 rows x 64 lanes per
+// work-group of 1024 lanes over the flattened plane of (i, row band)
+// positions, 60 stored positions x 14 stored rows each, so x tiles need not
+// end at the row; one work-group row per chunk of kz planes.
 inline int blocks_jacobi2_flat(int nx, int ny, int nz, int kz) {
     const int pos = (nx + 2) * ((ny + 13) / 14);
     return ((pos + 59) / 60) * ((nz + kz - 1) / kz);

@@ -839,13 +839,16 @@ __global__ __launch_bounds__(1024) void k_jacobi2_blk(const EnsArg<R> ea, int ps
     __shared__ R ring[5][kJ2Rows][64];
     __shared__ R prow[kRowsLds ? KZ + 2 : 1][kJ2Rows + 2][64];
     const int nx = I.nx, ny = I.ny, nz = I.nz, sx = I.sx, sy = I.sy;
-    const int nxt = (nx + kJ2OutLanes - 1) / kJ2OutLanes;
+    // tiles over the flattened plane of (i, row band) positions, as in
+    // k_jacobi2_dma: lane l of tile xt holds position 60 xt - 2 + l
     const int nbd = (ny + kJ2OutRows - 1) / kJ2OutRows;
-    const int per_z = nxt * nbd;
-    const int zc = lb / per_z, rem = lb - zc * per_z;
-    const int bd = rem / nxt, xt = rem - bd * nxt;
+    const int npos = sx * nbd;
+    const int per_z = (npos + kJ2OutLanes - 1) / kJ2OutLanes;
+    const int zc = lb / per_z, xt = lb - zc * per_z;
     const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
-    const int i = kJ2OutLanes * xt - 1 + lane;
+    const int t = min(max(kJ2OutLanes * xt - 2 + lane, 0), npos - 1);
+    const int bd = t / sx;
+    const int i = t - bd * sx;
     const int j = kJ2OutRows * bd + w;
     const int k0 = 1 + zc * KZ, k1 = min(nz, k0 + KZ - 1);
     const bool in1 = (i >= 1 && i <= nx && j >= 1 && j <= ny);
